@@ -1,0 +1,181 @@
+"""Throughput benchmark: moving-neighbourhood kriging on one MI355X.
+
+A step is one predict_many pass over the whole synthetic target grid, with
+data, search index and targets already resident in HBM (neighbour search +
+covariance build + solves).  Default workload: ok2d_large (n = 200,000 data
+points, Matern 3/2 with geometric anisotropy, k = 32, 2000x2000 = 4*10^6
+targets).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                  [--precision fp64|fp32r] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "geostat-rs_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+METRIC = ("kriged targets/s at k=32 (FP32-refined vs FP64, 1e-6 rel err); "
+          "p50/p99 req latency")
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and MFMA)
+HBM_PEAK_GBS = 8000.0
+
+
+def kriging_flops_per_target(n: int, m: int, c: int) -> float:
+    """Algorithmic flops per target: E*c + (2/3) N^3 + 2 N^2 + 2n + 2N + 1,
+    E = n(n+1)/2, N = n + m (LINPACK solve count for the bordered system)."""
+    N = n + m
+    E = n * (n + 1) / 2
+    return E * c + (2.0 / 3.0) * N ** 3 + 2 * N * N + 2 * n + 2 * N + 1
+
+
+# flops per covariance evaluation (+ - * / sqrt exp each 1), model matrix
+# precomputed: spherical iso 2-D 16, Matern 3/2 + 2-D anisotropy 23,
+# nugget + exponential + spherical 3-D rotated 60.
+COV_FLOPS = {"ok2d_small": 16, "ok2d_large": 23, "uk3d_octant": 60}
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(w, targets, budget_s: float = 10.0):
+    """The oracle (C restatement of the reference's rayon path) on host
+    cores over a strided sample of the same targets."""
+    from oracle import refcpu
+    import numpy as np
+    th = cpu_threads()
+    refcpu.lib()
+    t0 = time.perf_counter()
+    refcpu.predict_many(w.coords, w.values, w.model, w.config, targets[:1], threads=th)
+    build_s = time.perf_counter() - t0
+    probe = targets[:: max(1, len(targets) // 20000)]
+    t0 = time.perf_counter()
+    refcpu.predict_many(w.coords, w.values, w.model, w.config, probe, threads=th)
+    dt = max(time.perf_counter() - t0 - build_s, 1e-6)
+    rate = len(probe) / dt
+    n_s = int(min(len(targets), max(len(probe), rate * budget_s)))
+    sample = targets[:: max(1, len(targets) // n_s)]
+    t0 = time.perf_counter()
+    refcpu.predict_many(w.coords, w.values, w.model, w.config, sample, threads=th)
+    dt = max(time.perf_counter() - t0 - build_s, 1e-6)
+    return {"value": len(sample) / dt, "unit": "targets/s", "cores": th, "kind": "port",
+            "sample": f"{len(sample)} of {len(targets)} grid targets (strided), "
+                      f"oracle C restatement, {th} threads, kd-tree build excluded "
+                      f"({build_s:.2f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ok2d_large",
+                    choices=["ok2d_large", "ok2d_small", "uk3d_octant"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32r"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(targets shard trivially; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import geostat_hip as gh
+    from geostat_hip import workloads
+
+    dev = torch.device("cuda:0")
+    w = getattr(workloads, args.workload)()
+    targets = w.targets()
+    T = targets.shape[0]
+    k = gh.Kriging(w.coords, w.values, w.model, w.config)
+    D = targets.shape[1]
+    d_t = torch.from_numpy(targets).to(dev)
+    d_v = torch.empty(T, dtype=torch.float64, device=dev)
+    d_var = torch.empty(T, dtype=torch.float64, device=dev)
+    d_lag = torch.empty(T, dtype=torch.float64, device=dev)
+    d_st = torch.empty(T, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+
+    def step():
+        k.predict_many_device(d_t, d_v, d_var, d_lag, d_st, precision=args.precision, stream=sh)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    k.timing_summary()                 # drop warmup marks (timing off so far)
+    k.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    k.set_timing(False)
+    calls, ms_search, ms_solve = k.timing_summary()
+    st = d_st.cpu().numpy()
+    ok = int((st == 0).sum())
+
+    nb = k.max_neighbors
+    m = w.config.method
+    mdrift = 0 if m.kind == 0 else (1 if m.kind == 1 else (1 + D if m.degree == 1 else 1 + D + D * (D + 1) // 2))
+    if args.workload == "uk3d_octant":
+        # uneven systems: sum the per-target count over the size histogram
+        _, cnt = k.neighbors(targets)
+        hist = np.bincount(cnt[st == 0], minlength=65)
+        flops_launch = float(sum(hist[c] * kriging_flops_per_target(c, mdrift, COV_FLOPS[args.workload])
+                                 for c in range(len(hist)) if hist[c]))
+    else:
+        flops_launch = ok * kriging_flops_per_target(nb, mdrift, COV_FLOPS[args.workload])
+    solve_s = ms_solve / max(calls, 1) / 1e3
+    achieved = flops_launch / solve_s / 1e12 if solve_s > 0 else 0.0
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{args.workload}_{args.precision}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": T * args.steps / dt,
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded, reference bench recipe)",
+        "config": {"workload": args.workload, "n_data": int(w.coords.shape[0]),
+                   "max_neighbors": int(nb), "targets": int(T), "precision": args.precision,
+                   "ok_targets": ok},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "krige_fp64_kernel", "kernel_ms": solve_s * 1e3,
+                     "search_ms": ms_search / max(calls, 1),
+                     "flops_per_launch": flops_launch},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, targets)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,116 @@
+"""Builds libgeostat_hip.so (gfx950) and libgs_synth.so in-tree.
+
+The solve kernels are templates over (dimension, padded system size,
+drift terms); every instantiation gets its own generated translation unit
+so the fully unrolled kernels compile in parallel.  Output goes to
+geostat-rs_amd/build/ (git-ignored; the built .so files travel with the
+tree to the GPU host).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+BUILD = HERE / "build"
+GEN = BUILD / "gen"
+OBJ = BUILD / "obj"
+LIB = BUILD / "libgeostat_hip.so"
+SYNTH = BUILD / "libgs_synth.so"
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 (v_pk_*_f32) stays off for every kernel in this library
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+EXACT = ["-ffp-contract=off"]     # search + host: no a*b+c fusion (bit-exact distances)
+
+# (D, NB, M) instantiations of the kriging kernel: M = drift terms
+# (0 simple, 1 ordinary, 1+D universal deg 1, 1+D+D(D+1)/2 universal deg 2).
+KRIGE = [(d, nb, m) for d, ms in ((2, (0, 1, 3, 6)), (3, (0, 1, 4, 10)))
+         for nb in (16, 32, 64) for m in ms]
+VECCHIA = [(d, nb) for d in (2, 3) for nb in (16, 32, 64)]
+
+
+def _gen_sources() -> list[Path]:
+    GEN.mkdir(parents=True, exist_ok=True)
+    out = []
+    for d, nb, m in KRIGE:
+        p = GEN / f"krige_d{d}_nb{nb}_m{m}.hip"
+        txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_krige_inst<{d}, {nb}, {m}>"
+               f"(const SolveParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
+    for d, nb in VECCHIA:
+        p = GEN / f"vecchia_d{d}_nb{nb}.hip"
+        txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_vecchia_inst<{d}, {nb}>"
+               f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
+    return out
+
+
+def _compile(src: Path, extra: list[str]) -> Path:
+    obj = OBJ / (src.stem + ".o")
+    cmd = [HIPCC, *COMMON, *extra, "-c", str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(jobs: int | None = None, verbose: bool = True) -> Path:
+    """Compiles every HIP source for gfx950 and links the library."""
+    OBJ.mkdir(parents=True, exist_ok=True)
+    units: list[tuple[Path, list[str]]] = [
+        (CSRC / "gs_search.hip", EXACT),
+        (CSRC / "gs_solve.hip", []),
+        (CSRC / "gs_host.cpp", EXACT),
+    ]
+    units += [(p, []) for p in _gen_sources()]
+    jobs = jobs or max(1, min(16, os.cpu_count() or 4))
+    objs: list[Path] = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_compile, s, e): s for s, e in units}
+        for f in cf.as_completed(futs):
+            objs.append(f.result())
+            if verbose:
+                print(f"  built {futs[f].name}", flush=True)
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+           *sorted(map(str, objs)), "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    # synthetic-data generator (host C, no GPU code)
+    tmp = SYNTH.with_suffix(".so.tmp")
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11",
+           "-o", str(tmp), str(CSRC / "gs_synth.c"), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"synth build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, SYNTH)
+    return LIB
+
+
+def clean() -> None:
+    shutil.rmtree(BUILD, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    if "--clean" in sys.argv:
+        clean()
+    print(build())

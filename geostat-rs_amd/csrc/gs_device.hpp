@@ -1,0 +1,137 @@
+// Device-side covariance model for the gfx950 kernels.
+//
+// The host resolves a gs_model into a DevModel once per call: per-structure
+// inverse range, the anisotropy rotation+scaling matrix (computed on the
+// host with the same libm sin/cos as variogram/model.rs:353-367, 407-433,
+// instead of re-evaluating trig on every covariance as the reference does),
+// and the general-nu Matern normalisation.  Evaluation follows
+// VariogramModel::gamma_dh (model.rs:636-646): gamma(0) = 0 exactly, else
+// nugget + sum(sill * g(|T dh|, range)); entries are c0 - gamma.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gs {
+
+constexpr int kMaxStructures = 8;
+
+enum Kind : int32_t {
+    K_SPH = 0, K_EXP = 1, K_GAU = 2, K_MAT15 = 3, K_MAT25 = 4, K_MAT = 5,
+    K_CIR = 6, K_STA = 7, K_HOL = 8, K_WAV = 9, K_POW = 10
+};
+
+struct DevStructure {
+    int32_t kind;
+    int32_t aniso;        // 0 = isotropic, 1 = 2x2 / 3x3 matrix in m
+    double param;
+    double sill;
+    double inv_range;
+    double m[9];          // row-major transform: 2-D uses m[0..3]
+    double mat_norm;      // 2^(1-nu) / Gamma(nu)   (general Matern)
+    double mat_s;         // sqrt(2 nu)
+};
+
+// Gauss-Legendre nodes for the general-nu Matern K_nu quadrature
+// (variogram/bessel.rs:60-95): 60 nodes on [0,1] and 40 on [1,T].
+struct BesselNodes {
+    double x60[60], w60[60], x40[40], w40[40];
+};
+
+struct DevModel {
+    double nugget;
+    double c0;            // total sill, VariogramModel::total_sill (model.rs:658-660)
+    int32_t ns;
+    int32_t pad;
+    const BesselNodes *nodes;   // device copy, used by general-nu Matern only
+    DevStructure s[kMaxStructures];
+};
+
+#if defined(__HIPCC__)
+__device__ __forceinline__ double dev_bessel_k(const BesselNodes &gl, double nu, double x) {
+    // K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt (bessel.rs:70-95).
+    double integral = 0.0;
+    for (int i = 0; i < 60; i++) {
+        double t = 0.5 * (gl.x60[i] + 1.0);
+        integral += gl.w60[i] * (exp(-x * cosh(t)) * cosh(nu * t));
+    }
+    integral *= 0.5;
+    double xm = x > 1e-12 ? x : 1e-12;
+    double a = 60.0 / xm;
+    double tm = a < 1.0 ? __builtin_nan("") : log(a + sqrt(a - 1.0) * sqrt(a + 1.0));
+    double t_max = fmax(tm, 4.0) + 1.0;
+    double integral2 = 0.0;
+    for (int i = 0; i < 40; i++) {
+        double t = 1.0 + 0.5 * (t_max - 1.0) * (gl.x40[i] + 1.0);
+        integral2 += gl.w40[i] * (exp(-x * cosh(t)) * cosh(nu * t));
+    }
+    integral2 *= 0.5 * (t_max - 1.0);
+    return integral + integral2;
+}
+
+// Unit-sill shape g(h, range) (model.rs:122-181) in double precision.
+__device__ __forceinline__ double shape_g(const DevStructure &s, const BesselNodes *gl, double h) {
+    double d = h * s.inv_range;
+    switch (s.kind) {
+    case K_SPH: return d >= 1.0 ? 1.0 : 1.5 * d - 0.5 * d * d * d;
+    case K_EXP: return 1.0 - exp(-d);
+    case K_GAU: return 1.0 - exp(-(d * d));
+    case K_MAT15: { double q = 1.7320508075688772 * d; return 1.0 - (1.0 + q) * exp(-q); }
+    case K_MAT25: { double q = 2.23606797749979 * d; return 1.0 - (1.0 + q + q * q / 3.0) * exp(-q); }
+    case K_MAT: {
+        if (d <= 0.0) return 0.0;
+        double q = s.mat_s * d;
+        double corr = s.mat_norm * pow(q, s.param) * dev_bessel_k(*gl, s.param, q);
+        corr = corr < 0.0 ? 0.0 : (corr > 1.0 ? 1.0 : corr);
+        return 1.0 - corr;
+    }
+    case K_CIR: {
+        double dd = fmin(d, 1.0);
+        return 1.0 - 0.63661977236758134 * (acos(dd) - dd * sqrt(1.0 - dd * dd));
+    }
+    case K_STA: return 1.0 - exp(-pow(d, s.param));
+    case K_HOL: return d <= 0.0 ? 0.0 : 1.0 - sin(d) / d;
+    case K_WAV: {
+        if (d <= 0.0) return 0.0;
+        double pd = 3.141592653589793 * d;
+        return 1.0 - sin(pd) / pd;
+    }
+    default: return __builtin_nan("");
+    }
+}
+
+// Effective isotropic lag |T dh| (model.rs:521-539).
+template <int D>
+__device__ __forceinline__ double eff_h(const DevStructure &s, const double *dh) {
+    double acc = 0.0;
+    if (!s.aniso) {
+#pragma unroll
+        for (int d = 0; d < D; d++) acc += dh[d] * dh[d];
+    } else if (D == 2) {
+        double e0 = s.m[0] * dh[0] + s.m[1] * dh[1];
+        double e1 = s.m[2] * dh[0] + s.m[3] * dh[1];
+        acc = e0 * e0 + e1 * e1;
+    } else {
+        double e0 = s.m[0] * dh[0] + s.m[1] * dh[1] + s.m[2] * dh[2];
+        double e1 = s.m[3] * dh[0] + s.m[4] * dh[1] + s.m[5] * dh[2];
+        double e2 = s.m[6] * dh[0] + s.m[7] * dh[1] + s.m[8] * dh[2];
+        acc = e0 * e0 + e1 * e1 + e2 * e2;
+    }
+    return sqrt(acc);
+}
+
+// Covariance c0 - gamma(dh) for a non-zero separation (callers handle the
+// diagonal, where gamma(0) = 0 gives exactly c0).
+template <int D>
+__device__ __forceinline__ double cov_offdiag(const DevModel &M, const double *dh) {
+    bool allzero = true;
+#pragma unroll
+    for (int d = 0; d < D; d++) allzero = allzero && (dh[d] == 0.0);
+    if (allzero) return M.c0;
+    double gam = 0.0;
+    for (int i = 0; i < M.ns; i++) gam += M.s[i].sill * shape_g(M.s[i], M.nodes, eff_h<D>(M.s[i], dh));
+    return M.c0 - (M.nugget + gam);
+}
+#endif
+
+}  // namespace gs

@@ -1,0 +1,1284 @@
+// Host runtime and C ABI of libgeostat_hip.so.
+//
+// Mirrors the reference's build-time contract (Kriging::build,
+// kriging.rs:262-408; VariogramModel::new, model.rs:554-618; PointSet::new,
+// data.rs:57-77; vecchia_plan, vecchia.rs:232-282) on the host, keeps data
+// and search index resident in HBM, and drives the gfx950 kernels.  Index
+// construction (kd-tree layout, Vecchia ordering and conditioning sets) runs
+// here once per handle, as Kriging::new / vecchia_plan do in the reference;
+// every per-target / per-point computation runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/geostat_hip.h"
+#include "gs_internal.hpp"
+
+using gs::DevModel;
+
+namespace {
+
+thread_local std::string g_err;
+
+int32_t fail(int32_t code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? GS_ERR_OUT_OF_MEMORY : GS_ERR_DEVICE, \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+// Device buffer owned by a handle; grows, never shrinks.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t b) {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipMalloc(&p, b ? b : 8);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+inline double to_radians(double deg) { return deg * (M_PI / 180.0); }
+
+// total_cmp key (IEEE totalOrder) for f64.
+inline int64_t total_key(double x) {
+    int64_t b;
+    std::memcpy(&b, &x, 8);
+    return b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1);
+}
+
+bool aniso_valid(const gs_anisotropy &a) {   // model.rs:376-400
+    return a.ratio_z > 0.0 && std::isfinite(a.ratio_z) && a.ratio > 0.0 && std::isfinite(a.ratio) &&
+           std::isfinite(a.azimuth_deg) && std::isfinite(a.dip_deg) && std::isfinite(a.rake_deg);
+}
+
+// GSLIB setrot matrix (model.rs:407-433), row-major.
+void rotation_3d(const gs_anisotropy &a, double r[9]) {
+    double alpha = (a.azimuth_deg >= 0.0 && a.azimuth_deg < 270.0) ? to_radians(90.0 - a.azimuth_deg)
+                                                                   : to_radians(450.0 - a.azimuth_deg);
+    double beta = to_radians(-a.dip_deg);
+    double theta = to_radians(a.rake_deg);
+    double sina = std::sin(alpha), cosa = std::cos(alpha);
+    double sinb = std::sin(beta), cosb = std::cos(beta);
+    double sint = std::sin(theta), cost = std::cos(theta);
+    double afac1 = 1.0 / a.ratio, afac2 = 1.0 / a.ratio_z;
+    r[0] = cosb * cosa;
+    r[1] = cosb * sina;
+    r[2] = -sinb;
+    r[3] = afac1 * (-cost * sina + sint * sinb * cosa);
+    r[4] = afac1 * (cost * cosa + sint * sinb * sina);
+    r[5] = afac1 * (sint * cosb);
+    r[6] = afac2 * (sint * sina + cost * sinb * cosa);
+    r[7] = afac2 * (-sint * cosa + cost * sinb * sina);
+    r[8] = afac2 * (cost * cosb);
+}
+
+// search_point (kriging.rs:16-21) in the exact reference expression order.
+void search_xf_host(const gs_anisotropy *a, int D, const double *p, double *out) {
+    if (!a) {
+        for (int d = 0; d < D; d++) out[d] = p[d];
+        return;
+    }
+    if (D == 3) {
+        double r[9];
+        rotation_3d(*a, r);
+        out[0] = r[0] * p[0] + r[1] * p[1] + r[2] * p[2];
+        out[1] = r[3] * p[0] + r[4] * p[1] + r[5] * p[2];
+        out[2] = r[6] * p[0] + r[7] * p[1] + r[8] * p[2];
+    } else {
+        double ang = to_radians(a->azimuth_deg);
+        double s = std::sin(ang), c = std::cos(ang);
+        out[0] = p[0] * s + p[1] * c;
+        out[1] = (p[0] * c - p[1] * s) / a->ratio;
+    }
+}
+
+gs::SearchXf make_search_xf(const gs_kconfig &cfg, int D) {
+    gs::SearchXf X;
+    std::memset(&X, 0, sizeof(X));
+    if (!cfg.has_anisotropic_search) return X;
+    X.on = 1;
+    const gs_anisotropy &a = cfg.anisotropic_search;
+    double ang = to_radians(a.azimuth_deg);
+    X.s = std::sin(ang);
+    X.c = std::cos(ang);
+    X.ratio = a.ratio;
+    if (D == 3) rotation_3d(a, X.r);
+    return X;
+}
+
+// Lanczos Gamma (bessel.rs:100-123), for the general-nu Matern norm.
+double lanczos_gamma(double x) {
+    static const double C[9] = {0.9999999999998099, 676.5203681218851, -1259.1392167224028,
+                                771.3234287776531, -176.6150291621406, 12.507343278686905,
+                                -0.13857109526572012, 9.984369578019572e-6, 1.505632735149312e-7};
+    if (x < 0.5) return M_PI / (std::sin(M_PI * x) * lanczos_gamma(1.0 - x));
+    double xm1 = x - 1.0, a = C[0], t = xm1 + 7.5;
+    for (int i = 1; i < 9; i++) a += C[i] / (xm1 + (double)i);
+    return std::sqrt(2.0 * M_PI) * std::pow(t, xm1 + 0.5) * std::exp(-t) * a;
+}
+
+// Gauss-Legendre nodes by Newton on P_n (bessel.rs:29-57).
+void gauss_legendre(int n, double *x, double *w) {
+    int m = (n + 1) / 2;
+    for (int i = 0; i < m; i++) {
+        double z = std::cos((M_PI * ((double)i + 0.75)) / ((double)n + 0.5));
+        for (;;) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 0; j < n; j++) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * (double)j + 1.0) * z * p2 - (double)j * p3) / ((double)j + 1.0);
+            }
+            double pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+            double z1 = z;
+            z -= p1 / pp;
+            if (std::fabs(z - z1) <= 1e-15) {
+                w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+                x[i] = -z;
+                w[n - 1 - i] = w[i];
+                x[n - 1 - i] = z;
+                break;
+            }
+        }
+    }
+}
+
+double total_sill(const gs_model &m) {   // model.rs:658-660
+    double s = -0.0;
+    for (int i = 0; i < m.n_structures; i++) s += m.structures[i].sill;
+    return m.nugget + s;
+}
+
+// VariogramModel::new validation (model.rs:554-618) plus dimension validity
+// (model.rs:104-106) and the GPU's supported set.
+int32_t validate_model(const gs_model *m, int D, bool allow_power) {
+    if (!m) return fail(GS_ERR_INVALID_PARAMETER, "model is NULL");
+    if (!(m->nugget >= 0.0) || !std::isfinite(m->nugget))
+        return fail(GS_ERR_INVALID_PARAMETER, "nugget must be finite and >= 0");
+    if (m->n_structures < 0 || m->n_structures > GS_MAX_STRUCTURES)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: more than 8 nested structures");
+    for (int i = 0; i < m->n_structures; i++) {
+        const gs_structure &s = m->structures[i];
+        if (s.kind < GS_SPHERICAL || s.kind > GS_POWER)
+            return fail(GS_ERR_INVALID_PARAMETER, "unknown model kind");
+        if (!(s.sill >= 0.0) || !std::isfinite(s.sill))
+            return fail(GS_ERR_INVALID_PARAMETER, "partial sill must be finite and >= 0");
+        if (!(s.range > 0.0) || !std::isfinite(s.range))
+            return fail(GS_ERR_INVALID_PARAMETER, "range must be finite and > 0");
+        if (s.kind == GS_MATERN && (!(s.param > 0.0) || s.param > 15.0))
+            return fail(GS_ERR_INVALID_PARAMETER, "Matern nu must be finite, > 0 and <= 15");
+        if (s.kind == GS_STABLE && (!(s.param > 0.0) || s.param > 2.0))
+            return fail(GS_ERR_INVALID_PARAMETER, "Stable alpha must be in (0, 2]");
+        if (s.kind == GS_POWER && (!(s.param > 0.0) || s.param >= 2.0))
+            return fail(GS_ERR_INVALID_PARAMETER, "Power theta must be in (0, 2)");
+        if (s.has_anis && !aniso_valid(s.anis))
+            return fail(GS_ERR_INVALID_PARAMETER, "invalid anisotropy");
+    }
+    if (!(total_sill(*m) > 0.0)) return fail(GS_ERR_INVALID_PARAMETER, "total sill must be positive");
+    for (int i = 0; i < m->n_structures; i++) {
+        if (m->structures[i].kind == GS_CIRCULAR && D > 2)
+            return fail(GS_ERR_INVALID_PARAMETER, "Circular is not a valid covariance in 3 dimensions");
+        if (m->structures[i].kind == GS_POWER && !allow_power)
+            return fail(GS_ERR_INVALID_PARAMETER,
+                        "unsupported on GPU: Power (semivariogram-form) systems");
+    }
+    return GS_OK;
+}
+
+DevModel resolve_model(const gs_model &m, int D, const gs::BesselNodes *d_nodes) {
+    DevModel M;
+    std::memset(&M, 0, sizeof(M));
+    M.nugget = m.nugget;
+    M.c0 = total_sill(m);
+    M.ns = m.n_structures;
+    M.nodes = d_nodes;
+    for (int i = 0; i < m.n_structures; i++) {
+        const gs_structure &s = m.structures[i];
+        gs::DevStructure &o = M.s[i];
+        o.kind = s.kind;
+        o.param = s.param;
+        o.sill = s.sill;
+        o.inv_range = 1.0 / s.range;
+        o.aniso = s.has_anis ? 1 : 0;
+        if (s.has_anis) {
+            if (D == 2) {
+                double ang = to_radians(s.anis.azimuth_deg);
+                double sn = std::sin(ang), cs = std::cos(ang);
+                o.m[0] = sn;
+                o.m[1] = cs;
+                o.m[2] = cs / s.anis.ratio;
+                o.m[3] = -sn / s.anis.ratio;
+            } else {
+                rotation_3d(s.anis, o.m);
+            }
+        }
+        if (s.kind == GS_MATERN) {
+            o.mat_norm = std::pow(2.0, 1.0 - s.param) / lanczos_gamma(s.param);
+            o.mat_s = std::sqrt(2.0 * s.param);
+        }
+    }
+    return M;
+}
+
+// First duplicate location (data.rs:150-163): keys are (c + 0.0).to_bits().
+bool duplicate_pair(int D, const double *coords, int64_t n, int64_t pair[2]) {
+    struct KeyHash {
+        size_t operator()(const std::array<uint64_t, 3> &k) const {
+            uint64_t h = 1469598103934665603ULL;
+            for (uint64_t v : k) h = (h ^ v) * 1099511628211ULL ^ (v >> 29);
+            return (size_t)h;
+        }
+    };
+    std::unordered_map<std::array<uint64_t, 3>, int64_t, KeyHash> seen;
+    seen.reserve((size_t)n * 2);
+    for (int64_t i = 0; i < n; i++) {
+        std::array<uint64_t, 3> key{0, 0, 0};
+        for (int d = 0; d < D; d++) {
+            double c = coords[i * D + d] + 0.0;
+            std::memcpy(&key[d], &c, 8);
+        }
+        auto it = seen.find(key);
+        if (it != seen.end()) {
+            pair[0] = it->second;
+            pair[1] = i;
+            return true;
+        }
+        seen.emplace(key, i);
+    }
+    return false;
+}
+
+// Median-split kd-tree layout (search.rs:47-54, 102-113).  With distinct
+// axis values the layout is unique whatever selection algorithm produced it;
+// equal axis values are ordered by index.
+void kd_layout(int D, const double *tc, int64_t n, std::vector<uint32_t> &idx) {
+    idx.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) idx[(size_t)i] = (uint32_t)i;
+    struct Job { int64_t lo, hi; int axis; };
+    std::vector<Job> stack;
+    stack.push_back({0, n, 0});
+    while (!stack.empty()) {
+        Job j = stack.back();
+        stack.pop_back();
+        int64_t len = j.hi - j.lo;
+        if (len <= 1) continue;
+        int64_t mid = len / 2;
+        int axis = j.axis;
+        auto less = [&](uint32_t a, uint32_t b) {
+            int64_t ka = total_key(tc[(int64_t)a * D + axis]), kb = total_key(tc[(int64_t)b * D + axis]);
+            return ka < kb || (ka == kb && a < b);
+        };
+        std::nth_element(idx.begin() + j.lo, idx.begin() + j.lo + mid, idx.begin() + j.hi, less);
+        int na = (axis + 1) % D;
+        stack.push_back({j.lo, j.lo + mid, na});
+        stack.push_back({j.lo + mid + 1, j.hi, na});
+    }
+}
+
+int tree_depth(int64_t n) {
+    int d = 0;
+    while (n > 0) { n >>= 1; d++; }
+    return d;
+}
+
+int pad_nb(int64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
+
+}  // namespace
+
+struct gs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DBuf nodes;   // gs::BesselNodes
+};
+
+struct gs_kriging {
+    gs_ctx *ctx = nullptr;
+    int D = 2;
+    int64_t n = 0;
+    gs_kconfig cfg{};
+    gs_model model{};
+    int M = 1;
+    bool has_tree = false;
+    int64_t k_heap = 0;
+    int per_oct = 0;
+    int64_t nb_bound = 0;   // padded system bound (<= 64)
+    int64_t stride = 0;     // neighbour-list capacity per target
+    int32_t min_nb = 1;
+    int depth = 0;
+    double center[3] = {0, 0, 0};
+    double scale = 1.0;
+    double r2 = std::numeric_limits<double>::infinity();
+    gs::SearchXf xf{};
+    DBuf coords, values, tcoords, tidx, dmodel;
+    // per-call workspace (grows)
+    DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
+    std::mutex mu;
+    // measurement hook: (search start, search end / solve start, solve end)
+    bool timing = false;
+    std::vector<std::array<hipEvent_t, 3>> marks;
+    std::vector<hipEvent_t> event_pool;
+    ~gs_kriging() {
+        for (auto &m : marks) event_pool.insert(event_pool.end(), m.begin(), m.end());
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    }
+    hipError_t take_event(hipEvent_t *e) {
+        if (!event_pool.empty()) {
+            *e = event_pool.back();
+            event_pool.pop_back();
+            return hipSuccess;
+        }
+        return hipEventCreate(e);
+    }
+};
+
+struct gs_vplan {
+    gs_ctx *ctx = nullptr;
+    int D = 2;
+    int64_t n = 0, m = 0;
+    std::vector<int64_t> order;
+    std::vector<int32_t> nb;
+    std::vector<int32_t> nb_count;
+    int32_t max_count = 0;
+    DBuf coords, d_order, d_nb, d_count, zc, models, partial, out, status;
+    std::mutex mu;
+};
+
+extern "C" {
+
+const char *gs_last_error(void) { return g_err.c_str(); }
+int32_t gs_abi_version(void) { return GS_ABI_VERSION; }
+
+int32_t gs_ctx_create(int32_t device, gs_ctx **out) {
+    if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(GS_ERR_INVALID_PARAMETER, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    gs_ctx *c = new gs_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(GS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    gs::BesselNodes nodes;
+    gauss_legendre(60, nodes.x60, nodes.w60);
+    gauss_legendre(40, nodes.x40, nodes.w40);
+    if ((e = c->nodes.ensure(sizeof(nodes))) != hipSuccess ||
+        (e = hipMemcpy(c->nodes.p, &nodes, sizeof(nodes), hipMemcpyHostToDevice)) != hipSuccess) {
+        gs_ctx_destroy(c);
+        return fail(GS_ERR_DEVICE, std::string("bessel nodes: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return GS_OK;
+}
+
+void gs_ctx_destroy(gs_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int32_t gs_ctx_synchronize(gs_ctx *ctx) {
+    if (!ctx) return fail(GS_ERR_INVALID_PARAMETER, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GS_OK;
+}
+
+int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                       int64_t n, const gs_model *model, const gs_kconfig *config, gs_kriging **out,
+                       int64_t dup_pair[2]) {
+    if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
+    *out = nullptr;
+    if (!ctx || !config) return fail(GS_ERR_INVALID_PARAMETER, "ctx/config is NULL");
+    if (dim != 2 && dim != 3) return fail(GS_ERR_DIMENSION_MISMATCH, "dim must be 2 or 3");
+    // PointSet::new (data.rs:57-77)
+    if (n <= 0) return fail(GS_ERR_INSUFFICIENT_DATA, "no points provided");
+    if (!coords || !values) return fail(GS_ERR_INVALID_PARAMETER, "coords/values is NULL");
+    for (int64_t i = 0; i < n * dim; i++)
+        if (!std::isfinite(coords[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite coordinate");
+    for (int64_t i = 0; i < n; i++)
+        if (!std::isfinite(values[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite value");
+    int32_t st = validate_model(model, dim, true);
+    if (st != GS_OK) return st;
+    const gs_kconfig &cfg = *config;
+    // Kriging::build (kriging.rs:262-339), in the reference's check order.
+    if (cfg.method == GS_UNIVERSAL && !(cfg.degree == 1 || cfg.degree == 2))
+        return fail(GS_ERR_INVALID_PARAMETER, "universal kriging drift degree must be 1 or 2");
+    if (cfg.method < GS_SIMPLE || cfg.method > GS_UNIVERSAL)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: kriging method");
+    bool power = false;
+    for (int i = 0; i < model->n_structures; i++) power |= model->structures[i].kind == GS_POWER;
+    if (power && cfg.method == GS_SIMPLE)
+        return fail(GS_ERR_INVALID_PARAMETER, "Power models cannot be used with simple kriging");
+    if (cfg.has_search_radius && !(cfg.search_radius > 0.0))
+        return fail(GS_ERR_INVALID_PARAMETER, "search radius must be positive");
+    if (cfg.has_anisotropic_search && !aniso_valid(cfg.anisotropic_search))
+        return fail(GS_ERR_INVALID_PARAMETER, "invalid anisotropic_search");
+    if (cfg.max_neighbors == 0) return fail(GS_ERR_INVALID_PARAMETER, "max_neighbors must be at least 1");
+    int64_t pair[2];
+    if (duplicate_pair(dim, coords, n, pair)) {
+        if (dup_pair) { dup_pair[0] = pair[0]; dup_pair[1] = pair[1]; }
+        return fail(GS_ERR_DUPLICATE_POINTS, "duplicate points " + std::to_string(pair[0]) + ", " +
+                                                 std::to_string(pair[1]));
+    }
+    if (cfg.max_per_octant == 0) return fail(GS_ERR_INVALID_PARAMETER, "max_per_octant must be at least 1");
+    if (cfg.min_neighbors != GS_NONE) {
+        if (cfg.min_neighbors == 0) return fail(GS_ERR_INVALID_PARAMETER, "min_neighbors must be at least 1");
+        if (cfg.min_neighbors > n)
+            return fail(GS_ERR_INSUFFICIENT_DATA, "min_neighbors exceeds the number of data points");
+        if (cfg.max_neighbors != GS_NONE && cfg.min_neighbors > cfg.max_neighbors)
+            return fail(GS_ERR_INVALID_PARAMETER, "min_neighbors exceeds max_neighbors");
+    }
+    if (power) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Power (semivariogram-form) systems");
+    if (n > (int64_t)(1u << 29)) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: more than 2^29 points");
+
+    std::unique_ptr<gs_kriging> K(new gs_kriging());
+    K->ctx = ctx;
+    K->D = dim;
+    K->n = n;
+    K->cfg = cfg;
+    K->model = *model;
+    K->M = cfg.method == GS_SIMPLE ? 0
+         : cfg.method == GS_ORDINARY ? 1
+         : (cfg.degree == 1 ? 1 + dim : 1 + dim + dim * (dim + 1) / 2);   // kriging.rs:50-59
+    K->min_nb = cfg.min_neighbors == GS_NONE ? 1 : (int32_t)cfg.min_neighbors;
+    // centre / scale for the drift basis (kriging.rs:340-347)
+    double mn[3], mx[3];
+    for (int d = 0; d < dim; d++) { mn[d] = INFINITY; mx[d] = -INFINITY; }
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) {
+            mn[d] = std::fmin(mn[d], coords[i * dim + d]);
+            mx[d] = std::fmax(mx[d], coords[i * dim + d]);
+        }
+    double spread = 0.0;
+    for (int d = 0; d < dim; d++) {
+        K->center[d] = 0.5 * (mn[d] + mx[d]);
+        spread = std::fmax(spread, mx[d] - mn[d]);
+    }
+    K->scale = std::fmax(0.5 * spread, std::numeric_limits<double>::min());
+    K->has_tree = cfg.max_neighbors != GS_NONE || cfg.has_search_radius || cfg.max_per_octant != GS_NONE;
+    // Neighbour-list capacity (stride) and padded system bound (nb_bound).
+    // Systems hold at most 64 neighbours on the GPU.  When the configuration
+    // does not bound the list by 64 (radius-only search, max_neighbors above
+    // 64), the search keeps 65 slots: a target that fills them exceeds the
+    // GPU bound and gets a NaN estimate with status GS_ERR_INVALID_PARAMETER,
+    // while every other target is exact (its list never saturated).
+    if (!K->has_tree) {
+        if (n > 64)
+            return fail(GS_ERR_INVALID_PARAMETER,
+                        "unsupported on GPU: global neighbourhood with more than 64 points");
+        K->stride = n;
+    } else if (cfg.max_per_octant != GS_NONE) {
+        if (cfg.max_per_octant > 64 || (cfg.max_per_octant << dim) > 64)
+            return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: max_per_octant * 2^dim > 64");
+        int64_t cap = cfg.max_per_octant << dim;
+        K->per_oct = (int)cfg.max_per_octant;
+        K->stride = cfg.max_neighbors != GS_NONE ? std::min(cap, cfg.max_neighbors) : cap;
+    } else {
+        int64_t req = cfg.max_neighbors != GS_NONE ? cfg.max_neighbors : n;
+        K->k_heap = std::min<int64_t>(req, 65);
+        K->stride = std::min(K->k_heap, n);
+    }
+    K->nb_bound = std::min<int64_t>(K->stride, 64);
+    K->r2 = cfg.has_search_radius ? cfg.search_radius * cfg.search_radius
+                                  : std::numeric_limits<double>::infinity();
+    K->xf = make_search_xf(cfg, dim);
+
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(K->coords.ensure(sizeof(double) * n * dim));
+    HIP_TRY(K->values.ensure(sizeof(double) * n));
+    HIP_TRY(hipMemcpy(K->coords.p, coords, sizeof(double) * n * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(K->values.p, values, sizeof(double) * n, hipMemcpyHostToDevice));
+    DevModel dm = resolve_model(*model, dim, ctx->nodes.as<gs::BesselNodes>());
+    HIP_TRY(K->dmodel.ensure(sizeof(DevModel)));
+    HIP_TRY(hipMemcpy(K->dmodel.p, &dm, sizeof(DevModel), hipMemcpyHostToDevice));
+    if (K->has_tree) {
+        // kd-tree on search-frame coordinates (kriging.rs:367-381)
+        std::vector<double> tc((size_t)(n * dim));
+        const gs_anisotropy *an = cfg.has_anisotropic_search ? &cfg.anisotropic_search : nullptr;
+        for (int64_t i = 0; i < n; i++) search_xf_host(an, dim, coords + i * dim, &tc[(size_t)(i * dim)]);
+        std::vector<uint32_t> idx;
+        kd_layout(dim, tc.data(), n, idx);
+        std::vector<double> tree((size_t)(n * dim));
+        for (int64_t p = 0; p < n; p++)
+            for (int d = 0; d < dim; d++) tree[(size_t)(p * dim + d)] = tc[(size_t)idx[(size_t)p] * dim + d];
+        K->depth = tree_depth(n);
+        HIP_TRY(K->tcoords.ensure(sizeof(double) * n * dim));
+        HIP_TRY(K->tidx.ensure(sizeof(uint32_t) * n));
+        HIP_TRY(hipMemcpy(K->tcoords.p, tree.data(), sizeof(double) * n * dim, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(K->tidx.p, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    }
+    *out = K.release();
+    return GS_OK;
+}
+
+void gs_kriging_free(gs_kriging *k) {
+    if (!k) return;
+    (void)hipSetDevice(k->ctx->device);
+    delete k;
+}
+
+int64_t gs_kriging_max_neighbors(const gs_kriging *k) { return k ? k->stride : -1; }
+
+int32_t gs_kriging_set_timing(gs_kriging *k, int32_t enable) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    std::lock_guard<std::mutex> lk(k->mu);
+    k->timing = enable != 0;
+    return GS_OK;
+}
+
+int32_t gs_kriging_timing_summary(gs_kriging *k, int64_t *calls, double *ms_search,
+                                  double *ms_solve) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    double a = 0.0, b = 0.0;
+    for (auto &m : k->marks) {
+        HIP_TRY(hipEventSynchronize(m[2]));
+        float x = 0.f, y = 0.f;
+        HIP_TRY(hipEventElapsedTime(&x, m[0], m[1]));
+        HIP_TRY(hipEventElapsedTime(&y, m[1], m[2]));
+        a += x;
+        b += y;
+    }
+    if (calls) *calls = (int64_t)k->marks.size();
+    if (ms_search) *ms_search = a;
+    if (ms_solve) *ms_solve = b;
+    for (auto &m : k->marks) k->event_pool.insert(k->event_pool.end(), m.begin(), m.end());
+    k->marks.clear();
+    return GS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int32_t stride_of(const gs_kriging *K) { return (int32_t)std::max<int64_t>(K->stride, 1); }
+
+// Neighbour lists for T device targets into the handle's nbr/count buffers.
+int32_t run_neighbors(gs_kriging *K, const double *d_targets, int64_t T, hipStream_t s) {
+    const int32_t stride = stride_of(K);
+    HIP_TRY(K->nbr.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1) * stride));
+    HIP_TRY(K->count.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1)));
+    if (!K->has_tree) {
+        HIP_TRY(gs::launch_iota_nbr(K->nbr.as<int32_t>(), K->count.as<int32_t>(), T, (int32_t)K->n,
+                                    stride, s));
+        return GS_OK;
+    }
+    gs::KnnParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.tcoords = K->tcoords.as<double>();
+    p.tidx = K->tidx.as<uint32_t>();
+    p.n = K->n;
+    p.targets = d_targets;
+    p.T = T;
+    p.k = (int32_t)K->k_heap;
+    p.per_octant = K->per_oct;
+    p.max_nb = (int32_t)K->stride;
+    p.depth = K->depth;
+    p.r2 = K->r2;
+    p.xf = K->xf;
+    p.out_idx = K->nbr.as<int32_t>();
+    p.out_count = K->count.as<int32_t>();
+    p.stride = stride;
+    HIP_TRY(gs::launch_knn(K->D, p, s));
+    return GS_OK;
+}
+
+int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
+                   double *d_value, double *d_var, double *d_lag, int32_t *d_status,
+                   hipStream_t s);
+
+int32_t run_predict(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
+                    double *d_value, double *d_var, double *d_lag, int32_t *d_status,
+                    hipStream_t s) {
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    if (T == 0) return GS_OK;
+    std::array<hipEvent_t, 3> ev{};
+    if (K->timing) {
+        for (auto &e : ev) HIP_TRY(K->take_event(&e));
+        HIP_TRY(hipEventRecord(ev[0], s));
+    }
+    int32_t st = run_neighbors(K, d_targets, T, s);
+    if (st != GS_OK) return st;
+    if (K->timing) HIP_TRY(hipEventRecord(ev[1], s));
+    st = run_solves(K, d_targets, T, precision, d_value, d_var, d_lag, d_status, s);
+    if (st != GS_OK) return st;
+    if (K->timing) {
+        HIP_TRY(hipEventRecord(ev[2], s));
+        K->marks.push_back(ev);
+    }
+    return GS_OK;
+}
+
+int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
+                   double *d_value, double *d_var, double *d_lag, int32_t *d_status,
+                   hipStream_t s) {
+    gs::SolveParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.coords = K->coords.as<double>();
+    p.values = K->values.as<double>();
+    p.targets = d_targets;
+    p.nbr = K->nbr.as<int32_t>();
+    p.count = K->count.as<int32_t>();
+    p.T = T;
+    p.stride = stride_of(K);
+    p.min_nb = K->min_nb;
+    p.method = K->cfg.method;
+    p.degree = K->cfg.degree;
+    p.sk_mean = K->cfg.simple_mean;
+    for (int d = 0; d < 3; d++) p.center[d] = K->center[d];
+    p.scale = K->scale;
+    p.model = K->dmodel.as<DevModel>();
+    p.value = d_value;
+    p.variance = d_var;
+    p.lagrange = d_lag;
+    p.status = d_status;
+    auto launch = [&](int NB, const gs::SolveParams &pp, int64_t groups) -> hipError_t {
+        return precision == GS_PREC_FP64 ? gs::launch_krige_fp64(K->D, NB, K->M, pp, groups, s)
+                                         : gs::launch_krige_fp32r(K->D, NB, K->M, pp, groups, s);
+    };
+    if (K->per_oct > 0 && K->nb_bound > 16) {
+        // Uneven sizes: bucket by padded system size, one launch per bucket.
+        HIP_TRY(K->lists.ensure(sizeof(int32_t) * 3 * T));
+        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 4));
+        HIP_TRY(gs::launch_zero_i32(K->lens.as<int32_t>(), 4, s));
+        HIP_TRY(gs::launch_bucket(K->count.as<int32_t>(), T, K->lists.as<int32_t>(),
+                                  K->lens.as<int32_t>(), T, s));
+        const int nbs[3] = {16, 32, 64};
+        for (int b = 0; b < 3; b++) {
+            if (nbs[b] > pad_nb(K->nb_bound)) break;
+            gs::SolveParams pb = p;
+            pb.list = K->lists.as<int32_t>() + (int64_t)b * T;
+            pb.list_len = K->lens.as<int32_t>() + b;
+            // persistent grid: enough groups to fill the device, grid-strided
+            int64_t groups = std::min<int64_t>(T, (int64_t)256 * 64 * 4);
+            hipError_t e = launch(nbs[b], pb, groups);
+            if (e == hipErrorInvalidValue)
+                return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size/precision");
+            HIP_TRY(e);
+        }
+        return GS_OK;
+    }
+    hipError_t e = launch(pad_nb(K->nb_bound), p, T);
+    if (e == hipErrorInvalidValue)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size/precision");
+    HIP_TRY(e);
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, int64_t T,
+                                       int32_t precision, double *d_value, double *d_variance,
+                                       double *d_lagrange, int32_t *d_status, void *stream) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T > 0 && (!d_targets || !d_value || !d_variance))
+        return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : k->ctx->stream;
+    return run_predict(k, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status, s);
+}
+
+int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T, int32_t precision,
+                                double *value, double *variance, double *lagrange, int32_t *status) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T == 0) return GS_OK;
+    if (!targets || !value || !variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    hipStream_t s = k->ctx->stream;
+    HIP_TRY(k->h_targets.ensure(sizeof(double) * T * k->D));
+    HIP_TRY(k->h_value.ensure(sizeof(double) * T));
+    HIP_TRY(k->h_var.ensure(sizeof(double) * T));
+    HIP_TRY(k->h_lag.ensure(sizeof(double) * T));
+    HIP_TRY(k->h_status.ensure(sizeof(int32_t) * T));
+    HIP_TRY(hipMemcpyAsync(k->h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
+    int32_t st = run_predict(k, k->h_targets.as<double>(), T, precision, k->h_value.as<double>(),
+                             k->h_var.as<double>(), k->h_lag.as<double>(), k->h_status.as<int32_t>(), s);
+    if (st != GS_OK) return st;
+    HIP_TRY(hipMemcpyAsync(value, k->h_value.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(variance, k->h_var.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+    if (lagrange) HIP_TRY(hipMemcpyAsync(lagrange, k->h_lag.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+    if (status) HIP_TRY(hipMemcpyAsync(status, k->h_status.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GS_OK;
+}
+
+int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, int64_t max_out,
+                             int32_t *idx, int32_t *count) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T == 0) return GS_OK;
+    if (!targets || !idx || !count) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (max_out < k->stride) return fail(GS_ERR_INVALID_PARAMETER, "max_out smaller than the neighbourhood bound");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    hipStream_t s = k->ctx->stream;
+    HIP_TRY(k->h_targets.ensure(sizeof(double) * T * k->D));
+    HIP_TRY(hipMemcpyAsync(k->h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
+    int32_t st = run_neighbors(k, k->h_targets.as<double>(), T, s);
+    if (st != GS_OK) return st;
+    const int32_t stride = stride_of(k);
+    std::vector<int32_t> tmp((size_t)(T * stride));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), k->nbr.p, sizeof(int32_t) * T * stride, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(count, k->count.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int64_t t = 0; t < T; t++)
+        for (int64_t j = 0; j < max_out; j++)
+            idx[t * max_out + j] = j < stride ? tmp[(size_t)(t * stride + j)] : -1;
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Vecchia plan: maxmin ordering + nearest predecessors (vecchia.rs:144-282)
+// ---------------------------------------------------------------------------
+namespace {
+
+double dist2(const double *a, const double *b, int D) {   // vecchia.rs:95-102
+    double s = 0.0;
+    for (int d = 0; d < D; d++) {
+        double v = a[d] - b[d];
+        s += v * v;
+    }
+    return s;
+}
+
+// Bucket grid with the reference's sizing rule and shell-walk order
+// (search.rs:119-313).  Entries carry their insertion position; a query can
+// be restricted to positions < limit, which replays the reference's
+// incremental grid exactly (same buckets, same in-bucket order).
+struct BGrid {
+    int D;
+    double origin[3];
+    double cell;
+    int64_t nn[3], strides[3];
+    std::vector<int64_t> start;      // CSR buckets (built once) ...
+    std::vector<uint32_t> items;
+    std::vector<std::vector<uint32_t>> dyn;   // ... or growable buckets
+    const double *pts;               // [pos][D]
+    bool csr = false;
+
+    void init(int D_, const double *mn, const double *mx, int64_t expected) {   // search.rs:138-197
+        D = D_;
+        double ext[3] = {0, 0, 0}, max_extent = 0.0;
+        for (int d = 0; d < D; d++) { ext[d] = mx[d] - mn[d]; max_extent = std::fmax(max_extent, ext[d]); }
+        double tol = max_extent * 1e-9;
+        bool active[3];
+        int n_active = 0;
+        for (int d = 0; d < D; d++) { active[d] = ext[d] > tol; n_active += active[d]; }
+        if (n_active < 1) n_active = 1;
+        double volume = 1.0;
+        for (int d = 0; d < D; d++) if (active[d]) volume *= ext[d];
+        int64_t em = std::max<int64_t>(expected, 1);
+        double c = std::fmax(std::pow(volume, 1.0 / (double)n_active), 1e-12) *
+                   (1.0 / std::pow((double)em, 1.0 / (double)n_active));
+        int64_t cap = std::max<int64_t>(em * 4, 64);
+        for (;;) {
+            int64_t total = 1;
+            bool over = false;
+            for (int d = 0; d < D; d++) {
+                if (active[d]) {
+                    double q = std::ceil(ext[d] / c);
+                    int64_t v = q >= 9.2e18 ? INT64_MAX : (q > 0 ? (int64_t)q : 0);
+                    nn[d] = std::max<int64_t>(v, 1);
+                } else nn[d] = 1;
+                if (total > INT64_MAX / nn[d]) { over = true; break; }
+                total *= nn[d];
+            }
+            if (!over && total <= cap) break;
+            c *= 2.0;
+        }
+        int64_t total = 1;
+        for (int d = 0; d < D; d++) { strides[d] = total; total *= nn[d]; origin[d] = mn[d]; }
+        cell = c;
+        dyn.clear();
+    }
+    int64_t nbuckets() const {
+        int64_t t = 1;
+        for (int d = 0; d < D; d++) t *= nn[d];
+        return t;
+    }
+    static int64_t sat(double v) {
+        if (v != v) return 0;
+        if (v >= 9.2233720368547758e18) return INT64_MAX;
+        if (v <= -9.2233720368547758e18) return INT64_MIN;
+        return (int64_t)v;
+    }
+    void cell_of(const double *p, int64_t *c) const {   // search.rs:199-206
+        for (int d = 0; d < D; d++) {
+            int64_t v = sat((p[d] - origin[d]) / cell);
+            c[d] = std::min(std::max<int64_t>(v, 0), nn[d] - 1);
+        }
+    }
+    int64_t bucket_of(const double *p) const {
+        int64_t c[3];
+        cell_of(p, c);
+        int64_t b = 0;
+        for (int d = 0; d < D; d++) b += c[d] * strides[d];
+        return b;
+    }
+    // All points inserted in position order, stored as CSR.
+    void build_all(const double *p, int64_t npos) {
+        pts = p;
+        int64_t nb = nbuckets();
+        start.assign((size_t)nb + 1, 0);
+        std::vector<int64_t> bk((size_t)npos);
+        for (int64_t i = 0; i < npos; i++) { bk[(size_t)i] = bucket_of(p + i * D); start[(size_t)bk[(size_t)i] + 1]++; }
+        for (int64_t b = 0; b < nb; b++) start[(size_t)b + 1] += start[(size_t)b];
+        items.resize((size_t)npos);
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t i = 0; i < npos; i++) items[(size_t)fill[(size_t)bk[(size_t)i]]++] = (uint32_t)i;
+        csr = true;
+    }
+};
+
+// Bounded max-heap over (d2, pos) with the reference's push rule
+// (search.rs:15-30).
+struct KHeap {
+    std::vector<std::pair<double, uint32_t>> v;
+    int64_t k;
+    static bool gt(const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+        return a.first > b.first || (a.first == b.first && a.second > b.second);
+    }
+    void reset(int64_t k_) { k = k_; v.clear(); }
+    double worst() const { return (int64_t)v.size() == k ? v[0].first : INFINITY; }
+    void push(double d2, uint32_t i) {
+        if ((int64_t)v.size() < k) {
+            v.push_back({d2, i});
+            size_t e = v.size() - 1;
+            while (e > 0) {
+                size_t p = (e - 1) / 2;
+                if (gt(v[e], v[p])) { std::swap(v[e], v[p]); e = p; } else break;
+            }
+        } else if (d2 < v[0].first) {
+            v[0] = {d2, i};
+            size_t e = 0, n = v.size();
+            for (;;) {
+                size_t l = 2 * e + 1, r = l + 1, g = e;
+                if (l < n && gt(v[l], v[g])) g = l;
+                if (r < n && gt(v[r], v[g])) g = r;
+                if (g == e) break;
+                std::swap(v[e], v[g]);
+                e = g;
+            }
+        }
+    }
+};
+
+// BucketGrid::k_nearest (search.rs:221-313) over positions < limit.
+void grid_knn(const BGrid &G, const double *target, int64_t k, int64_t limit, KHeap &h,
+              std::vector<uint32_t> &out) {
+    out.clear();
+    h.reset(k);
+    if (k == 0 || limit == 0) return;
+    const int D = G.D;
+    int64_t tc[3];
+    G.cell_of(target, tc);
+    int64_t max_ring = G.nn[0];
+    for (int d = 1; d < D; d++) max_ring = std::max(max_ring, G.nn[d]);
+    auto visit = [&](const int64_t *cur) {
+        for (int d = 0; d < D; d++) if (cur[d] < 0 || cur[d] >= G.nn[d]) return;
+        int64_t b = 0;
+        for (int d = 0; d < D; d++) b += cur[d] * G.strides[d];
+        for (int64_t q = G.start[(size_t)b]; q < G.start[(size_t)b + 1]; q++) {
+            uint32_t i = G.items[(size_t)q];
+            if ((int64_t)i >= limit) break;   // positions ascend inside a bucket
+            double d2 = 0.0;
+            for (int d = 0; d < D; d++) {
+                double dd = G.pts[(int64_t)i * D + d] - target[d];
+                d2 += dd * dd;
+            }
+            h.push(d2, i);
+        }
+    };
+    for (int64_t ring = 0; ring <= max_ring; ring++) {
+        double rm = std::fmax((double)ring - 1.0, 0.0) * G.cell;
+        if (rm * rm > h.worst()) break;
+        if (ring == 0) { visit(tc); continue; }
+        int64_t r = ring;
+        for (int fd = 0; fd < D; fd++) {
+            for (int sd = 0; sd < 2; sd++) {
+                int64_t side = sd == 0 ? -r : r;
+                int64_t lo[3], hi[3];
+                bool empty = false;
+                for (int d = 0; d < D; d++) {
+                    if (d < fd) { lo[d] = tc[d] - r + 1; hi[d] = tc[d] + r - 1; }
+                    else if (d == fd) { lo[d] = tc[d] + side; hi[d] = tc[d] + side; }
+                    else { lo[d] = tc[d] - r; hi[d] = tc[d] + r; }
+                    if (lo[d] > hi[d]) empty = true;
+                }
+                if (empty) continue;
+                int64_t cur[3] = {lo[0], lo[1], lo[2]};
+                for (;;) {
+                    visit(cur);
+                    int d = 0;
+                    for (; d < D; d++) {
+                        if (++cur[d] <= hi[d]) break;
+                        cur[d] = lo[d];
+                    }
+                    if (d == D) break;
+                }
+            }
+        }
+    }
+    std::sort(h.v.begin(), h.v.end());
+    for (auto &e : h.v) out.push_back(e.second);
+}
+
+// maxmin_order (vecchia.rs:144-215): exact farthest-point traversal, ties
+// to the larger index.  Lazy max-heap of stale upper bounds; a popped key
+// is refreshed against the selected set (brute force while it has <= 48
+// points, else a 1-NN query on a bucket grid of the selected points).
+void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &order) {
+    order.clear();
+    if (n == 0) return;
+    double c[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < D; d++) c[d] += coords[i * D + d] / (double)n;
+    int64_t first = 0;
+    double bd = dist2(coords, c, D);
+    for (int64_t i = 1; i < n; i++) {
+        double dd = dist2(coords + i * D, c, D);
+        if (total_key(dd) < total_key(bd)) { bd = dd; first = i; }
+    }
+    double mn[3], mx[3];
+    for (int d = 0; d < D; d++) { mn[d] = coords[d]; mx[d] = coords[d]; }
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < D; d++) {
+            mn[d] = std::fmin(mn[d], coords[i * D + d]);
+            mx[d] = std::fmax(mx[d], coords[i * D + d]);
+        }
+    // Growable grid of selected points (sized as the reference: ~1 per bucket at n).
+    BGrid G;
+    G.init(D, mn, mx, n);
+    const int64_t nb = G.nbuckets();
+    std::vector<std::vector<uint32_t>> buckets((size_t)nb);
+    std::vector<double> sel;
+    sel.reserve((size_t)(n * D));
+    auto add_sel = [&](int64_t i) {
+        int64_t b = G.bucket_of(coords + i * D);
+        buckets[(size_t)b].push_back((uint32_t)(sel.size() / D));
+        for (int d = 0; d < D; d++) sel.push_back(coords[i * D + d]);
+        order.push_back(i);
+    };
+    add_sel(first);
+    struct Key { double d2; int64_t idx; };
+    auto kgt = [](const Key &a, const Key &b) {   // MaxminKey order (vecchia.rs:114-125)
+        int64_t ka = total_key(a.d2), kb = total_key(b.d2);
+        return ka > kb || (ka == kb && a.idx > b.idx);
+    };
+    auto cmp = [&](const Key &a, const Key &b) { return kgt(b, a); };   // max-heap
+    std::vector<Key> heap;
+    heap.reserve((size_t)n);
+    for (int64_t i = 0; i < n; i++)
+        if (i != first) heap.push_back({dist2(coords + i * D, coords + first * D, D), i});
+    std::make_heap(heap.begin(), heap.end(), cmp);
+    int64_t max_ring = G.nn[0];
+    for (int d = 1; d < D; d++) max_ring = std::max(max_ring, G.nn[d]);
+    while (!heap.empty()) {
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        Key top = heap.back();
+        heap.pop_back();
+        const double *pi = coords + top.idx * D;
+        const int64_t nsel = (int64_t)(sel.size() / D);
+        double current = INFINITY;
+        if (nsel <= 48) {
+            for (int64_t s = 0; s < nsel; s++) current = std::fmin(current, dist2(pi, &sel[(size_t)(s * D)], D));
+        } else {
+            // 1-NN over the selected set: the minimum distance is what matters
+            // and it does not depend on which of several tied points is kept.
+            int64_t tc[3];
+            G.cell_of(pi, tc);
+            double best = INFINITY;
+            for (int64_t ring = 0; ring <= max_ring; ring++) {
+                double rm = std::fmax((double)ring - 1.0, 0.0) * G.cell;
+                if (rm * rm > best) break;
+                auto visit = [&](const int64_t *cur) {
+                    for (int d = 0; d < D; d++) if (cur[d] < 0 || cur[d] >= G.nn[d]) return;
+                    int64_t b = 0;
+                    for (int d = 0; d < D; d++) b += cur[d] * G.strides[d];
+                    for (uint32_t s : buckets[(size_t)b]) {
+                        double d2 = dist2(pi, &sel[(size_t)s * D], D);
+                        if (d2 < best) best = d2;
+                    }
+                };
+                if (ring == 0) { visit(tc); continue; }
+                int64_t r = ring;
+                for (int fd = 0; fd < D; fd++)
+                    for (int sd = 0; sd < 2; sd++) {
+                        int64_t side = sd == 0 ? -r : r;
+                        int64_t lo[3], hi[3];
+                        bool empty = false;
+                        for (int d = 0; d < D; d++) {
+                            if (d < fd) { lo[d] = tc[d] - r + 1; hi[d] = tc[d] + r - 1; }
+                            else if (d == fd) { lo[d] = tc[d] + side; hi[d] = tc[d] + side; }
+                            else { lo[d] = tc[d] - r; hi[d] = tc[d] + r; }
+                            if (lo[d] > hi[d]) empty = true;
+                        }
+                        if (empty) continue;
+                        int64_t cur[3] = {lo[0], lo[1], lo[2]};
+                        for (;;) {
+                            visit(cur);
+                            int d = 0;
+                            for (; d < D; d++) {
+                                if (++cur[d] <= hi[d]) break;
+                                cur[d] = lo[d];
+                            }
+                            if (d == D) break;
+                        }
+                    }
+            }
+            current = best;
+        }
+        Key refreshed{current, top.idx};
+        if (heap.empty() || !kgt(heap.front(), refreshed)) {
+            add_sel(top.idx);
+        } else {
+            heap.push_back(refreshed);
+            std::push_heap(heap.begin(), heap.end(), cmp);
+        }
+    }
+}
+
+int host_threads() {
+    unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(h, 64u));
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int64_t n, int64_t m,
+                            const int64_t *order, gs_vplan **out) {
+    if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
+    *out = nullptr;
+    if (!ctx) return fail(GS_ERR_INVALID_PARAMETER, "ctx is NULL");
+    if (dim != 2 && dim != 3) return fail(GS_ERR_DIMENSION_MISMATCH, "dim must be 2 or 3");
+    if (n < 2) return fail(GS_ERR_INSUFFICIENT_DATA, "Vecchia requires at least 2 points");
+    if (m <= 0) return fail(GS_ERR_INVALID_PARAMETER, "conditioning size m must be at least 1");
+    if (!coords) return fail(GS_ERR_INVALID_PARAMETER, "coords is NULL");
+    for (int64_t i = 0; i < n * dim; i++)
+        if (!std::isfinite(coords[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite coordinate");
+    int64_t pair[2];
+    if (duplicate_pair(dim, coords, n, pair))
+        return fail(GS_ERR_DUPLICATE_POINTS, "duplicate points " + std::to_string(pair[0]) + ", " +
+                                                 std::to_string(pair[1]));
+    if (m > 64) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: conditioning size above 64");
+    std::unique_ptr<gs_vplan> P(new gs_vplan());
+    P->ctx = ctx;
+    P->D = dim;
+    P->n = n;
+    P->m = m;
+    if (order) {
+        std::vector<char> seen((size_t)n, 0);
+        for (int64_t i = 0; i < n; i++) {
+            if (order[i] < 0 || order[i] >= n || seen[(size_t)order[i]])
+                return fail(GS_ERR_INVALID_PARAMETER, "order is not a permutation of 0..n");
+            seen[(size_t)order[i]] = 1;
+        }
+        P->order.assign(order, order + n);
+    } else {
+        maxmin_order(dim, coords, n, P->order);
+    }
+    // Nearest predecessors (vecchia.rs:262-281): all points inserted in
+    // ordered position; query k only sees positions < k.
+    std::vector<double> opts((size_t)(n * dim));
+    for (int64_t k = 0; k < n; k++)
+        for (int d = 0; d < dim; d++) opts[(size_t)(k * dim + d)] = coords[P->order[(size_t)k] * dim + d];
+    double mn[3], mx[3];
+    for (int d = 0; d < dim; d++) { mn[d] = coords[d]; mx[d] = coords[d]; }
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) {
+            mn[d] = std::fmin(mn[d], coords[i * dim + d]);
+            mx[d] = std::fmax(mx[d], coords[i * dim + d]);
+        }
+    BGrid G;
+    G.init(dim, mn, mx, n);
+    G.build_all(opts.data(), n);
+    P->nb.assign((size_t)(n * m), -1);
+    P->nb_count.assign((size_t)n, 0);
+    std::atomic<int64_t> next(0);
+    auto worker = [&]() {
+        KHeap h;
+        std::vector<uint32_t> res;
+        for (;;) {
+            int64_t s = next.fetch_add(256);
+            if (s >= n) break;
+            int64_t e = std::min(n, s + 256);
+            for (int64_t k = s; k < e; k++) {
+                grid_knn(G, &opts[(size_t)(k * dim)], m, k, h, res);
+                P->nb_count[(size_t)k] = (int32_t)res.size();
+                for (size_t q = 0; q < res.size(); q++)
+                    P->nb[(size_t)(k * m) + q] = (int32_t)P->order[res[q]];
+            }
+        }
+    };
+    int nt = host_threads();
+    std::vector<std::thread> th;
+    for (int i = 1; i < nt; i++) th.emplace_back(worker);
+    worker();
+    for (auto &t : th) t.join();
+    for (int64_t k = 0; k < n; k++) P->max_count = std::max(P->max_count, P->nb_count[(size_t)k]);
+
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(P->coords.ensure(sizeof(double) * n * dim));
+    HIP_TRY(P->d_order.ensure(sizeof(int64_t) * n));
+    HIP_TRY(P->d_nb.ensure(sizeof(int32_t) * n * m));
+    HIP_TRY(P->d_count.ensure(sizeof(int32_t) * n));
+    HIP_TRY(hipMemcpy(P->coords.p, coords, sizeof(double) * n * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_order.p, P->order.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_nb.p, P->nb.data(), sizeof(int32_t) * n * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_count.p, P->nb_count.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    *out = P.release();
+    return GS_OK;
+}
+
+void gs_vecchia_plan_free(gs_vplan *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->device);
+    delete p;
+}
+
+int32_t gs_vecchia_plan_get(const gs_vplan *p, int64_t *order, int32_t *nb, int32_t *nb_count) {
+    if (!p) return fail(GS_ERR_INVALID_PARAMETER, "plan is NULL");
+    if (order) std::memcpy(order, p->order.data(), sizeof(int64_t) * p->n);
+    if (nb) std::memcpy(nb, p->nb.data(), sizeof(int32_t) * p->n * p->m);
+    if (nb_count) std::memcpy(nb_count, p->nb_count.data(), sizeof(int32_t) * p->n);
+    return GS_OK;
+}
+
+int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_model *models,
+                                int32_t n_models, int32_t precision, double *out, int32_t *status) {
+    if (!p) return fail(GS_ERR_INVALID_PARAMETER, "plan is NULL");
+    if (n_models < 0) return fail(GS_ERR_INVALID_PARAMETER, "n_models < 0");
+    if (n_models == 0) return GS_OK;
+    if (!values || !models || !out) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    if (precision != GS_PREC_FP64)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined Vecchia");
+    const int64_t n = p->n;
+    for (int64_t i = 0; i < n; i++)
+        if (!std::isfinite(values[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite value");
+    for (int32_t i = 0; i < n_models; i++) {
+        int32_t st = validate_model(&models[i], p->D, false);
+        if (st != GS_OK) return st;
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    // z = v - mean (vecchia.rs:290-292), mean as a left-to-right sum.
+    double mean = -0.0;
+    for (int64_t i = 0; i < n; i++) mean += values[i];
+    mean = mean / (double)n;
+    std::vector<double> zc((size_t)n);
+    for (int64_t i = 0; i < n; i++) zc[(size_t)i] = values[i] - mean;
+    std::vector<DevModel> dm((size_t)n_models);
+    for (int32_t i = 0; i < n_models; i++) dm[(size_t)i] = resolve_model(models[i], p->D, p->ctx->nodes.as<gs::BesselNodes>());
+    const int NB = pad_nb(std::max<int32_t>(p->max_count, 1));
+    const int32_t ppb = 64;   // points per block
+    const int64_t n_blocks = (n + ppb - 1) / ppb;
+    HIP_TRY(p->zc.ensure(sizeof(double) * n));
+    HIP_TRY(p->models.ensure(sizeof(DevModel) * n_models));
+    HIP_TRY(p->partial.ensure(sizeof(double) * n_blocks * gs::kMaxModelsPerLaunch));
+    HIP_TRY(p->out.ensure(sizeof(double) * n_models));
+    HIP_TRY(p->status.ensure(sizeof(int32_t) * n_models));
+    HIP_TRY(hipMemcpyAsync(p->zc.p, zc.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(p->models.p, dm.data(), sizeof(DevModel) * n_models, hipMemcpyHostToDevice, s));
+    HIP_TRY(gs::launch_zero_i32(p->status.as<int32_t>(), n_models, s));
+    for (int32_t b0 = 0; b0 < n_models; b0 += gs::kMaxModelsPerLaunch) {
+        int32_t nm = std::min<int32_t>(gs::kMaxModelsPerLaunch, n_models - b0);
+        gs::VecchiaParams vp;
+        std::memset(&vp, 0, sizeof(vp));
+        vp.coords = p->coords.as<double>();
+        vp.zc = p->zc.as<double>();
+        vp.order = p->d_order.as<int64_t>();
+        vp.nb = p->d_nb.as<int32_t>();
+        vp.nb_count = p->d_count.as<int32_t>();
+        vp.n = n;
+        vp.m = (int32_t)p->m;
+        vp.n_models = nm;
+        vp.models = p->models.as<DevModel>() + b0;
+        vp.partial = p->partial.as<double>();
+        vp.status = p->status.as<int32_t>() + b0;
+        vp.points_per_block = ppb;
+        hipError_t e = gs::launch_vecchia(p->D, NB, 0, vp, n_blocks, s);
+        if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
+        HIP_TRY(e);
+        HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm, p->out.as<double>() + b0, s));
+    }
+    std::vector<int32_t> st((size_t)n_models);
+    HIP_TRY(hipMemcpyAsync(out, p->out.p, sizeof(double) * n_models, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(st.data(), p->status.p, sizeof(int32_t) * n_models, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int32_t i = 0; i < n_models; i++) {
+        if (st[(size_t)i] != 0) out[i] = std::numeric_limits<double>::quiet_NaN();
+        if (status) status[i] = st[(size_t)i];
+    }
+    return GS_OK;
+}
+
+int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                          int64_t n, const gs_model *model, int64_t m, const int64_t *order,
+                          double *out) {
+    if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
+    // reject_power_model / reject_invalid_dim_model come first (vecchia.rs:880-881)
+    int32_t st = validate_model(model, dim, false);
+    if (st != GS_OK) return st;
+    gs_vplan *p = nullptr;
+    st = gs_vecchia_plan_new(ctx, dim, coords, n, m, order, &p);
+    if (st != GS_OK) return st;
+    int32_t s1 = 0;
+    st = gs_vecchia_loglik_batch(p, values, model, 1, GS_PREC_FP64, out, &s1);
+    gs_vecchia_plan_free(p);
+    if (st != GS_OK) return st;
+    if (s1 != 0) return fail(s1, "singular conditioning system");
+    return GS_OK;
+}
+
+}  // extern "C"

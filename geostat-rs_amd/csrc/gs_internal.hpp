@@ -1,0 +1,110 @@
+// Shared declarations between the host runtime (gs_host.cpp) and the
+// kernel launchers (gs_search.hip, gs_solve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gs_device.hpp"
+
+namespace gs {
+
+// Search frame: search_point (kriging.rs:16-21) applies
+// Anisotropy::transform_vector to absolute coordinates.  The 2-D form keeps
+// the reference's exact expression [x s + y c, (x c - y s) / ratio]
+// (model.rs:362-364) and the 3-D form the GSLIB matrix rows summed left to
+// right (model.rs:358-360), so search distances are bit-identical.
+struct SearchXf {
+    int32_t on;
+    int32_t pad;
+    double s, c, ratio;
+    double r[9];
+};
+
+struct KnnParams {
+    const double *tcoords;   // [n][D] search-frame coords in kd-tree layout order
+    const uint32_t *tidx;    // kd-tree layout position -> original index
+    int64_t n;
+    const double *targets;   // [T][D] original frame
+    int64_t T;
+    int32_t k;               // heap mode: max_neighbors (or n)
+    int32_t per_octant;      // > 0: octant mode
+    int32_t max_nb;          // octant mode: truncation length
+    int32_t depth;           // kd-tree depth bound (stack size)
+    double r2;               // radius^2 or +inf
+    SearchXf xf;
+    int32_t *out_idx;        // [T][stride]
+    int32_t *out_count;      // [T]
+    int32_t stride;
+};
+
+struct SolveParams {
+    const double *coords;    // [n][D] original frame
+    const double *values;    // [n]
+    const double *targets;   // [T][D]
+    const int32_t *nbr;      // [T][stride]
+    const int32_t *count;    // [T]
+    const int32_t *list;     // optional bucket list of target ids
+    const int32_t *list_len; // device counter for list
+    int64_t T;
+    int32_t stride;
+    int32_t min_nb;
+    int32_t method;
+    int32_t degree;
+    double sk_mean;
+    double center[3];
+    double scale;
+    const DevModel *model;   // device memory
+    double *value;
+    double *variance;
+    double *lagrange;
+    int32_t *status;
+};
+
+struct VecchiaParams {
+    const double *coords;    // [n][D]
+    const double *zc;        // [n] centred values
+    const int64_t *order;    // [n]
+    const int32_t *nb;       // [n][m] rows in ordered position
+    const int32_t *nb_count; // [n]
+    int64_t n;
+    int32_t m;
+    int32_t n_models;
+    const DevModel *models;  // [n_models] in device memory
+    double *partial;         // [n_blocks][n_models]
+    int32_t *status;         // [n_models]
+    int32_t points_per_block;
+};
+
+hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s);
+// Returns hipErrorInvalidValue when (D, NB, M) has no instantiation.
+hipError_t launch_krige_fp64(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
+                             hipStream_t s);
+hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
+                              hipStream_t s);
+hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
+                           hipStream_t s);
+hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
+hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
+                         int64_t list_cap, hipStream_t s);
+hipError_t launch_vecchia(int D, int NB, int fp32, const VecchiaParams &p, int64_t n_blocks,
+                          hipStream_t s);
+hipError_t launch_vecchia_reduce(const double *partial, int64_t n_blocks, int32_t n_models,
+                                 double *out, hipStream_t s);
+
+// Per-instantiation launchers (explicit instantiations live in generated
+// translation units, one per template argument set, built in parallel).
+template <int D, int NB, int M>
+hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
+template <int D, int NB, int M>
+hipError_t launch_krige32_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
+template <int D, int NB>
+hipError_t launch_vecchia_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
+
+// Models evaluated by one Vecchia launch (host splits larger batches).
+constexpr int kMaxModelsPerLaunch = 64;
+
+// Group size (lanes per system) for a padded system size NB.
+constexpr int group_size(int NB) { return NB == 16 ? 2 : (NB == 32 ? 8 : 16); }
+
+}  // namespace gs

@@ -1,0 +1,142 @@
+// Dispatch for the templated solve kernels plus the small helper kernels
+// (size bucketing, deterministic Vecchia reduction).  The template bodies
+// are in gs_solve_impl.hpp; each (D, NB, M) instantiation is compiled in
+// its own generated translation unit (see build_lib.py).
+#include "gs_internal.hpp"
+
+namespace gs {
+
+constexpr int kW = 64;
+
+// Size buckets for uneven neighbourhoods (octant search): targets are
+// appended to the list of the smallest padded size that holds them.
+__global__ void bucket_kernel(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
+                              int64_t cap) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int c = count[t];
+    const int b = c <= 16 ? 0 : (c <= 32 ? 1 : 2);
+    const int pos = atomicAdd(&lens[b], 1);
+    lists[(int64_t)b * cap + pos] = (int32_t)t;
+}
+
+// Per-model sum of the per-block partials in a fixed order: lane-strided
+// partial sums, then a fixed pairwise tree.
+__global__ void vecchia_reduce_kernel(const double *partial, int64_t n_blocks, int32_t n_models,
+                                      double *out) {
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    for (int64_t b = lane; b < n_blocks; b += kW) s += partial[b * n_models + p];
+    __shared__ double red[kW];
+    red[lane] = s;
+    __syncthreads();
+    for (int w = kW / 2; w > 0; w >>= 1) {
+        if (lane < w) red[lane] += red[lane + w];
+        __syncthreads();
+    }
+    if (lane == 0) out[p] = red[0];
+}
+
+// Global neighbourhood (kriging.rs:441-444): every target conditions on all
+// data points in index order.
+__global__ void iota_nbr_kernel(int32_t *nbr, int32_t *count, int64_t T, int32_t n,
+                                int32_t stride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    for (int j = 0; j < stride; j++) nbr[t * stride + j] = j < n ? j : -1;
+    count[t] = n;
+}
+
+__global__ void zero_i32_kernel(int32_t *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
+                           hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    hipLaunchKernelGGL(iota_nbr_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, nbr,
+                       count, T, n, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+template <int D, int M>
+static hipError_t krige_nb(int NB, const SolveParams &p, int64_t gg, hipStream_t s) {
+    switch (NB) {
+    case 16: return launch_krige_inst<D, 16, M>(p, gg, s);
+    case 32: return launch_krige_inst<D, 32, M>(p, gg, s);
+    case 64: return launch_krige_inst<D, 64, M>(p, gg, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_krige_fp64(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
+                             hipStream_t s) {
+    if (D == 2) {
+        switch (M) {
+        case 0: return krige_nb<2, 0>(NB, p, grid_groups, s);
+        case 1: return krige_nb<2, 1>(NB, p, grid_groups, s);
+        case 3: return krige_nb<2, 3>(NB, p, grid_groups, s);
+        case 6: return krige_nb<2, 6>(NB, p, grid_groups, s);
+        default: return hipErrorInvalidValue;
+        }
+    } else if (D == 3) {
+        switch (M) {
+        case 0: return krige_nb<3, 0>(NB, p, grid_groups, s);
+        case 1: return krige_nb<3, 1>(NB, p, grid_groups, s);
+        case 4: return krige_nb<3, 4>(NB, p, grid_groups, s);
+        case 10: return krige_nb<3, 10>(NB, p, grid_groups, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
+                              hipStream_t s) {
+    (void)D; (void)NB; (void)M; (void)p; (void)grid_groups; (void)s;
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
+                         int64_t list_cap, hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, count,
+                       T, lists, lens, list_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_vecchia(int D, int NB, int fp32, const VecchiaParams &p, int64_t n_blocks,
+                          hipStream_t s) {
+    (void)fp32;
+    if (D == 2) {
+        switch (NB) {
+        case 16: return launch_vecchia_inst<2, 16>(p, n_blocks, s);
+        case 32: return launch_vecchia_inst<2, 32>(p, n_blocks, s);
+        case 64: return launch_vecchia_inst<2, 64>(p, n_blocks, s);
+        }
+    } else if (D == 3) {
+        switch (NB) {
+        case 16: return launch_vecchia_inst<3, 16>(p, n_blocks, s);
+        case 32: return launch_vecchia_inst<3, 32>(p, n_blocks, s);
+        case 64: return launch_vecchia_inst<3, 64>(p, n_blocks, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_vecchia_reduce(const double *partial, int64_t n_blocks, int32_t n_models,
+                                 double *out, hipStream_t s) {
+    hipLaunchKernelGGL(vecchia_reduce_kernel, dim3((unsigned)n_models), dim3(kW), 0, s, partial,
+                       n_blocks, n_models, out);
+    return hipGetLastError();
+}
+
+}  // namespace gs

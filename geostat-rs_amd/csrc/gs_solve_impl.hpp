@@ -1,0 +1,465 @@
+// Batched kriging / Vecchia solves on gfx950 (CDNA4).
+//
+// One kriging system per group of G lanes inside a 64-wide wavefront.  The
+// data-data covariance block C (n <= NB) is distributed by rows over the
+// group: lane g owns rows i = G*r + g (r < R = NB/G) and keeps, for each
+// owned row, the columns up to the end of its row block, entirely in VGPRs.
+// A right-looking Cholesky runs with the pivot row and the column-j
+// multipliers broadcast inside the group by DPP (quad_perm / row_share) or
+// ds_swizzle, so no LDS traffic sits in the O(n^3) loop.
+//
+// The reference solves the bordered (n+m)^2 system [C F; F' 0][w; mu] =
+// [b; f] by LU with partial pivoting (kriging.rs:527-644, linalg.rs:27-111).
+// Here the same system is solved through its Schur complement, with only
+// forward substitutions fused into the factorization:
+//   yb = L^-1 b,  yz = L^-1 z,  YF = L^-1 F
+//   S  = YF' YF,  mu = S^-1 (YF' yb - f)
+//   value    = yb'yz - mu' YF'yz            (= w'z)
+//   variance = c0 - (yb'yb - mu' YF'yb + mu'f)   (= c0 - w'b0, kriging.rs:625-641)
+//   lagrange = mu[0]                         (= w[n])
+// Simple kriging (m = 0) and the Vecchia conditional (vecchia.rs:302-334)
+// are the m = 0 case of the same routine.
+//
+// Covariance entries are evaluated by a runtime loop into a lane-private
+// LDS staging row and then moved to registers with compile-time indices, so
+// the model code appears once per row block instead of once per entry.
+#pragma once
+
+#include <type_traits>
+
+#include "gs_internal.hpp"
+
+namespace gs {
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// ---- broadcast / reduction inside a group of G lanes --------------------
+template <int G, int SRC>
+__device__ __forceinline__ int bcast32(int v) {
+    if constexpr (G == 1) {
+        return v;
+    } else if constexpr (G == 2) {
+        constexpr int c = SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6);
+        return __builtin_amdgcn_mov_dpp(v, c, 0xF, 0xF, false);
+    } else if constexpr (G == 4) {
+        constexpr int c = SRC | (SRC << 2) | (SRC << 4) | (SRC << 6);
+        return __builtin_amdgcn_mov_dpp(v, c, 0xF, 0xF, false);
+    } else if constexpr (G == 8) {
+        // ds_swizzle bit mode inside 32 lanes: src = (lane & 0x18) | SRC
+        return __builtin_amdgcn_ds_swizzle(v, 0x18 | (SRC << 5));
+    } else {
+        static_assert(G == 16, "group size");
+        return __builtin_amdgcn_mov_dpp(v, 0x150 + SRC, 0xF, 0xF, false);  // row_share
+    }
+}
+
+template <int G, int SRC>
+__device__ __forceinline__ double bcast(double v) {
+    const int lo = __double2loint(v);
+    const int hi = __double2hiint(v);
+    return __hiloint2double(bcast32<G, SRC>(hi), bcast32<G, SRC>(lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __double2loint(v);
+    const int hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false),
+                            __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false));
+}
+template <int PAT>
+__device__ __forceinline__ double swz_d(double v) {
+    const int lo = __double2loint(v);
+    const int hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_ds_swizzle(hi, PAT),
+                            __builtin_amdgcn_ds_swizzle(lo, PAT));
+}
+
+// Butterfly sum; every lane of the group ends with the bitwise-same value.
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+    if constexpr (G >= 2) v += dpp_d<0xB1>(v);                  // quad_perm(1,0,3,2)
+    if constexpr (G >= 4) v += dpp_d<0x4E>(v);                  // quad_perm(2,3,0,1)
+    if constexpr (G >= 8) v += swz_d<0x1F | (4 << 10)>(v);      // lane ^ 4
+    if constexpr (G >= 16) v += swz_d<0x1F | (8 << 10)>(v);     // lane ^ 8
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Drift basis (kriging.rs:410-439): OK [1]; UK deg 1 [1, xn]; deg 2 adds
+// the monomials xn_a xn_b (a <= b).  M = 0 is simple kriging.
+template <int D, int M>
+__device__ __forceinline__ void basis(const SolveParams &P, const double *x, double *f) {
+    if constexpr (M >= 1) f[0] = 1.0;
+    if constexpr (M > 1) {
+        double xn[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) xn[d] = (x[d] - P.center[d]) / P.scale;
+#pragma unroll
+        for (int d = 0; d < D; d++) f[1 + d] = xn[d];
+        if constexpr (M > 1 + D) {
+            int idx = 1 + D;
+#pragma unroll
+            for (int a = 0; a < D; a++)
+#pragma unroll
+                for (int b = a; b < D; b++) f[idx++] = xn[a] * xn[b];
+        }
+    }
+}
+
+constexpr int kWave = 64;
+
+// Core: builds C (rows owned by this lane), factors it and forward-solves
+// the Q right-hand sides held in y.  `rowx` are the coordinates of the
+// owned rows, `xy` the group's neighbour coordinates in LDS, `tg` the
+// target; y[0] is filled here with the covariance to the target.  Returns
+// false when a pivot falls below 1e-12 * c0 (the reference's singular
+// threshold relative to the matrix scale, linalg.rs:36-57, 135-140).
+template <int D, int NB, int G, int Q>
+__device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const int cnt,
+                                                   const double (&rowx)[NB / G][D],
+                                                   const double *xy, const double *tg,
+                                                   double *stage, const int lane, const int g,
+                                                   double (&y)[Q][NB / G]) {
+    constexpr int R = NB / G;
+    const double c0 = Mdl.c0;
+    double a[R][NB];
+    static_for<0, R>([&](auto RR) {
+        constexpr int r = decltype(RR)::value;
+        constexpr int ncol = G * (r + 1);
+        const int i = G * r + g;
+        for (int c = 0; c <= ncol; c++) {
+            double v;
+            double dh[D];
+            const bool tgt = (c == ncol);
+            const bool off = i < cnt && (tgt || c < i);
+            if (off) {
+#pragma unroll
+                for (int d = 0; d < D; d++)
+                    dh[d] = rowx[r][d] - (tgt ? tg[d] : xy[(c < NB ? c : NB - 1) * D + d]);
+                v = cov_offdiag<D>(Mdl, dh);
+            } else {
+                v = (c == i) ? c0 : 0.0;
+            }
+            stage[c * kWave + lane] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < ncol; c++) a[r][c] = stage[c * kWave + lane];
+        y[0][r] = stage[ncol * kWave + lane];
+    });
+
+    const double thresh = 1e-12 * c0;
+    bool ok = true;
+    static_for<0, NB>([&](auto JJ) {
+        constexpr int j = decltype(JJ)::value;
+        constexpr int oj = j % G;
+        constexpr int rj = j / G;
+        const double d = bcast<G, oj>(a[rj][j]);
+        ok = ok && (d > thresh);
+        const double inv = 1.0 / sqrt(d);
+        double l[R];
+#pragma unroll
+        for (int r = rj; r < R; r++) l[r] = a[r][j] * inv;
+        l[rj] = (g > oj) ? l[rj] : 0.0;
+        static_for<0, Q>([&](auto QQ) {
+            constexpr int qq = decltype(QQ)::value;
+            const double yj = bcast<G, oj>(y[qq][rj]) * inv;
+#pragma unroll
+            for (int r = rj; r < R; r++) y[qq][r] -= l[r] * yj;
+            y[qq][rj] = (g == oj) ? yj : y[qq][rj];
+        });
+        static_for<rj, R>([&](auto RK) {
+            constexpr int rk = decltype(RK)::value;
+            static_for<0, G>([&](auto GK) {
+                constexpr int gk = decltype(GK)::value;
+                constexpr int k = G * rk + gk;
+                if constexpr (k > j) {
+                    const double lk = bcast<G, gk>(l[rk]);
+#pragma unroll
+                    for (int r = rk; r < R; r++) a[r][k] -= l[r] * lk;
+                }
+            });
+        });
+    });
+    return ok;
+}
+
+template <int D, int NB, int G, int M>
+__global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
+    constexpr int R = NB / G;
+    constexpr int Q = 2 + M;
+    constexpr int GPW = kWave / G;
+    __shared__ double s_xy[GPW * NB * D];
+    __shared__ double s_stage[(NB + 1) * kWave];
+    const int lane = threadIdx.x;
+    const int g = lane % G;
+    const int grp = lane / G;
+    double *xy = s_xy + grp * NB * D;
+    const DevModel &Mdl = *P.model;
+    const double c0 = Mdl.c0;
+    const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
+
+    for (int64_t sys = (int64_t)blockIdx.x * GPW + grp; sys < nsys;
+         sys += (int64_t)gridDim.x * GPW) {
+        const int64_t t = P.list ? (int64_t)P.list[sys] : sys;
+        const int cnt = P.count[t];
+        if (cnt < P.min_nb || cnt < M || cnt > NB) {
+            if (g == 0) {
+                P.value[t] = __builtin_nan("");
+                P.variance[t] = __builtin_nan("");
+                if (P.lagrange) P.lagrange[t] = __builtin_nan("");
+                if (P.status)
+                    P.status[t] = cnt > NB ? 4 : (cnt < P.min_nb ? 5 : 1);
+            }
+            continue;
+        }
+        double tg[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) tg[d] = P.targets[t * D + d];
+        const int32_t *nbl = P.nbr + t * (int64_t)P.stride;
+        for (int j = g; j < cnt; j += G) {
+            const int64_t id = nbl[j];
+#pragma unroll
+            for (int d = 0; d < D; d++) xy[j * D + d] = P.coords[id * D + d];
+        }
+        double zr[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = G * r + g;
+            zr[r] = i < cnt ? P.values[nbl[i]] : 0.0;
+        }
+        wave_sync();
+        double rowx[R][D];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = G * r + g;
+#pragma unroll
+            for (int d = 0; d < D; d++) rowx[r][d] = i < cnt ? xy[i * D + d] : 0.0;
+        }
+        double y[Q][R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = G * r + g;
+            const bool live = i < cnt;
+            y[1][r] = live ? (M == 0 ? zr[r] - P.sk_mean : zr[r]) : 0.0;
+            if constexpr (M > 0) {
+                double f[M];
+                basis<D, M>(P, rowx[r], f);
+#pragma unroll
+                for (int q = 0; q < M; q++) y[2 + q][r] = live ? f[q] : 0.0;
+            }
+        }
+        bool ok = build_factor_solve<D, NB, G, Q>(Mdl, cnt, rowx, xy, tg, s_stage, lane, g, y);
+        wave_sync();  // the group's LDS rows are rewritten by its next system
+
+        double dq[Q][Q];
+        static_for<0, Q>([&](auto QA) {
+            constexpr int qa = decltype(QA)::value;
+            static_for<qa, Q>([&](auto QB) {
+                constexpr int qb = decltype(QB)::value;
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; r++) s += y[qa][r] * y[qb][r];
+                dq[qa][qb] = gsum<G>(s);
+            });
+        });
+
+        double value, var, lag;
+        if constexpr (M == 0) {
+            value = P.sk_mean + dq[0][1];
+            var = c0 - dq[0][0];
+            lag = __builtin_nan("");
+        } else {
+            double fT[M];
+            basis<D, M>(P, tg, fT);
+            double S[M][M], mu[M];
+#pragma unroll
+            for (int a = 0; a < M; a++) {
+#pragma unroll
+                for (int b = a; b < M; b++) S[a][b] = S[b][a] = dq[2 + a][2 + b];
+                mu[a] = dq[0][2 + a] - fT[a];
+            }
+            double smax = 0.0;
+#pragma unroll
+            for (int a = 0; a < M; a++) smax = fmax(smax, S[a][a]);
+            // unpivoted Cholesky of the m x m Schur complement, then solve
+#pragma unroll
+            for (int kk = 0; kk < M; kk++) {
+                double dd = S[kk][kk];
+#pragma unroll
+                for (int jj = 0; jj < kk; jj++) dd -= S[kk][jj] * S[kk][jj];
+                ok = ok && (dd > 1e-12 * smax);
+                dd = sqrt(dd);
+                S[kk][kk] = dd;
+#pragma unroll
+                for (int ii = kk + 1; ii < M; ii++) {
+                    double s = S[ii][kk];
+#pragma unroll
+                    for (int jj = 0; jj < kk; jj++) s -= S[ii][jj] * S[kk][jj];
+                    S[ii][kk] = s / dd;
+                }
+            }
+#pragma unroll
+            for (int ii = 0; ii < M; ii++) {
+                double s = mu[ii];
+#pragma unroll
+                for (int jj = 0; jj < ii; jj++) s -= S[ii][jj] * mu[jj];
+                mu[ii] = s / S[ii][ii];
+            }
+#pragma unroll
+            for (int ii = M - 1; ii >= 0; ii--) {
+                double s = mu[ii];
+#pragma unroll
+                for (int jj = ii + 1; jj < M; jj++) s -= S[jj][ii] * mu[jj];
+                mu[ii] = s / S[ii][ii];
+            }
+            double v = dq[0][1], wb = dq[0][0], mf = 0.0;
+#pragma unroll
+            for (int a = 0; a < M; a++) {
+                v -= mu[a] * dq[1][2 + a];
+                wb -= mu[a] * dq[0][2 + a];
+                mf += mu[a] * fT[a];
+            }
+            value = v;
+            var = c0 - (wb + mf);
+            lag = mu[0];
+        }
+        if (g == 0) {
+            if (ok) {
+                P.value[t] = value;
+                P.variance[t] = fmax(var, 0.0);
+                if (P.lagrange) P.lagrange[t] = lag;
+                if (P.status) P.status[t] = 0;
+            } else {
+                P.value[t] = __builtin_nan("");
+                P.variance[t] = __builtin_nan("");
+                if (P.lagrange) P.lagrange[t] = __builtin_nan("");
+                if (P.status) P.status[t] = 3;
+            }
+        }
+    }
+}
+
+// ---- Vecchia log-likelihood ---------------------------------------------
+// loglik_with_plan (vecchia.rs:285-336) for many models at once.  A group
+// handles one ordered point and loops over the models, reusing the
+// gathered conditioning coordinates.  Per-point terms are summed in a
+// fixed order (group-serial, then groups in index order inside the block,
+// then blocks in index order by vecchia_reduce), so the result does not
+// depend on scheduling.
+
+template <int D, int NB, int G>
+__global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
+    constexpr int R = NB / G;
+    constexpr int GPW = kWave / G;
+    __shared__ double s_xy[GPW * NB * D];
+    __shared__ double s_stage[(NB + 1) * kWave];
+    __shared__ double s_sum[GPW * kMaxModelsPerLaunch];
+    const int lane = threadIdx.x;
+    const int g = lane % G;
+    const int grp = lane / G;
+    double *xy = s_xy + grp * NB * D;
+    const int nm = P.n_models;
+    for (int p = g; p < nm; p += G) s_sum[grp * kMaxModelsPerLaunch + p] = 0.0;
+    wave_sync();
+    const double ln2pi = 1.8378770664093453;  // ln(2 pi)
+
+    const int64_t k0 = (int64_t)blockIdx.x * P.points_per_block;
+    int64_t k1 = k0 + P.points_per_block;
+    if (k1 > P.n) k1 = P.n;
+    for (int64_t k = k0 + grp; k < k1; k += GPW) {
+        const int64_t self = P.order[k];
+        const int cnt = P.nb_count[k];
+        const int32_t *nbl = P.nb + k * (int64_t)P.m;
+        double tg[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) tg[d] = P.coords[self * D + d];
+        const double zi = P.zc[self];
+        for (int j = g; j < cnt; j += G) {
+            const int64_t id = nbl[j];
+#pragma unroll
+            for (int d = 0; d < D; d++) xy[j * D + d] = P.coords[id * D + d];
+        }
+        double zr[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = G * r + g;
+            zr[r] = i < cnt ? P.zc[nbl[i]] : 0.0;
+        }
+        wave_sync();
+        double rowx[R][D];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = G * r + g;
+#pragma unroll
+            for (int d = 0; d < D; d++) rowx[r][d] = i < cnt ? xy[i * D + d] : 0.0;
+        }
+        for (int p = 0; p < nm; p++) {
+            const DevModel &Mdl = P.models[p];
+            const double sill = Mdl.c0;
+            double mu = 0.0, var = sill;
+            bool ok = true;
+            if (cnt > 0) {
+                double y[2][R];
+#pragma unroll
+                for (int r = 0; r < R; r++) y[1][r] = zr[r];
+                ok = build_factor_solve<D, NB, G, 2>(Mdl, cnt, rowx, xy, tg, s_stage, lane, g, y);
+                double s01 = 0.0, s00 = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    s01 += y[0][r] * y[1][r];
+                    s00 += y[0][r] * y[0][r];
+                }
+                mu = gsum<G>(s01);
+                const double red = gsum<G>(s00);
+                var = fmax(sill - red, 1e-12);
+            }
+            const double rr = zi - mu;
+            const double ll = -0.5 * (ln2pi + log(var) + rr * rr / var);
+            if (g == 0) {
+                s_sum[grp * kMaxModelsPerLaunch + p] += ll;
+                if (!ok) atomicMax(&P.status[p], 3);
+            }
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    for (int p = lane; p < nm; p += kWave) {
+        double s = 0.0;
+        for (int q = 0; q < GPW; q++) s += s_sum[q * kMaxModelsPerLaunch + p];
+        P.partial[(int64_t)blockIdx.x * nm + p] = s;
+    }
+}
+
+// ---- launchers (instantiated per template argument set) -------------------
+template <int D, int NB, int M>
+hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s) {
+    constexpr int G = group_size(NB);
+    constexpr int GPW = kWave / G;
+    int64_t blocks = (grid_groups + GPW - 1) / GPW;
+    if (blocks < 1) blocks = 1;
+    if (blocks > (1 << 30)) blocks = 1 << 30;
+    hipLaunchKernelGGL((krige_fp64_kernel<D, NB, G, M>), dim3((unsigned)blocks), dim3(kWave), 0,
+                       s, p);
+    return hipGetLastError();
+}
+
+template <int D, int NB>
+hipError_t launch_vecchia_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s) {
+    constexpr int G = group_size(NB);
+    hipLaunchKernelGGL((vecchia_kernel<D, NB, G>), dim3((unsigned)n_blocks), dim3(kWave), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,532 @@
+"""geostat_hip — Python mirror of geostat-core's predictor / likelihood API
+over the MI355X C ABI (include/geostat_hip.h).
+
+Names, argument meaning and error behaviour follow the reference
+(franciscoparrao/geostat-rs, crates/geostat-core/src/):
+
+* ``Kriging(coords, values, model, config)``  — Kriging::new (kriging.rs:166)
+* ``Kriging.predict / predict_many / predict_grid`` — kriging.rs:494, 648, 783
+* ``Kriging.neighbors``                        — kriging.rs:441-490
+* ``vecchia_plan`` / ``VecchiaPlan.loglik``    — vecchia.rs:232, 285
+* ``vecchia_loglik``                           — vecchia.rs:874
+* ``VariogramModel`` / ``Structure`` / ``Anisotropy`` / ``ModelKind`` — variogram/model.rs
+* ``GeostatError`` and subclasses              — error.rs:15-40
+
+Build-time problems raise; per-target problems give NaN estimates
+(kriging.rs:130-134, 648-651).  Every call runs on the GPU through
+libgeostat_hip.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import GS_NONE, PREC_FP32_REFINED, PREC_FP64
+
+__all__ = [
+    "GeostatError", "InsufficientData", "DimensionMismatch", "SingularSystem",
+    "InvalidParameter", "NoNeighbors", "DuplicatePoints", "DeviceError",
+    "ModelKind", "Anisotropy", "Structure", "VariogramModel", "KrigingMethod",
+    "KrigingConfig", "KrigingEstimate", "Context", "Kriging", "Grid2D", "Grid3D",
+    "VecchiaPlan", "vecchia_plan", "vecchia_loglik", "PREC_FP64", "PREC_FP32_REFINED",
+]
+
+
+# ---------------------------------------------------------------- errors --
+class GeostatError(Exception):
+    code = -1
+
+
+class InsufficientData(GeostatError):
+    code = 1
+
+
+class DimensionMismatch(GeostatError):
+    code = 2
+
+
+class SingularSystem(GeostatError):
+    code = 3
+
+
+class InvalidParameter(GeostatError):
+    code = 4
+
+
+class NoNeighbors(GeostatError):
+    code = 5
+
+
+class DuplicatePoints(GeostatError):
+    code = 6
+
+    def __init__(self, i: int, j: int, msg: str = ""):
+        super().__init__(msg or f"duplicate points {i}, {j}")
+        self.i, self.j = i, j
+
+
+class DeviceError(GeostatError):
+    code = 100
+
+
+_ERRORS = {c.code: c for c in (InsufficientData, DimensionMismatch, SingularSystem,
+                               InvalidParameter, NoNeighbors, DeviceError)}
+
+
+def _check(status: int, dup=None) -> None:
+    if status == 0:
+        return
+    msg = _abi.last_error()
+    if status == DuplicatePoints.code:
+        raise DuplicatePoints(int(dup[0]), int(dup[1]), msg)
+    raise _ERRORS.get(status, DeviceError)(msg)
+
+
+# ------------------------------------------------------------------ model --
+class ModelKind:
+    """ModelKind (model.rs:47-95); parametrised kinds are (kind, param) pairs."""
+    SPHERICAL = (0, 0.0)
+    EXPONENTIAL = (1, 0.0)
+    GAUSSIAN = (2, 0.0)
+    MATERN15 = (3, 0.0)
+    MATERN25 = (4, 0.0)
+    CIRCULAR = (6, 0.0)
+    HOLE = (8, 0.0)
+    WAVE = (9, 0.0)
+
+    @staticmethod
+    def Matern(nu: float):
+        return (5, float(nu))
+
+    @staticmethod
+    def Stable(alpha: float):
+        return (7, float(alpha))
+
+    @staticmethod
+    def Power(theta: float):
+        return (10, float(theta))
+
+
+@dataclass
+class Anisotropy:
+    """Anisotropy (model.rs:321-341)."""
+    azimuth_deg: float
+    ratio: float
+    ratio_z: float = 1.0
+    dip_deg: float = 0.0
+    rake_deg: float = 0.0
+
+    def _c(self) -> _abi.gs_anisotropy:
+        return _abi.gs_anisotropy(self.azimuth_deg, self.ratio, self.ratio_z, self.dip_deg,
+                                  self.rake_deg)
+
+
+@dataclass
+class Structure:
+    """Structure (model.rs:438-449)."""
+    kind: tuple
+    sill: float
+    range: float
+    anis: Optional[Anisotropy] = None
+
+    @classmethod
+    def new(cls, kind, sill, rng):
+        return cls(kind, sill, rng)
+
+    @classmethod
+    def with_anisotropy(cls, kind, sill, rng, azimuth_deg, ratio):
+        return cls(kind, sill, rng, Anisotropy(azimuth_deg, ratio))
+
+    @classmethod
+    def with_rotation(cls, kind, sill, rng, azimuth_deg, dip_deg, rake_deg, ratio, ratio_z):
+        return cls(kind, sill, rng, Anisotropy(azimuth_deg, ratio, ratio_z, dip_deg, rake_deg))
+
+
+@dataclass
+class VariogramModel:
+    """VariogramModel (model.rs:543-549): nugget + nested structures."""
+    nugget: float
+    structures: list = field(default_factory=list)
+
+    def total_sill(self) -> float:
+        s = -0.0
+        for st in self.structures:
+            s += st.sill
+        return self.nugget + s
+
+    def _c(self) -> _abi.gs_model:
+        if len(self.structures) > _abi.GS_MAX_STRUCTURES:
+            raise InvalidParameter("unsupported on GPU: more than 8 nested structures")
+        m = _abi.gs_model()
+        m.nugget = self.nugget
+        m.n_structures = len(self.structures)
+        for i, s in enumerate(self.structures):
+            cs = m.structures[i]
+            cs.kind, cs.param = s.kind
+            cs.sill = s.sill
+            cs.range = s.range
+            cs.has_anis = 1 if s.anis is not None else 0
+            if s.anis is not None:
+                cs.anis = s.anis._c()
+        return m
+
+
+# ----------------------------------------------------------------- config --
+@dataclass
+class KrigingMethod:
+    """KrigingMethod (kriging.rs:25-44): kind 0 simple, 1 ordinary, 2 universal."""
+    kind: int = 1
+    mean: float = 0.0
+    degree: int = 1
+
+    @staticmethod
+    def Simple(mean: float):
+        return KrigingMethod(0, mean=mean)
+
+    @staticmethod
+    def Ordinary():
+        return KrigingMethod(1)
+
+    @staticmethod
+    def Universal(degree: int):
+        return KrigingMethod(2, degree=degree)
+
+
+@dataclass
+class KrigingConfig:
+    """KrigingConfig (kriging.rs:73-115); None = not set."""
+    method: KrigingMethod = field(default_factory=KrigingMethod.Ordinary)
+    max_neighbors: Optional[int] = None
+    search_radius: Optional[float] = None
+    min_neighbors: Optional[int] = None
+    max_per_octant: Optional[int] = None
+    anisotropic_search: Optional[Anisotropy] = None
+
+    def _c(self) -> _abi.gs_kconfig:
+        c = _abi.gs_kconfig()
+        c.method = self.method.kind
+        c.simple_mean = self.method.mean
+        c.degree = self.method.degree
+        c.max_neighbors = GS_NONE if self.max_neighbors is None else int(self.max_neighbors)
+        c.min_neighbors = GS_NONE if self.min_neighbors is None else int(self.min_neighbors)
+        c.max_per_octant = GS_NONE if self.max_per_octant is None else int(self.max_per_octant)
+        c.has_search_radius = 0 if self.search_radius is None else 1
+        c.search_radius = 0.0 if self.search_radius is None else float(self.search_radius)
+        c.has_anisotropic_search = 0 if self.anisotropic_search is None else 1
+        if self.anisotropic_search is not None:
+            c.anisotropic_search = self.anisotropic_search._c()
+        return c
+
+
+@dataclass
+class KrigingEstimate:
+    """KrigingEstimate (kriging.rs:118-128)."""
+    value: float
+    variance: float
+    lagrange: Optional[float]
+
+
+# ---------------------------------------------------------------- context --
+class Context:
+    """A device plus its default stream (gs_ctx)."""
+    _default = None
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(_abi.lib().gs_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @classmethod
+    def default(cls) -> "Context":
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def synchronize(self) -> None:
+        _check(_abi.lib().gs_ctx_synchronize(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi._lib is not None:
+            _abi.lib().gs_ctx_destroy(h)
+            self._h = None
+
+
+def _f64(a, shape_last=None) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    return a
+
+
+_PREC = {"fp64": PREC_FP64, "fp32r": PREC_FP32_REFINED, PREC_FP64: PREC_FP64,
+         PREC_FP32_REFINED: PREC_FP32_REFINED}
+
+
+# ---------------------------------------------------------------- kriging --
+class Kriging:
+    """Kriging::<D, VariogramModel>::new (kriging.rs:166) bound to a device.
+
+    coords: (n, D) array; values: (n,).  Data and the search index stay
+    resident in device memory until the object is freed.
+    """
+
+    def __init__(self, coords, values, model: VariogramModel,
+                 config: Optional[KrigingConfig] = None, ctx: Optional[Context] = None):
+        coords = _f64(coords)
+        values = _f64(values).reshape(-1)
+        if coords.ndim != 2 or coords.shape[1] not in (2, 3):
+            raise DimensionMismatch(f"coords must be (n, 2|3), got {coords.shape}")
+        if coords.shape[0] != values.shape[0]:
+            raise DimensionMismatch(f"{coords.shape[0]} coordinates vs {values.shape[0]} values")
+        self.ctx = ctx or Context.default()
+        self.dim = coords.shape[1]
+        self.n = coords.shape[0]
+        self.model = model
+        self.config = config or KrigingConfig()
+        cm = model._c()
+        cc = self.config._c()
+        h = C.c_void_p()
+        dup = np.zeros(2, dtype=np.int64)
+        st = _abi.lib().gs_kriging_new(self.ctx._h, self.dim, _abi.dptr(coords),
+                                       _abi.dptr(values), self.n, C.byref(cm), C.byref(cc),
+                                       C.byref(h), _abi.i64ptr(dup))
+        _check(st, dup)
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi._lib is not None:
+            _abi.lib().gs_kriging_free(h)
+            self._h = None
+
+    @property
+    def max_neighbors(self) -> int:
+        return int(_abi.lib().gs_kriging_max_neighbors(self._h))
+
+    def predict_many(self, targets, precision="fp64"):
+        """predict_many (kriging.rs:648-652): arrays (value, variance,
+        lagrange, status); failed targets are NaN with a non-zero status."""
+        t = _f64(targets).reshape(-1, self.dim)
+        T = t.shape[0]
+        v = np.empty(T)
+        var = np.empty(T)
+        lag = np.empty(T)
+        st = np.empty(T, dtype=np.int32)
+        _check(_abi.lib().gs_kriging_predict_many(self._h, _abi.dptr(t), T, _PREC[precision],
+                                                  _abi.dptr(v), _abi.dptr(var), _abi.dptr(lag),
+                                                  _abi.i32ptr(st)))
+        return v, var, lag, st
+
+    def predict(self, target, precision="fp64") -> KrigingEstimate:
+        """predict (kriging.rs:494-501): raises the per-target error."""
+        v, var, lag, st = self.predict_many(np.asarray(target, dtype=np.float64).reshape(1, -1),
+                                            precision)
+        if st[0] != 0:
+            raise _ERRORS.get(int(st[0]), GeostatError)(f"target failed with status {st[0]}")
+        lg = None if self.config.method.kind == 0 else float(lag[0])
+        return KrigingEstimate(float(v[0]), float(var[0]), lg)
+
+    def predict_grid(self, grid: "Grid2D", precision="fp64"):
+        """predict_grid (kriging.rs:783-786): (values, variances) in
+        storage order iy*nx + ix."""
+        v, var, _, _ = self.predict_many(grid.centers(), precision)
+        return v, var
+
+    def predict_many_device(self, targets, value, variance, lagrange=None, status=None,
+                            precision="fp64", stream=None):
+        """Enqueues predict_many on device tensors (torch CUDA tensors or raw
+        pointers); returns immediately."""
+        def ptr(x):
+            if x is None:
+                return None
+            return x if isinstance(x, int) else x.data_ptr()
+        T = targets.shape[0] if hasattr(targets, "shape") else None
+        if T is None:
+            raise InvalidParameter("targets must have a shape")
+        _check(_abi.lib().gs_kriging_predict_many_device(
+            self._h, ptr(targets), T, _PREC[precision], ptr(value), ptr(variance),
+            ptr(lagrange), ptr(status), stream))
+
+    def set_timing(self, enable: bool) -> None:
+        """Records HIP events around the search and solve phases of every
+        subsequent predict call (measurement hook)."""
+        _check(_abi.lib().gs_kriging_set_timing(self._h, 1 if enable else 0))
+
+    def timing_summary(self):
+        """(calls, total search ms, total solve ms) since the last summary."""
+        calls = C.c_int64()
+        a = C.c_double()
+        b = C.c_double()
+        _check(_abi.lib().gs_kriging_timing_summary(self._h, C.byref(calls), C.byref(a),
+                                                    C.byref(b)))
+        return int(calls.value), float(a.value), float(b.value)
+
+    def neighbors(self, targets):
+        """Kriging::neighbors (kriging.rs:441-490) for each target: (idx, count)
+        with idx an (T, max_neighbors) int32 array padded with -1."""
+        t = _f64(targets).reshape(-1, self.dim)
+        T = t.shape[0]
+        k = max(self.max_neighbors, 1)
+        idx = np.empty((T, k), dtype=np.int32)
+        cnt = np.empty(T, dtype=np.int32)
+        _check(_abi.lib().gs_kriging_neighbors(self._h, _abi.dptr(t), T, k, _abi.i32ptr(idx),
+                                               _abi.i32ptr(cnt)))
+        return idx, cnt
+
+
+# ------------------------------------------------------------------ grids --
+@dataclass
+class Grid2D:
+    """Grid2D (grid.rs:11-96): cell centres x0 + (ix + 0.5) dx, order iy*nx+ix."""
+    x0: float
+    y0: float
+    dx: float
+    dy: float
+    nx: int
+    ny: int
+
+    @classmethod
+    def from_bbox(cls, mn, mx, nx, ny):
+        if nx == 0 or ny == 0:
+            raise InvalidParameter("grid must have at least one cell per axis")
+        if not (mx[0] > mn[0]) or not (mx[1] > mn[1]):
+            raise InvalidParameter("degenerate bbox")
+        return cls(mn[0], mn[1], (mx[0] - mn[0]) / nx, (mx[1] - mn[1]) / ny, nx, ny)
+
+    def n_cells(self) -> int:
+        return self.nx * self.ny
+
+    def centers(self) -> np.ndarray:
+        ix = np.arange(self.nx, dtype=np.float64)
+        iy = np.arange(self.ny, dtype=np.float64)
+        cx = self.x0 + (ix + 0.5) * self.dx
+        cy = self.y0 + (iy + 0.5) * self.dy
+        out = np.empty((self.ny, self.nx, 2))
+        out[:, :, 0] = cx[None, :]
+        out[:, :, 1] = cy[:, None]
+        return out.reshape(-1, 2)
+
+
+@dataclass
+class Grid3D:
+    """Grid3D (grid.rs:127-202): x fastest, then y, then z."""
+    x0: float
+    y0: float
+    z0: float
+    dx: float
+    dy: float
+    dz: float
+    nx: int
+    ny: int
+    nz: int
+
+    @classmethod
+    def from_bbox(cls, mn, mx, nx, ny, nz):
+        if nx == 0 or ny == 0 or nz == 0:
+            raise InvalidParameter("grid must have at least one cell per axis")
+        for d in range(3):
+            if not (mx[d] > mn[d]):
+                raise InvalidParameter("degenerate bbox")
+        return cls(mn[0], mn[1], mn[2], (mx[0] - mn[0]) / nx, (mx[1] - mn[1]) / ny,
+                   (mx[2] - mn[2]) / nz, nx, ny, nz)
+
+    def n_cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def centers(self) -> np.ndarray:
+        cx = self.x0 + (np.arange(self.nx, dtype=np.float64) + 0.5) * self.dx
+        cy = self.y0 + (np.arange(self.ny, dtype=np.float64) + 0.5) * self.dy
+        cz = self.z0 + (np.arange(self.nz, dtype=np.float64) + 0.5) * self.dz
+        out = np.empty((self.nz, self.ny, self.nx, 3))
+        out[..., 0] = cx[None, None, :]
+        out[..., 1] = cy[None, :, None]
+        out[..., 2] = cz[:, None, None]
+        return out.reshape(-1, 3)
+
+
+# ---------------------------------------------------------------- Vecchia --
+class VecchiaPlan:
+    """VecchiaPlan (vecchia.rs:223-228) resident on the device."""
+
+    def __init__(self, coords, m: int, order=None, ctx: Optional[Context] = None):
+        coords = _f64(coords)
+        if coords.ndim != 2 or coords.shape[1] not in (2, 3):
+            raise DimensionMismatch(f"coords must be (n, 2|3), got {coords.shape}")
+        self.ctx = ctx or Context.default()
+        self.dim = coords.shape[1]
+        self.n = coords.shape[0]
+        self.m = int(m)
+        o = None
+        if order is not None:
+            o = np.ascontiguousarray(np.asarray(order, dtype=np.int64))
+            if o.shape[0] != self.n:
+                raise DimensionMismatch(f"order has {o.shape[0]} entries for {self.n} points")
+        h = C.c_void_p()
+        _check(_abi.lib().gs_vecchia_plan_new(self.ctx._h, self.dim, _abi.dptr(coords), self.n,
+                                              self.m, None if o is None else _abi.i64ptr(o),
+                                              C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi._lib is not None:
+            _abi.lib().gs_vecchia_plan_free(h)
+            self._h = None
+
+    def arrays(self):
+        """(order, neighbours[n, m] padded with -1, counts)."""
+        order = np.empty(self.n, dtype=np.int64)
+        nb = np.empty((self.n, self.m), dtype=np.int32)
+        cnt = np.empty(self.n, dtype=np.int32)
+        _check(_abi.lib().gs_vecchia_plan_get(self._h, _abi.i64ptr(order), _abi.i32ptr(nb),
+                                              _abi.i32ptr(cnt)))
+        return order, nb, cnt
+
+    @property
+    def order(self):
+        return self.arrays()[0]
+
+    def loglik_batch(self, values, models: Sequence[VariogramModel], precision="fp64"):
+        """Batched loglik_with_plan (vecchia.rs:285-336): (loglik, status)
+        arrays; status 3 where the reference returns Err(SingularSystem)."""
+        v = _f64(values).reshape(-1)
+        if v.shape[0] != self.n:
+            raise DimensionMismatch(f"{v.shape[0]} values for {self.n} points")
+        arr = (_abi.gs_model * len(models))(*[mm._c() for mm in models])
+        out = np.empty(len(models))
+        st = np.empty(len(models), dtype=np.int32)
+        _check(_abi.lib().gs_vecchia_loglik_batch(self._h, _abi.dptr(v), arr, len(models),
+                                                  _PREC[precision], _abi.dptr(out),
+                                                  _abi.i32ptr(st)))
+        return out, st
+
+    def loglik(self, values, model: VariogramModel, precision="fp64") -> float:
+        out, st = self.loglik_batch(values, [model], precision)
+        if st[0] != 0:
+            raise SingularSystem("singular conditioning system")
+        return float(out[0])
+
+
+def vecchia_plan(coords, m: int, order=None, ctx: Optional[Context] = None) -> VecchiaPlan:
+    """vecchia_plan (vecchia.rs:232-282)."""
+    return VecchiaPlan(coords, m, order, ctx)
+
+
+def vecchia_loglik(coords, values, model: VariogramModel, m: int, order=None,
+                   ctx: Optional[Context] = None) -> float:
+    """vecchia_loglik (vecchia.rs:874-884)."""
+    ctx = ctx or Context.default()
+    coords = _f64(coords)
+    values = _f64(values).reshape(-1)
+    o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
+    cm = model._c()
+    out = C.c_double()
+    _check(_abi.lib().gs_vecchia_loglik(ctx._h, coords.shape[1], _abi.dptr(coords),
+                                        _abi.dptr(values), coords.shape[0], C.byref(cm), int(m),
+                                        None if o is None else _abi.i64ptr(o), C.byref(out)))
+    return float(out.value)

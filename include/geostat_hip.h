@@ -1,0 +1,188 @@
+/*
+ * geostat_hip.h — C ABI of libgeostat_hip.so, the MI355X (gfx950) moving-
+ * neighbourhood kriging predictor and Vecchia log-likelihood.
+ *
+ * This is the drop-in boundary for geostat-core's rayon `parallel` path:
+ * every entry point below names the reference interface it replaces
+ * (franciscoparrao/geostat-rs, paths under crates/geostat-core/src/).
+ *
+ * Conventions
+ *  - The caller owns every host buffer; handles own their device memory
+ *    until *_free.  No C++ exception crosses this boundary.
+ *  - Return values are gs_status codes.  Codes 1..6 mirror GeostatError
+ *    (error.rs:15-40); gs_last_error() returns a thread-local message.
+ *  - Per-target failures are NaN estimates plus a per-target status code,
+ *    never a call failure (kriging.rs:130-134, 648-651).
+ *  - "None" in an Option<usize> config field is GS_NONE (-1).
+ *  - Unsupported configurations return GS_ERR_INVALID_PARAMETER with a
+ *    message starting "unsupported on GPU"; they never diverge silently.
+ */
+#ifndef GEOSTAT_HIP_H
+#define GEOSTAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_ABI_VERSION 1
+#define GS_MAX_STRUCTURES 8
+#define GS_NONE (-1)
+
+typedef enum {
+    GS_OK = 0,
+    GS_ERR_INSUFFICIENT_DATA = 1,   /* GeostatError::InsufficientData */
+    GS_ERR_DIMENSION_MISMATCH = 2,  /* GeostatError::DimensionMismatch */
+    GS_ERR_SINGULAR_SYSTEM = 3,     /* GeostatError::SingularSystem */
+    GS_ERR_INVALID_PARAMETER = 4,   /* GeostatError::InvalidParameter */
+    GS_ERR_NO_NEIGHBORS = 5,        /* GeostatError::NoNeighbors */
+    GS_ERR_DUPLICATE_POINTS = 6,    /* GeostatError::DuplicatePoints */
+    GS_ERR_DEVICE = 100,            /* HIP runtime failure */
+    GS_ERR_OUT_OF_MEMORY = 101      /* device allocation failed */
+} gs_status;
+
+/* ModelKind (variogram/model.rs:47-95).  `param` carries nu / alpha / theta. */
+typedef enum {
+    GS_SPHERICAL = 0, GS_EXPONENTIAL = 1, GS_GAUSSIAN = 2, GS_MATERN15 = 3,
+    GS_MATERN25 = 4, GS_MATERN = 5, GS_CIRCULAR = 6, GS_STABLE = 7,
+    GS_HOLE = 8, GS_WAVE = 9, GS_POWER = 10
+} gs_kind;
+
+/* KrigingMethod (kriging.rs:25-44); external drift is not exposed. */
+typedef enum { GS_SIMPLE = 0, GS_ORDINARY = 1, GS_UNIVERSAL = 2 } gs_method;
+
+/* Arithmetic of the covariance build and solve. */
+typedef enum {
+    GS_PREC_FP64 = 0,          /* FP64 throughout */
+    GS_PREC_FP32_REFINED = 1   /* FP32 factorization + FP64-residual refinement */
+} gs_precision;
+
+/* Anisotropy (variogram/model.rs:321-341). */
+typedef struct {
+    double azimuth_deg, ratio, ratio_z, dip_deg, rake_deg;
+} gs_anisotropy;
+
+/* Structure (variogram/model.rs:438-449). */
+typedef struct {
+    int32_t kind;        /* gs_kind */
+    double param;        /* Matern nu, Stable alpha, Power theta; else ignored */
+    double sill;
+    double range;
+    int32_t has_anis;
+    gs_anisotropy anis;
+} gs_structure;
+
+/* VariogramModel (variogram/model.rs:543-549). */
+typedef struct {
+    double nugget;
+    int32_t n_structures;
+    gs_structure structures[GS_MAX_STRUCTURES];
+} gs_model;
+
+/* KrigingConfig (kriging.rs:73-115). */
+typedef struct {
+    int32_t method;            /* gs_method */
+    double simple_mean;        /* KrigingMethod::Simple { mean } */
+    int32_t degree;            /* KrigingMethod::Universal { degree } */
+    int64_t max_neighbors;     /* Option<usize>: GS_NONE or >= 0 */
+    int64_t min_neighbors;     /* Option<usize> */
+    int64_t max_per_octant;    /* Option<usize> */
+    int32_t has_search_radius;
+    double search_radius;
+    int32_t has_anisotropic_search;
+    gs_anisotropy anisotropic_search;
+} gs_kconfig;
+
+typedef struct gs_ctx gs_ctx;
+typedef struct gs_kriging gs_kriging;
+typedef struct gs_vplan gs_vplan;
+
+/* ---- context ------------------------------------------------------ */
+
+/* Opens `device` and creates the context's default stream.  (No reference
+ * counterpart: the rayon path has no device state.) */
+int32_t gs_ctx_create(int32_t device, gs_ctx **out);
+void gs_ctx_destroy(gs_ctx *ctx);
+int32_t gs_ctx_synchronize(gs_ctx *ctx);
+/* Thread-local message of the last failing call on this thread. */
+const char *gs_last_error(void);
+int32_t gs_abi_version(void);
+
+/* ---- kriging ------------------------------------------------------ */
+
+/* Kriging::<D, VariogramModel>::new (kriging.rs:166-173 → build 262-408):
+ * validates data, model and config, rejects duplicate locations
+ * (data.rs:150-163; the first pair goes to dup_pair), builds the search
+ * index on the (optionally anisotropic) search frame and keeps data and
+ * index resident in HBM.  coords: n*dim AoS doubles; values: n doubles. */
+int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                       int64_t n, const gs_model *model, const gs_kconfig *config,
+                       gs_kriging **out, int64_t dup_pair[2]);
+void gs_kriging_free(gs_kriging *k);
+
+/* Kriging::predict_many (kriging.rs:648-652) over host buffers: targets is
+ * T*dim AoS doubles; outputs have T entries each.  lagrange is NaN where the
+ * reference returns None; status holds the per-target gs_status (0 = ok;
+ * NaN estimate otherwise).  lagrange and status may be NULL. */
+int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T,
+                                int32_t precision, double *value, double *variance,
+                                double *lagrange, int32_t *status);
+
+/* Same as gs_kriging_predict_many with every buffer already in device
+ * memory, enqueued on `stream` (a hipStream_t; NULL = the context stream).
+ * Asynchronous: returns after enqueueing.  Workspace comes from the
+ * handle's cache, so calls on one handle must not overlap. */
+int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, int64_t T,
+                                       int32_t precision, double *d_value, double *d_variance,
+                                       double *d_lagrange, int32_t *d_status, void *stream);
+
+/* Kriging::neighbors (kriging.rs:441-490) for T targets: idx is T*max_out
+ * int32 (original data indices in the reference's order, -1 padded),
+ * count has T entries (the neighbourhood size before any min_neighbors
+ * check).  max_out must be >= the configured neighbourhood bound. */
+int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, int64_t max_out,
+                             int32_t *idx, int32_t *count);
+
+/* Largest neighbourhood one target can receive under this handle's config. */
+int64_t gs_kriging_max_neighbors(const gs_kriging *k);
+
+/* Per-phase device timing (no reference counterpart; a measurement hook).
+ * While enabled, every predict call on the handle records HIP events on its
+ * launch stream around the neighbour search and around the solves.
+ * gs_kriging_timing_summary waits for the recorded events, returns the
+ * number of calls and the summed milliseconds of each phase since the last
+ * summary, and clears the record. */
+int32_t gs_kriging_set_timing(gs_kriging *k, int32_t enable);
+int32_t gs_kriging_timing_summary(gs_kriging *k, int64_t *calls, double *ms_search,
+                                  double *ms_solve);
+
+/* ---- Vecchia ------------------------------------------------------ */
+
+/* vecchia_plan (vecchia.rs:232-282): maxmin ordering (order == NULL) or the
+ * given permutation, then each ordered point's m nearest predecessors. */
+int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int64_t n, int64_t m,
+                            const int64_t *order, gs_vplan **out);
+void gs_vecchia_plan_free(gs_vplan *p);
+/* Copies the plan out: order (n), nb (n*m, row k = ordered point k, -1
+ * padded, original indices), nb_count (n).  Any pointer may be NULL. */
+int32_t gs_vecchia_plan_get(const gs_vplan *p, int64_t *order, int32_t *nb, int32_t *nb_count);
+
+/* Batched loglik_with_plan (vecchia.rs:285-336) for n_models models over
+ * one plan (the reference's plan-reusing loglik is private; this is the
+ * batched extension its MLE loop needs).  out[i] is model i's
+ * log-likelihood; status[i] is GS_ERR_SINGULAR_SYSTEM when any conditional
+ * of model i fails (the reference's whole-call Err), GS_OK otherwise. */
+int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_model *models,
+                                int32_t n_models, int32_t precision, double *out,
+                                int32_t *status);
+
+/* vecchia_loglik (vecchia.rs:874-884): plan + one likelihood. */
+int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                          int64_t n, const gs_model *model, int64_t m, const int64_t *order,
+                          double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

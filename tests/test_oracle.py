@@ -1,0 +1,471 @@
+"""Pins the CPU oracle (oracle/refcpu.c) before it is trusted as the checker.
+
+Each test names the reference test or the mathematical fact it pins.
+Reference constants (gstat, R besselK, published xoshiro vectors) are
+copied as data.  Paths are under crates/geostat-core/src/ unless stated.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.stats import multivariate_normal
+
+import _cases as cs
+from geostat_hip import (Anisotropy, KrigingConfig, KrigingMethod, ModelKind, Structure,
+                         VariogramModel)
+
+
+# ---------------------------------------------------------------- rng.rs --
+def test_xoshiro256pp_reference_vector(oracle):
+    # Published xoshiro256++ outputs for state s = [1, 2, 3, 4] (the
+    # reference implementation's test vector, as used by rand_xoshiro).
+    r = oracle.Rng(0)
+    for i, v in enumerate([1, 2, 3, 4]):
+        r._r.s[i] = v
+    want = [41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+            9973669472204895162, 14011001112246962877, 12406186145184390807,
+            15849039046786891736, 10450023813501588000]
+    assert [r.next_u64() for _ in range(10)] == want
+
+
+def test_splitmix64_seeding_known_value(oracle):
+    # splitmix64 from state 0 first outputs 0xE220A8397B1DCDAF: Rng::new(0)
+    # takes it as s[0] (rng.rs:25-37).
+    r = oracle.Rng(0)
+    assert r._r.s[0] == 0xE220A8397B1DCDAF
+
+
+def test_rng_uniform_and_normal_moments(oracle):
+    # rng.rs:99-127 uniform_in_unit_interval / normal_moments
+    r = oracle.Rng(1)
+    u = np.array([r.uniform() for _ in range(10000)])
+    assert (u >= 0).all() and (u < 1).all()
+    r = oracle.Rng(7)
+    z = np.array([r.normal() for _ in range(100000)])
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1.0) < 0.05
+
+
+def test_product_generator_matches_oracle_stream(oracle):
+    # The bench's generator (geostat_hip.workloads) and the oracle Rng are
+    # independent implementations of rng.rs; the engine.rs recipe drawn
+    # through both must agree bit for bit.
+    from geostat_hip import workloads
+    r = oracle.Rng(123)
+    raw = [r.next_u64() for _ in range(64)]
+    assert list(map(int, workloads.raw_u64(123, 64))) == raw
+    c, v = workloads.points(42, 300, [1000.0, 1000.0])
+    r = oracle.Rng(42)
+    for i in range(300):
+        x = r.uniform() * 1000.0
+        y = r.uniform() * 1000.0
+        val = math.sin(x / 150.0) + math.cos(y / 200.0) + 0.3 * r.normal()
+        assert c[i, 0] == x and c[i, 1] == y and v[i] == val
+
+
+# ------------------------------------------------------ variogram/bessel.rs --
+def test_gamma_matches_known_values(oracle):
+    # bessel.rs:129-146
+    for x, want in [(0.5, math.sqrt(math.pi)), (1.0, 1.0), (2.0, 1.0), (3.0, 2.0),
+                    (4.5, 11.6317283965674), (0.1, 9.51350769866873)]:
+        assert abs((oracle.gamma_fn(x) - want) / want) < 1e-9
+
+
+def test_bessel_k_matches_r_besselk(oracle):
+    # bessel.rs:148-165 (R besselK values)
+    for nu, x, want in [(0.5, 1.0, 0.461068504447895), (1.5, 1.0, 0.922137008895789),
+                        (2.5, 2.0, 0.3897977588962), (0.3, 5.0, 0.00372166932887342),
+                        (4.0, 10.0, 3.7861437160892e-5)]:
+        assert abs((oracle.bessel_k(nu, x) - want) / want) < 1e-8
+
+
+# ------------------------------------------------------- variogram/model.rs --
+def test_gamma_boundary_behavior(oracle):
+    # model.rs:714-727
+    m = VariogramModel(0.1, [Structure(ModelKind.SPHERICAL, 0.9, 100.0)])
+    assert oracle.gamma_dh(m, [0.0, 0.0]) == 0.0
+    assert oracle.gamma_dh(m, [1e-9, 0.0]) >= 0.1
+    assert abs(oracle.gamma_dh(m, [100.0, 0.0]) - 1.0) < 1e-12
+    assert abs(oracle.gamma_dh(m, [0.0, 500.0]) - 1.0) < 1e-12
+
+
+def test_rotation_3d_matches_gstat_setrot(oracle):
+    # model.rs:911-952: gstat variogramLine with anis = c(30, 20, 40, 0.5, 0.3)
+    m = VariogramModel(0.0, [Structure(ModelKind.SPHERICAL, 1.0, 100.0,
+                                       Anisotropy(30.0, 0.5, 0.3, 20.0, 40.0))])
+    cases = [([1.0, 0.0, 0.0], 0.605458280770632), ([0.0, 1.0, 0.0], 0.577390658041395),
+             ([0.0, 0.0, 1.0], 0.732990706603906),
+             ([0.577350269190, 0.577350269190, 0.577350269190], 0.432164835884142),
+             ([0.666666666667, -0.666666666667, 0.333333333333], 0.810043758779196),
+             ([-0.534522483825, 0.267261241912, 0.801783725737], 0.575462768274161)]
+    for d, want in cases:
+        assert abs(oracle.gamma_dh(m, [d[0] * 20, d[1] * 20, d[2] * 20]) - want) < 1e-9
+
+
+def test_new_families_match_gstat(oracle):
+    # model.rs:1131-1207 (gstat Cir / Hol / Wav / Exc values at range 100)
+    hs = [1.0, 10.0, 30.0, 50.0, 80.0, 100.0, 150.0, 200.0, 300.0, 500.0]
+    cir = [0.01273218, 0.12711143, 0.37616234, 0.60899778, 0.89591196, 1, 1, 1, 1, 1]
+    hol = [1.666658e-05, 1.665834e-03, 1.493264e-02, 4.114892e-02, 1.033049e-01, 1.585290e-01,
+           3.350033e-01, 5.453513e-01, 9.529600e-01, 1.191785e+00]
+    wav = [0.0001644853, 0.0163683569, 0.1416063087, 0.3633802276, 0.7661276791, 1.0,
+           1.2122065908, 1.0, 1.0, 1.0]
+    exc = [0.0009995002, 0.0311280057, 0.1515267892, 0.2978114987, 0.5110728376, 0.6321205588,
+           0.8407240915, 0.9408942534, 0.9944621693, 0.9999860543]
+    for i, h in enumerate(hs):
+        assert abs(oracle.shape_g(ModelKind.CIRCULAR, h, 100.0) - cir[i]) < 1e-7
+        assert abs(oracle.shape_g(ModelKind.HOLE, h, 100.0) - hol[i]) < 1e-6
+        assert abs(oracle.shape_g(ModelKind.WAVE, h, 100.0) - wav[i]) < 1e-6
+        assert abs(oracle.shape_g(ModelKind.Stable(1.5), h, 100.0) - exc[i]) < 1e-7
+
+
+def test_matern_closed_forms_match_bessel_quadrature(oracle):
+    # model.rs:1074-1106: Matern(1.5/2.5) quadrature == closed forms,
+    # Matern(0.5) == exponential; Stable(1) / Stable(2) == Exp / Gaussian.
+    ok = lambda a, b: abs(a - b) < 1e-9 or abs(a - b) / max(b, 1e-12) < 1e-6
+    for d in [0.001, 0.01, 0.1, 0.3, 0.5, 1.0, 2.0, 5.0]:
+        assert ok(oracle.shape_g(ModelKind.Matern(1.5), d, 1.0), oracle.shape_g(ModelKind.MATERN15, d, 1.0))
+        assert ok(oracle.shape_g(ModelKind.Matern(2.5), d, 1.0), oracle.shape_g(ModelKind.MATERN25, d, 1.0))
+        assert ok(oracle.shape_g(ModelKind.Matern(0.5), d, 1.0), oracle.shape_g(ModelKind.EXPONENTIAL, d, 1.0))
+        assert abs(oracle.shape_g(ModelKind.Stable(1.0), d, 1.0) - oracle.shape_g(ModelKind.EXPONENTIAL, d, 1.0)) < 1e-15
+        assert abs(oracle.shape_g(ModelKind.Stable(2.0), d, 1.0) - oracle.shape_g(ModelKind.GAUSSIAN, d, 1.0)) < 1e-15
+
+
+def test_2d_anisotropy_axes(oracle):
+    # model.rs:353-367 convention: azimuth 0 puts the major axis on +y; the
+    # minor (x) separation is stretched by 1/ratio.  azimuth 90 -> major on x.
+    iso = VariogramModel(0.0, [Structure(ModelKind.EXPONENTIAL, 1.0, 50.0)])
+    for az, major, minor in [(0.0, [0.0, 7.0], [7.0, 0.0]), (90.0, [7.0, 0.0], [0.0, 7.0])]:
+        an = VariogramModel(0.0, [Structure(ModelKind.EXPONENTIAL, 1.0, 50.0, Anisotropy(az, 0.5))])
+        assert abs(oracle.gamma_dh(an, major) - oracle.gamma_dh(iso, [7.0, 0.0])) < 1e-15
+        assert abs(oracle.gamma_dh(an, minor) - oracle.gamma_dh(iso, [14.0, 0.0])) < 1e-15
+
+
+# ------------------------------------------------------------- linalg.rs --
+def test_linalg_known_system_and_singular(oracle):
+    # linalg.rs:224-240
+    st, x = oracle.lu_solve([[2.0, 1.0, -1.0], [-3.0, -1.0, 2.0], [-2.0, 1.0, 2.0]], [8.0, -11.0, -3.0])
+    assert st == 0 and np.allclose(x, [2.0, 3.0, -1.0], atol=1e-12)
+    st, _ = oracle.lu_solve([[1.0, 2.0], [2.0, 4.0]], [1.0, 2.0])
+    assert st == 3
+
+
+def test_cholesky_matches_lu_and_rejects_non_pd(oracle):
+    # linalg.rs:242-262, 264-277
+    a = np.array([[4.0, 1.0, 0.5], [1.0, 3.0, 0.2], [0.5, 0.2, 2.0]])
+    st, xc = oracle.cholesky_solve(a, [1.0, 2.0, 3.0])
+    st2, xl = oracle.lu_solve(a, [1.0, 2.0, 3.0])
+    assert st == 0 and st2 == 0 and np.allclose(xc, xl, atol=1e-12)
+    assert oracle.cholesky_solve([[1.0, 2.0], [2.0, 4.0]], [1.0, 2.0])[0] == 3
+    a = np.array([[1.0, 0.3, 1.0], [0.3, 1.0, 1.0], [1.0, 1.0, 0.0]])
+    st, x = oracle.lu_solve(a, [0.7, 0.5, 1.0])
+    assert st == 0 and np.abs(a @ x - [0.7, 0.5, 1.0]).max() < 1e-12 and abs(x[0] + x[1] - 1) < 1e-12
+
+
+def test_lu_random_spd_against_numpy(oracle):
+    # linalg.rs:279-313 proptest (residual), checked against numpy's LAPACK
+    rs = np.random.default_rng(0)
+    for n in range(2, 9):
+        m = rs.uniform(-2, 2, (n, n))
+        a = m.T @ m + 0.5 * np.eye(n)
+        b = rs.uniform(-5, 5, n)
+        st, x = oracle.lu_solve(a, b)
+        assert st == 0 and np.allclose(x, np.linalg.solve(a, b), rtol=1e-10, atol=1e-12)
+
+
+# ------------------------------------------------------------- search.rs --
+def _brute_knn(pts, t, k, r=None):
+    # search.rs:316-349 k_nearest_brute: the k smallest by d2 (tie-free data)
+    d2 = (pts[:, 0] - t[0]) * (pts[:, 0] - t[0]) + (pts[:, 1] - t[1]) * (pts[:, 1] - t[1])
+    idx = np.arange(len(pts))
+    if r is not None:
+        keep = d2 <= r * r
+        d2, idx = d2[keep], idx[keep]
+    o = np.lexsort((idx, d2))[:k]
+    return list(idx[o])
+
+
+def test_kdtree_matches_brute_force(oracle):
+    # search.rs:363-376 (same point and target streams)
+    pts = cs.rng_points(oracle, 500, 5)
+    r = oracle.Rng(6)
+    targets = np.array([[r.uniform() * 120.0 - 10.0, r.uniform() * 120.0 - 10.0] for _ in range(200)])
+    for k, rad in [(1, None), (8, None), (16, 15.0), (1000, 5.0)]:
+        idx, cnt = oracle.kdtree_knn(pts, targets, k, rad)
+        for t in range(200):
+            assert list(idx[t, :cnt[t]]) == _brute_knn(pts, targets[t], k, rad)
+
+
+def test_kdtree_empty_and_degenerate(oracle):
+    # search.rs:398-410
+    idx, cnt = oracle.kdtree_knn(np.array([[1.0, 1.0]]), np.array([[0.0, 0.0]]), 5)
+    assert cnt[0] == 1 and idx[0, 0] == 0
+    idx, cnt = oracle.kdtree_knn(np.array([[1.0, 1.0]]), np.array([[0.0, 0.0]]), 5, 0.5)
+    assert cnt[0] == 0
+
+
+class _PyKd:
+    """Second, independent restatement of KdTree (search.rs:40-113) in
+    Python with heapq; pins the C port on tie-heavy lattices, where the
+    result depends on traversal order."""
+
+    def __init__(self, pts):
+        import heapq  # noqa: F401
+        self.p = pts
+        idx = list(range(len(pts)))
+        self.idx = idx
+        self._build(0, len(idx), 0)
+
+    def _build(self, lo, hi, axis):
+        if hi - lo <= 1:
+            return
+        mid = (hi - lo) // 2
+        seg = sorted(self.idx[lo:hi], key=lambda i: (self.p[i][axis], i))
+        self.idx[lo:hi] = seg
+        self._build(lo, lo + mid, (axis + 1) % 2)
+        self._build(lo + mid + 1, hi, (axis + 1) % 2)
+
+    def knn(self, t, k, r2=math.inf):
+        import heapq
+        heap = []  # max-heap via negated keys: (-d2, -idx)
+
+        def worst():
+            return -heap[0][0] if len(heap) == k else math.inf
+
+        def rec(lo, hi, axis):
+            if lo >= hi:
+                return
+            mid = lo + (hi - lo) // 2
+            i = self.idx[mid]
+            p = self.p[i]
+            d2 = 0.0
+            for d in range(2):
+                dd = p[d] - t[d]
+                d2 += dd * dd
+            if d2 <= r2:
+                if len(heap) < k:
+                    heapq.heappush(heap, (-d2, -i))
+                elif d2 < -heap[0][0]:
+                    heapq.heapreplace(heap, (-d2, -i))
+            diff = t[axis] - p[axis]
+            near, far = ((lo, mid), (mid + 1, hi)) if diff <= 0 else ((mid + 1, hi), (lo, mid))
+            rec(*near, (axis + 1) % 2)
+            if diff * diff <= min(worst(), r2):
+                rec(*far, (axis + 1) % 2)
+
+        rec(0, len(self.idx), 0)
+        return [-e[1] for e in sorted(heap, key=lambda e: (-e[0], -e[1]))]
+
+
+def test_kdtree_ties_match_independent_restatement(oracle):
+    # Lattice data: both restatements order equal axis values by index
+    # (the reference's choice there is unpinned); given that layout, the
+    # heap / traversal rules decide which tied points survive.
+    pts, _ = cs.lattice(13, 11)
+    tree = _PyKd([tuple(p) for p in pts])
+    targets = np.array([[x + 0.5, y + 0.5] for x in range(0, 12, 3) for y in range(0, 10, 3)]
+                       + [[x, y] for x in range(0, 13, 4) for y in range(0, 11, 4)], dtype=float)
+    for k in (1, 4, 5, 8, 13):
+        idx, cnt = oracle.kdtree_knn(pts, targets, k)
+        for t in range(len(targets)):
+            assert list(idx[t, :cnt[t]]) == tree.knn(targets[t], k), (k, targets[t])
+
+
+def test_kdtree_layout_matches_restatement(oracle):
+    pts, _ = cs.lattice(9, 7)
+    assert list(oracle.kdtree_layout(pts)) == _PyKd([tuple(p) for p in pts]).idx
+
+
+# ------------------------------------------------------------ kriging.rs --
+def test_power_model_matches_gstat_ordinary_kriging(oracle):
+    # kriging.rs:1221-1268 (gstat krige with vgm(2.0, "Pow", 1.2))
+    c = np.array([[0, 0], [10, 0], [0, 10], [10, 10], [5, 5]], dtype=float)
+    v = np.array([1.0, 2.0, 1.5, 2.5, 1.8])
+    m = VariogramModel(0.0, [Structure(ModelKind.Power(1.2), 2.0, 1.0)])
+    t = np.array([[3.0, 4.0], [7.0, 8.0], [5.0, 5.0]])
+    rc, val, var, lag, st, _ = oracle.predict_many(c, v, m, KrigingConfig(), t)
+    assert rc == 0 and (st == 0).all()
+    assert abs(val[0] - 1.5252052569) < 1e-6 and abs(var[0] - 6.4437897810) < 1e-5
+    assert abs(val[1] - 2.1366210000) < 1e-6 and abs(var[1] - 7.7780140000) < 1e-5
+    assert abs(val[2] - 1.8) < 1e-6 and var[2] < 1e-6
+
+
+def test_ok_exact_at_data_and_far_field(oracle):
+    # kriging.rs:1204-1219, 1360-1382
+    c, v = cs.sample_data()
+    m = cs.sample_model()
+    rc, val, var, *_ = oracle.predict_many(c, v, m, KrigingConfig(), c)
+    assert np.abs(val - v).max() < 1e-8 and var.max() < 1e-8
+    _, val, var, *_ = oracle.predict_many(c, v, m, KrigingConfig(), [[1e6, 1e6]])
+    assert var[0] >= 0.99 * m.total_sill() and np.isfinite(val[0])
+    sk = KrigingConfig(method=KrigingMethod.Simple(1.74))
+    _, val, var, lag, *_ = oracle.predict_many(c, v, m, sk, [[1e6, 1e6]])
+    assert abs(val[0] - 1.74) < 1e-9 and abs(var[0] - m.total_sill()) < 1e-9 and np.isnan(lag[0])
+
+
+def test_uk_reproduces_exact_drift(oracle):
+    # kriging.rs:1384-1412
+    c = np.array([[i, j] for i in range(4) for j in range(4)], dtype=float)
+    v = 2.0 + 3.0 * c[:, 0] - c[:, 1]
+    t = np.array([[1.5, 1.5], [5.0, 2.0]])
+    _, val, *_ = oracle.predict_many(c, v, cs.sample_model(),
+                                     KrigingConfig(method=KrigingMethod.Universal(1)), t)
+    assert np.abs(val - (2.0 + 3.0 * t[:, 0] - t[:, 1])).max() < 1e-7
+
+
+def test_kdtree_neighbourhood_matches_global_at_full_k(oracle):
+    # kriging.rs:1570-1591
+    c, v = cs.sample_data()
+    t = np.array([[0.3, 0.3], [0.9, 0.1], [2.0, 2.0]])
+    _, a, av, *_ = oracle.predict_many(c, v, cs.sample_model(), KrigingConfig(), t)
+    _, b, bv, *_ = oracle.predict_many(c, v, cs.sample_model(), KrigingConfig(max_neighbors=5), t)
+    assert np.abs(a - b).max() < 1e-12 and np.abs(av - bv).max() < 1e-12
+
+
+def test_min_neighbors_and_duplicates(oracle):
+    # kriging.rs:913-957, 1191-1202
+    c, v = cs.sample_data()
+    k = KrigingConfig(search_radius=0.75, min_neighbors=3)
+    _, val, _, _, st, _ = oracle.predict_many(c, v, cs.sample_model(), k, [[0.5, 0.5], [1.5, 1.5]])
+    assert st[0] == 0 and st[1] == 5 and np.isnan(val[1])
+    d = np.array([[0.0, 0.0], [1.0, 1.0], [1.0, 1.0]])
+    rc, *_, dup = oracle.predict_many(d, [1.0, 2.0, 3.0], cs.sample_model(), KrigingConfig(), [[0.0, 0.0]])
+    assert rc == 6 and tuple(dup) == (1, 2)
+    assert oracle.predict_many(c, v, cs.sample_model(), KrigingConfig(min_neighbors=0), c)[0] == 4
+    assert oracle.predict_many(c, v, cs.sample_model(),
+                               KrigingConfig(max_neighbors=2, min_neighbors=3), c)[0] == 4
+
+
+def test_octant_and_anisotropic_search(oracle):
+    # kriging.rs:960-1011, 1013-1188
+    c, v = cs.octant_cluster()
+    m = cs.sample_model()
+    near = oracle.predict_many(c, v, m, KrigingConfig(max_neighbors=4), [[0.0, 0.0]])[1][0]
+    octv = oracle.predict_many(c, v, m, KrigingConfig(max_neighbors=4, max_per_octant=1), [[0.0, 0.0]])[1][0]
+    assert near > 9.0 and octv < near - 2.0
+    c, v = cs.octant_cluster_rotated()
+    near = oracle.predict_many(c, v, m, KrigingConfig(max_neighbors=4), [[0.0, 0.0]])[1][0]
+    octv = oracle.predict_many(c, v, m, KrigingConfig(max_neighbors=4, max_per_octant=1,
+                                                      anisotropic_search=Anisotropy(90.0, 1.0)),
+                               [[0.0, 0.0]])[1][0]
+    assert near > 9.0 and octv < near - 2.0
+    ab = np.array([[0.0, 3.0], [1.0, 0.0]])
+    iso = oracle.predict_many(ab, [100.0, 0.0], m, KrigingConfig(max_neighbors=1), [[0.0, 0.0]])[1][0]
+    ani = oracle.predict_many(ab, [100.0, 0.0], m, KrigingConfig(
+        max_neighbors=1, anisotropic_search=Anisotropy(0.0, 0.2)), [[0.0, 0.0]])[1][0]
+    assert abs(iso) < 1e-8 and abs(ani - 100.0) < 1e-8
+    io = np.array([[0.0, 5.0], [5.0, 0.0]])
+    est = oracle.predict_many(io, [100.0, 999.0], m, KrigingConfig(
+        search_radius=5.0, anisotropic_search=Anisotropy(0.0, 0.5)), [[0.0, 0.0]])[1][0]
+    assert abs(est - 100.0) < 1e-6
+    a3 = np.array([[0.0, 0.0, 4.0], [4.0, 0.0, 0.0]])
+    est = oracle.predict_many(a3, [100.0, 0.0], m, KrigingConfig(
+        max_neighbors=1, anisotropic_search=Anisotropy(0.0, 0.2, 1.0, 90.0, 0.0)), [[0.0, 0.0, 0.0]])[1][0]
+    assert abs(est - 100.0) < 1e-6
+
+
+def _dense_ok(c, v, m, oracle, t, nb, method="ok"):
+    """Bordered kriging system assembled from the oracle's gamma and solved
+    by numpy (LAPACK): an independent check of build_lhs + LU + outputs."""
+    c0 = m.total_sill()
+    n = len(nb)
+    mm = 1 if method == "ok" else 0
+    A = np.zeros((n + mm, n + mm))
+    b = np.zeros(n + mm)
+    for a in range(n):
+        for q in range(n):
+            A[a, q] = c0 if a == q else c0 - oracle.gamma_dh(m, c[nb[a]] - c[nb[q]])
+        b[a] = c0 - oracle.gamma_dh(m, c[nb[a]] - t)
+        if mm:
+            A[a, n] = A[n, a] = 1.0
+    if mm:
+        b[n] = 1.0
+    w = np.linalg.solve(A, b)
+    return float(w[:n] @ v[nb]), float(max(c0 - w @ b, 0.0))
+
+
+@pytest.mark.parametrize("mname", ["sph", "mat15_aniso", "nested", "matern_nu", "hole"])
+def test_oracle_predict_against_dense_numpy(oracle, mname):
+    from geostat_hip import workloads
+    c, v = workloads.points(7, 400, [1000.0, 1000.0])
+    m = cs.MODELS_2D[mname]
+    t = workloads.points(8, 25, [1000.0, 1000.0])[0]
+    rc, val, var, lag, st, _ = oracle.predict_many(c, v, m, KrigingConfig(max_neighbors=12), t)
+    _, idx, cnt = oracle.neighbors(c, v, m, KrigingConfig(max_neighbors=12), t, 12)
+    for i in range(len(t)):
+        ev, evar = _dense_ok(c, v, m, oracle, t[i], idx[i, :cnt[i]])
+        assert abs(val[i] - ev) < 1e-9 * max(1, abs(ev)) and abs(var[i] - evar) < 1e-9
+
+
+# ------------------------------------------------------------ vecchia.rs --
+def test_full_conditioning_equals_exact(oracle):
+    # vecchia.rs:1997-2009 (field(14, 3), Exp(1, 40) + 0.05 nugget, 1e-9)
+    c, v = cs.vecchia_field(oracle, 14, 3)
+    m = VariogramModel(0.05, [Structure(ModelKind.EXPONENTIAL, 1.0, 40.0)])
+    rc, order, nb, cnt = oracle.vecchia_plan(c, 13)
+    rc2, ll = oracle.vecchia_loglik_plan(c, v, m, order, nb, cnt)
+    rc3, ex = oracle.dense_mvn_loglik(c, v, m)
+    assert rc == rc2 == rc3 == 0 and abs(ll - ex) < 1e-9
+
+
+def test_dense_mvn_matches_scipy(oracle):
+    c, v = cs.vecchia_field(oracle, 30, 5)
+    m = VariogramModel(0.05, [Structure(ModelKind.MATERN25, 1.0, 30.0)])
+    _, ex = oracle.dense_mvn_loglik(c, v, m)
+    d = np.sqrt(((c[:, None, :] - c[None, :, :]) ** 2).sum(-1))
+    s = np.sqrt(5.0) * d / 30.0
+    cov = 1.0 * (1 + s + s * s / 3) * np.exp(-s)
+    cov[np.diag_indices(30)] = 1.05
+    want = multivariate_normal(mean=np.zeros(30), cov=cov).logpdf(v - v.mean())
+    assert abs(ex - want) < 1e-9 * abs(want)
+
+
+def test_vecchia_approximation_close_with_maxmin(oracle):
+    # vecchia.rs:2011-2027
+    c, v = cs.vecchia_field(oracle, 90, 11)
+    m = VariogramModel(0.05, [Structure(ModelKind.EXPONENTIAL, 1.0, 30.0)])
+    _, order, nb, cnt = oracle.vecchia_plan(c, 12)
+    _, ll = oracle.vecchia_loglik_plan(c, v, m, order, nb, cnt)
+    _, ex = oracle.dense_mvn_loglik(c, v, m)
+    assert abs(ll - ex) / abs(ex) < 0.02
+
+
+@pytest.mark.parametrize("seed", [3, 11, 29])
+def test_lazy_maxmin_matches_exact_traversal(oracle, seed):
+    # vecchia.rs:2275-2287
+    c, _ = cs.vecchia_field(oracle, 400, seed)
+    assert list(oracle.maxmin_order(c)) == list(oracle.maxmin_order(c, brute=True))
+
+
+def test_maxmin_brute_against_numpy(oracle):
+    # independent farthest-point traversal (first = nearest to the mean,
+    # ties to the larger index)
+    c, _ = cs.vecchia_field(oracle, 120, 17)
+    n = len(c)
+    cen = np.zeros(2)
+    for p in c:
+        cen = cen + p / n
+    d0 = ((c - cen) ** 2).sum(1)
+    first = int(np.argmin(d0))
+    order = [first]
+    md = ((c - c[first]) ** 2).sum(1)
+    chosen = np.zeros(n, bool)
+    chosen[first] = True
+    for _ in range(n - 1):
+        cand = np.where(~chosen)[0]
+        best = cand[np.lexsort((cand, md[cand]))[-1]]
+        order.append(int(best))
+        chosen[best] = True
+        md = np.minimum(md, ((c - c[best]) ** 2).sum(1))
+    assert list(oracle.maxmin_order(c, brute=True)) == order
+
+
+def test_incremental_predecessors_match_brute_force(oracle):
+    # vecchia.rs:2289-2310
+    c, _ = cs.vecchia_field(oracle, 300, 13)
+    m = 8
+    _, order, nb, cnt = oracle.vecchia_plan(c, m)
+    for k in range(300):
+        prev = order[:k]
+        d2 = ((c[prev] - c[order[k]]) ** 2).sum(1)
+        want = sorted(prev[np.lexsort((prev, d2))][:m])
+        assert sorted(nb[k, :cnt[k]]) == want
