@@ -195,6 +195,101 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
     return ok;
 }
 
+// Drift block by modified Gram-Schmidt on the augmented columns
+// [YF | yb | yz] (YF = L^-1 F, yb = L^-1 b, yz = L^-1 z, rows distributed
+// over the group).  With YF = Q R, g = R^-T f and the residuals
+// yb_perp = yb - Q Q'yb, yz_perp = yz - Q Q'yz:
+//   value    = yb_perp' yz_perp + g' Q'yz
+//   variance = c0 - |yb_perp|^2 - g'(2 Q'yb - g)
+//   mu       = R^-1 (Q'yb - g),  lagrange = mu[0]
+// This never forms YF'YF, so the drift block's conditioning is not squared.
+// A column whose orthogonal remainder falls below 1e-12 of the largest
+// drift column norm makes the system singular (rank-deficient drift).
+template <int R, int G, int M, int Q>
+__device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&fT)[M], double c0,
+                                              bool &ok, double &var, double &lag) {
+    double rr[M][M], qb[M], qz[M];
+    double n0 = 0.0;
+    static_for<0, M>([&](auto JJ) {
+        constexpr int j = decltype(JJ)::value;
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) s += y[2 + j][r] * y[2 + j][r];
+        n0 = fmax(n0, gsum<G>(s));
+    });
+    const double tol = 1e-12 * sqrt(n0);
+    static_for<0, M>([&](auto JJ) {
+        constexpr int j = decltype(JJ)::value;
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) s += y[2 + j][r] * y[2 + j][r];
+        const double rjj = sqrt(gsum<G>(s));
+        ok = ok && (rjj > tol);
+        rr[j][j] = rjj;
+        const double inv = 1.0 / rjj;
+#pragma unroll
+        for (int r = 0; r < R; r++) y[2 + j][r] *= inv;
+        static_for<j + 1, M>([&](auto KK) {
+            constexpr int k = decltype(KK)::value;
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; r++) t += y[2 + j][r] * y[2 + k][r];
+            t = gsum<G>(t);
+            rr[j][k] = t;
+#pragma unroll
+            for (int r = 0; r < R; r++) y[2 + k][r] -= t * y[2 + j][r];
+        });
+        double tb = 0.0, tz = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            tb += y[2 + j][r] * y[0][r];
+            tz += y[2 + j][r] * y[1][r];
+        }
+        tb = gsum<G>(tb);
+        tz = gsum<G>(tz);
+        qb[j] = tb;
+        qz[j] = tz;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            y[0][r] -= tb * y[2 + j][r];
+            y[1][r] -= tz * y[2 + j][r];
+        }
+    });
+    double pbz = 0.0, pbb = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        pbz += y[0][r] * y[1][r];
+        pbb += y[0][r] * y[0][r];
+    }
+    pbz = gsum<G>(pbz);
+    pbb = gsum<G>(pbb);
+    double gg[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {        // R' g = f
+        double s = fT[j];
+#pragma unroll
+        for (int i = 0; i < j; i++) s -= rr[i][j] * gg[i];
+        gg[j] = s / rr[j][j];
+    }
+    double mu[M];
+#pragma unroll
+    for (int j = M - 1; j >= 0; j--) {   // R mu = Q'yb - g
+        double s = qb[j] - gg[j];
+#pragma unroll
+        for (int k = j + 1; k < M; k++) s -= rr[j][k] * mu[k];
+        mu[j] = s / rr[j][j];
+    }
+    double value = pbz, red = pbb;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        value += gg[j] * qz[j];
+        red += gg[j] * (2.0 * qb[j] - gg[j]);
+    }
+    var = c0 - red;
+    lag = mu[0];
+    return value;
+}
+
 template <int D, int NB, int G, int M>
 __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
     constexpr int R = NB / G;
@@ -263,77 +358,21 @@ __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
         bool ok = build_factor_solve<D, NB, G, Q>(Mdl, cnt, rowx, xy, tg, s_stage, lane, g, y);
         wave_sync();  // the group's LDS rows are rewritten by its next system
 
-        double dq[Q][Q];
-        static_for<0, Q>([&](auto QA) {
-            constexpr int qa = decltype(QA)::value;
-            static_for<qa, Q>([&](auto QB) {
-                constexpr int qb = decltype(QB)::value;
-                double s = 0.0;
-#pragma unroll
-                for (int r = 0; r < R; r++) s += y[qa][r] * y[qb][r];
-                dq[qa][qb] = gsum<G>(s);
-            });
-        });
-
         double value, var, lag;
         if constexpr (M == 0) {
-            value = P.sk_mean + dq[0][1];
-            var = c0 - dq[0][0];
+            double bz = 0.0, bb = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                bz += y[0][r] * y[1][r];
+                bb += y[0][r] * y[0][r];
+            }
+            value = P.sk_mean + gsum<G>(bz);
+            var = c0 - gsum<G>(bb);
             lag = __builtin_nan("");
         } else {
             double fT[M];
             basis<D, M>(P, tg, fT);
-            double S[M][M], mu[M];
-#pragma unroll
-            for (int a = 0; a < M; a++) {
-#pragma unroll
-                for (int b = a; b < M; b++) S[a][b] = S[b][a] = dq[2 + a][2 + b];
-                mu[a] = dq[0][2 + a] - fT[a];
-            }
-            double smax = 0.0;
-#pragma unroll
-            for (int a = 0; a < M; a++) smax = fmax(smax, S[a][a]);
-            // unpivoted Cholesky of the m x m Schur complement, then solve
-#pragma unroll
-            for (int kk = 0; kk < M; kk++) {
-                double dd = S[kk][kk];
-#pragma unroll
-                for (int jj = 0; jj < kk; jj++) dd -= S[kk][jj] * S[kk][jj];
-                ok = ok && (dd > 1e-12 * smax);
-                dd = sqrt(dd);
-                S[kk][kk] = dd;
-#pragma unroll
-                for (int ii = kk + 1; ii < M; ii++) {
-                    double s = S[ii][kk];
-#pragma unroll
-                    for (int jj = 0; jj < kk; jj++) s -= S[ii][jj] * S[kk][jj];
-                    S[ii][kk] = s / dd;
-                }
-            }
-#pragma unroll
-            for (int ii = 0; ii < M; ii++) {
-                double s = mu[ii];
-#pragma unroll
-                for (int jj = 0; jj < ii; jj++) s -= S[ii][jj] * mu[jj];
-                mu[ii] = s / S[ii][ii];
-            }
-#pragma unroll
-            for (int ii = M - 1; ii >= 0; ii--) {
-                double s = mu[ii];
-#pragma unroll
-                for (int jj = ii + 1; jj < M; jj++) s -= S[jj][ii] * mu[jj];
-                mu[ii] = s / S[ii][ii];
-            }
-            double v = dq[0][1], wb = dq[0][0], mf = 0.0;
-#pragma unroll
-            for (int a = 0; a < M; a++) {
-                v -= mu[a] * dq[1][2 + a];
-                wb -= mu[a] * dq[0][2 + a];
-                mf += mu[a] * fT[a];
-            }
-            value = v;
-            var = c0 - (wb + mf);
-            lag = mu[0];
+            value = drift_solve<R, G, M>(y, fT, c0, ok, var, lag);
         }
         if (g == 0) {
             if (ok) {

@@ -73,6 +73,55 @@ def lattice(nx, ny, step=1.0, jitter_seed=None):
     return c, v
 
 
+def drift_basis(coords, method, p):
+    """kriging.rs:340-347, 410-439: [1, xn, (xn_a xn_b, a <= b)] with
+    xn = (p - centre) / scale over the data bounding box."""
+    if method.kind == 0:
+        return []
+    if method.kind == 1:
+        return [1.0]
+    mn, mx = coords.min(0), coords.max(0)
+    centre = 0.5 * (mn + mx)
+    scale = max(0.5 * float((mx - mn).max()), 2.2250738585072014e-308)
+    xn = (np.asarray(p) - centre) / scale
+    out = [1.0] + list(xn)
+    if method.degree == 2:
+        D = len(xn)
+        out += [xn[a] * xn[b] for a in range(D) for b in range(a, D)]
+    return out
+
+
+def exact_estimate(oracle, coords, values, model, method, target, nb, digits=40):
+    """The bordered kriging system (kriging.rs:527-644) with entries from the
+    oracle's gamma, solved in `digits`-digit arithmetic: the exact answer the
+    double-precision solvers approximate.  Returns (value, variance)."""
+    import mpmath as mp
+    mp.mp.dps = digits
+    c0 = model.total_sill()
+    n = len(nb)
+    F = [drift_basis(coords, method, coords[i]) for i in nb]
+    m = len(F[0]) if F else 0
+    A = mp.zeros(n + m, n + m)
+    b = mp.zeros(n + m, 1)
+    for a in range(n):
+        for q in range(n):
+            A[a, q] = c0 if a == q else c0 - oracle.gamma_dh(model, coords[nb[a]] - coords[nb[q]])
+        b[a] = c0 - oracle.gamma_dh(model, coords[nb[a]] - target)
+        for k in range(m):
+            A[a, n + k] = F[a][k]
+            A[n + k, a] = F[a][k]
+    for k, fk in enumerate(drift_basis(coords, method, target)):
+        b[n + k] = fk
+    w = mp.lu_solve(A, b)
+    if method.kind == 0:
+        val = method.mean + sum(w[a] * (values[nb[a]] - method.mean) for a in range(n))
+        var = c0 - sum(w[a] * b[a] for a in range(n))
+    else:
+        val = sum(w[a] * values[nb[a]] for a in range(n))
+        var = c0 - sum(w[a] * b[a] for a in range(n + m))
+    return float(val), max(float(var), 0.0)
+
+
 MODELS_2D = {
     "sph": VariogramModel(0.1, [Structure(ModelKind.SPHERICAL, 1.0, 250.0)]),
     "exp": VariogramModel(0.05, [Structure(ModelKind.EXPONENTIAL, 1.0, 150.0)]),
