@@ -86,7 +86,20 @@ def main() -> int:
                     choices=["ok2d_large", "ok2d_small", "uk3d_octant"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32r"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times "
+                         "(scales rocprof FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(targets shard trivially; see DESIGN.md)", file=sys.stderr)

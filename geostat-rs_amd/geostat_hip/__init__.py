@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import re
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -83,6 +84,9 @@ def _check(status: int, dup=None) -> None:
         return
     msg = _abi.last_error()
     if status == DuplicatePoints.code:
+        if dup is None:   # the message carries the pair: "duplicate points i, j"
+            found = re.search(r"duplicate points (\d+), (\d+)", msg)
+            dup = (int(found.group(1)), int(found.group(2))) if found else (-1, -1)
         raise DuplicatePoints(int(dup[0]), int(dup[1]), msg)
     raise _ERRORS.get(status, DeviceError)(msg)
 

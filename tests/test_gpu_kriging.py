@@ -18,13 +18,15 @@ TOL64 = 1e-10
 TOL32R = 1e-6
 
 
-def rel_inf(a, b):
+def rel_inf(a, b, floor=1e-300):
+    """||a - b||_inf / max(||b||_inf, floor) over finite entries; the floor
+    keeps an all-zero reference (variance at data points) meaningful."""
     a, b = np.asarray(a), np.asarray(b)
     assert np.array_equal(np.isnan(a), np.isnan(b)), "NaN patterns differ"
     m = ~np.isnan(b)
     if not m.any():
         return 0.0
-    scale = max(np.abs(b[m]).max(), 1e-300)
+    scale = max(np.abs(b[m]).max(), floor)
     return float(np.abs(a[m] - b[m]).max() / scale)
 
 
@@ -33,7 +35,9 @@ def closer_to_exact(oracle, coords, values, model, config, targets, got, want, w
     rounding noise of an ill-conditioned system: then the GPU must be at
     least as close as the oracle (the reference's LU) to the exact solution
     of the same system, computed in 40-digit arithmetic."""
-    scale = np.abs(want[~np.isnan(want)]).max()
+    c0 = model.total_sill()
+    floor = 1e-3 * (np.sqrt(c0) if which == 0 else c0)
+    scale = max(np.abs(want[~np.isnan(want)]).max(), floor)
     worst = np.argsort(-np.nan_to_num(np.abs(got - want)))[:8]
     _, idx, cnt = oracle.neighbors(coords, values, model, config, targets[worst],
                                    max(config.max_neighbors or 64, 1))
@@ -50,8 +54,10 @@ def check_predict(oracle, coords, values, model, config, targets, prec="fp64", t
     rc, ov, ovar, olag, ost, _ = oracle.predict_many(coords, values, model, config, targets)
     assert rc == 0
     assert np.array_equal(st, ost), (np.nonzero(st != ost)[0][:10], st[st != ost][:10], ost[st != ost][:10])
+    c0 = model.total_sill()
     for which, got, want in ((0, v, ov), (1, var, ovar)):
-        if rel_inf(got, want) >= tol:
+        floor = 1e-3 * (np.sqrt(c0) if which == 0 else c0)
+        if rel_inf(got, want, floor) >= tol:
             closer_to_exact(oracle, coords, values, model, config, targets, got, want, which)
     if config.method.kind != 0 and rel_inf(lag, olag) >= tol * 10:
         # the multiplier of an ill-conditioned drift block: bound by the
@@ -225,8 +231,30 @@ def test_min_neighbors_error_and_build_errors():
         gh.Kriging(np.zeros((0, 2)), np.zeros(0), m)
     with pytest.raises(gh.InvalidParameter):   # Power: semivariogram form not on GPU
         gh.Kriging(c, v, gh.VariogramModel(0.0, [gh.Structure(gh.ModelKind.Power(1.2), 2.0, 1.0)]))
-    with pytest.raises(gh.InvalidParameter):   # more than 64 neighbours
-        gh.Kriging(*workloads.points(1, 200, [10.0, 10.0]), m, gh.KrigingConfig(max_neighbors=100))
+    # a neighbourhood above the GPU's 64-point systems is a per-target
+    # InvalidParameter (NaN estimate), never a silently truncated system
+    big = gh.Kriging(*workloads.points(1, 200, [10.0, 10.0]), m, gh.KrigingConfig(max_neighbors=100))
+    val, _, _, st = big.predict_many([[5.0, 5.0], [1.0, 2.0]])
+    assert (st == 4).all() and np.isnan(val).all()
+    with pytest.raises(gh.InvalidParameter):
+        big.predict([5.0, 5.0])
+
+
+def test_radius_neighbourhood_overflow_is_flagged(oracle):
+    # radius-only search keeps every in-radius point (kriging.rs:445-451);
+    # targets with more than 64 get status 4, all others match the oracle
+    c, v = workloads.points(13, 20000, [1000.0, 1000.0])
+    t = workloads.points(14, 3000, [1000.0, 1000.0])[0]
+    m = cs.MODELS_2D["exp"]
+    cfg = gh.KrigingConfig(search_radius=30.0)
+    k = gh.Kriging(c, v, m, cfg)
+    val, var, _, st = k.predict_many(t)
+    rc, ov, ovar, _, ost, _ = oracle.predict_many(c, v, m, cfg, t)
+    _, _, ocnt = oracle.neighbors(c, v, m, cfg, t, 200)
+    over = ocnt > 64
+    assert over.any() and (~over).any()
+    assert (st[over] == 4).all() and np.array_equal(st[~over], ost[~over])
+    assert rel_inf(val[~over], ov[~over]) < TOL64 and rel_inf(var[~over], ovar[~over], 1e-3) < TOL64
 
 
 def test_singular_targets_are_nan_with_status(oracle):

@@ -105,5 +105,6 @@ def test_singular_and_rejections(oracle):
         gh.vecchia_plan(c[:1], 3)
     with pytest.raises(gh.InvalidParameter):
         gh.vecchia_plan(c, 0)
-    with pytest.raises(gh.DuplicatePoints):
+    with pytest.raises(gh.DuplicatePoints) as ei:
         gh.vecchia_plan(np.vstack([c, c[:1]]), 3)
+    assert (ei.value.i, ei.value.j) == (0, 5)
