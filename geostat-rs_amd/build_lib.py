@@ -35,26 +35,35 @@ EXACT = ["-ffp-contract=off"]     # search + host: no a*b+c fusion (bit-exact di
 
 # (D, NB, M) instantiations of the kriging kernel: M = drift terms
 # (0 simple, 1 ordinary, 1+D universal deg 1, 1+D+D(D+1)/2 universal deg 2).
-KRIGE = [(d, nb, m) for d, ms in ((2, (0, 1, 3, 6)), (3, (0, 1, 4, 10)))
-         for nb in (16, 32, 64) for m in ms]
-VECCHIA = [(d, nb) for d in (2, 3) for nb in (16, 32, 64)]
+# Group sizes G (lanes per system) compiled for each padded size NB; the
+# runtime tuner picks one per configuration (gs_internal.hpp group_size).
+# The lean model class (spherical / exponential / Gaussian / closed-form
+# Matern) gets both group sizes; the general class only the first.
+GROUPS = {16: (2, 4), 32: (8, 16), 64: (32, 64)}
+KRIGE = [(d, nb, m, g, lean) for d, ms in ((2, (0, 1, 3, 6)), (3, (0, 1, 4, 10)))
+         for nb in (16, 32, 64) for m in ms for lean in (1, 0)
+         for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
+VECCHIA = [(d, nb, g, lean) for d in (2, 3) for nb in (16, 32, 64) for lean in (1, 0)
+           for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
 
 
 def _gen_sources() -> list[Path]:
     GEN.mkdir(parents=True, exist_ok=True)
     out = []
-    for d, nb, m in KRIGE:
-        p = GEN / f"krige_d{d}_nb{nb}_m{m}.hip"
+    for d, nb, m, g, lean in KRIGE:
+        p = GEN / f"krige_d{d}_nb{nb}_m{m}_g{g}_{'lean' if lean else 'gen'}.hip"
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
-               f"namespace gs {{\ntemplate hipError_t launch_krige_inst<{d}, {nb}, {m}>"
+               f"namespace gs {{\ntemplate hipError_t launch_krige_inst<{d}, {nb}, {m}, {g}, "
+               f"{'true' if lean else 'false'}>"
                f"(const SolveParams &, int64_t, hipStream_t);\n}}\n")
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
         out.append(p)
-    for d, nb in VECCHIA:
-        p = GEN / f"vecchia_d{d}_nb{nb}.hip"
+    for d, nb, g, lean in VECCHIA:
+        p = GEN / f"vecchia_d{d}_nb{nb}_g{g}_{'lean' if lean else 'gen'}.hip"
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
-               f"namespace gs {{\ntemplate hipError_t launch_vecchia_inst<{d}, {nb}>"
+               f"namespace gs {{\ntemplate hipError_t launch_vecchia_inst<{d}, {nb}, {g}, "
+               f"{'true' if lean else 'false'}>"
                f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
@@ -62,8 +71,15 @@ def _gen_sources() -> list[Path]:
     return out
 
 
-def _compile(src: Path, extra: list[str]) -> Path:
+def _deps_mtime() -> float:
+    return max(p.stat().st_mtime for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".hip", ".cpp"))
+
+
+def _compile(src: Path, extra: list[str], incremental: bool = False) -> Path:
     obj = OBJ / (src.stem + ".o")
+    if incremental and obj.exists() and obj.stat().st_mtime > max(_deps_mtime(), src.stat().st_mtime,
+                                                                   (HERE.parent / "include" / "geostat_hip.h").stat().st_mtime):
+        return obj
     cmd = [HIPCC, *COMMON, *extra, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -71,8 +87,9 @@ def _compile(src: Path, extra: list[str]) -> Path:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> Path:
-    """Compiles every HIP source for gfx950 and links the library."""
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> Path:
+    """Compiles every HIP source for gfx950 and links the library
+    (incremental=True skips objects newer than every source and header)."""
     OBJ.mkdir(parents=True, exist_ok=True)
     units: list[tuple[Path, list[str]]] = [
         (CSRC / "gs_search.hip", EXACT),
@@ -83,7 +100,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     jobs = jobs or max(1, min(16, os.cpu_count() or 4))
     objs: list[Path] = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(_compile, s, e): s for s, e in units}
+        futs = {ex.submit(_compile, s, e, incremental): s for s, e in units}
         for f in cf.as_completed(futs):
             objs.append(f.result())
             if verbose:
@@ -113,4 +130,4 @@ def clean() -> None:
 if __name__ == "__main__":
     if "--clean" in sys.argv:
         clean()
-    print(build())
+    print(build(incremental="--incremental" in sys.argv))

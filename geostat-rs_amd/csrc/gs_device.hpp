@@ -120,18 +120,105 @@ __device__ __forceinline__ double eff_h(const DevStructure &s, const double *dh)
     return sqrt(acc);
 }
 
-// Covariance c0 - gamma(dh) for a non-zero separation (callers handle the
-// diagonal, where gamma(0) = 0 gives exactly c0).
+// ---- lean model class: spherical, exponential, Gaussian, Matern 3/2, 5/2 --
+// These five kinds need only exp and sqrt.  Kernels specialised on this
+// class use the two routines below instead of the general libm-grade ones,
+// which keeps the covariance build short and its register footprint small.
+
+// exp(-s) for s >= 0: Cody-Waite reduction s = k ln2 + r, |r| <= ln2/2,
+// then the degree-12 Taylor polynomial of exp(-r) (truncation below 2e-16
+// relative on that interval) and a power-of-two scale.  s is clamped at 800
+// (exp(-800) underflows to 0 in double).
+__device__ __forceinline__ double exp_neg(double s) {
+    s = fmin(s, 800.0);
+    const double kf = rint(s * 1.4426950408889634);
+    double r = fma(-kf, 6.93147180369123816490e-01, s);   // ln2 high part
+    r = fma(-kf, 1.90821492927058770002e-10, r);           // ln2 low part
+    const double x = -r;
+    double p = 2.08767569878680989792e-09;                 // 1/12!
+    p = fma(p, x, 2.50521083854417187751e-08);             // 1/11!
+    p = fma(p, x, 2.75573192239858906526e-07);             // 1/10!
+    p = fma(p, x, 2.75573192239858906526e-06);             // 1/9!
+    p = fma(p, x, 2.48015873015873015873e-05);             // 1/8!
+    p = fma(p, x, 1.98412698412698412698e-04);             // 1/7!
+    p = fma(p, x, 1.38888888888888888889e-03);             // 1/6!
+    p = fma(p, x, 8.33333333333333333333e-03);             // 1/5!
+    p = fma(p, x, 4.16666666666666666667e-02);             // 1/4!
+    p = fma(p, x, 1.66666666666666666667e-01);             // 1/3!
+    p = fma(p, x, 0.5);
+    p = fma(p, x, 1.0);
+    p = fma(p, x, 1.0);
+    return ldexp(p, -(int)kf);
+}
+
+// sqrt(q) for q >= 0: hardware reciprocal-square-root estimate, then one
+// Markstein correction h + (q - h^2) y/2 (about 1 ulp).
+__device__ __forceinline__ double sqrt_fast(double q) {
+    const double y = __builtin_amdgcn_rsq(q);
+    const double h = q * y;
+    const double h1 = fma(fma(-h, h, q), 0.5 * y, h);
+    return q > 0.0 ? h1 : 0.0;
+}
+
+// |T dh|^2 for one structure (model.rs:521-539 before the square root).
 template <int D>
+__device__ __forceinline__ double eff_h2(const DevStructure &s, const double *dh) {
+    if (!s.aniso) {
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) acc += dh[d] * dh[d];
+        return acc;
+    }
+    if (D == 2) {
+        double e0 = s.m[0] * dh[0] + s.m[1] * dh[1];
+        double e1 = s.m[2] * dh[0] + s.m[3] * dh[1];
+        return e0 * e0 + e1 * e1;
+    }
+    double e0 = s.m[0] * dh[0] + s.m[1] * dh[1] + s.m[2] * dh[2];
+    double e1 = s.m[3] * dh[0] + s.m[4] * dh[1] + s.m[5] * dh[2];
+    double e2 = s.m[6] * dh[0] + s.m[7] * dh[1] + s.m[8] * dh[2];
+    return e0 * e0 + e1 * e1 + e2 * e2;
+}
+
+__device__ __forceinline__ double shape_lean(const DevStructure &s, double h2) {
+    if (s.kind == K_GAU) {                       // 1 - exp(-(h/a)^2): no root needed
+        return 1.0 - exp_neg(h2 * (s.inv_range * s.inv_range));
+    }
+    const double d = sqrt_fast(h2) * s.inv_range;
+    switch (s.kind) {
+    case K_SPH: return d >= 1.0 ? 1.0 : 1.5 * d - 0.5 * d * d * d;
+    case K_EXP: return 1.0 - exp_neg(d);
+    case K_MAT15: { double q = 1.7320508075688772 * d; return 1.0 - (1.0 + q) * exp_neg(q); }
+    default: { double q = 2.23606797749979 * d; return 1.0 - (1.0 + q + q * q / 3.0) * exp_neg(q); }
+    }
+}
+
+// Covariance c0 - gamma(dh) for a separation (the diagonal, where gamma(0) =
+// 0 gives exactly c0, is handled by the callers; an all-zero separation to
+// a target returns c0 too).  LEAN selects the five-kind fast path.
+template <int D, bool LEAN>
 __device__ __forceinline__ double cov_offdiag(const DevModel &M, const double *dh) {
     bool allzero = true;
 #pragma unroll
     for (int d = 0; d < D; d++) allzero = allzero && (dh[d] == 0.0);
     if (allzero) return M.c0;
     double gam = 0.0;
-    for (int i = 0; i < M.ns; i++) gam += M.s[i].sill * shape_g(M.s[i], M.nodes, eff_h<D>(M.s[i], dh));
+    for (int i = 0; i < M.ns; i++) {
+        if constexpr (LEAN) gam += M.s[i].sill * shape_lean(M.s[i], eff_h2<D>(M.s[i], dh));
+        else gam += M.s[i].sill * shape_g(M.s[i], M.nodes, eff_h<D>(M.s[i], dh));
+    }
     return M.c0 - (M.nugget + gam);
 }
+
 #endif
+
+// True when every structure of the model is in the lean class.
+inline bool model_is_lean(const DevModel &M) {
+    for (int i = 0; i < M.ns; i++) {
+        const int k = M.s[i].kind;
+        if (!(k == K_SPH || k == K_EXP || k == K_GAU || k == K_MAT15 || k == K_MAT25)) return false;
+    }
+    return true;
+}
 
 }  // namespace gs

@@ -14,6 +14,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -315,6 +316,74 @@ int tree_depth(int64_t n) {
 
 int pad_nb(int64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
 
+// ---- group-size tuner ------------------------------------------------------
+// Each solve kernel is compiled for two group sizes (lanes per system).  The
+// first launch of a configuration times both on the call's own inputs and
+// caches the faster one per (device, kernel family, dim, NB, drift terms,
+// model signature); GS_SOLVE_VARIANT=0|1 pins the choice.
+std::mutex g_tune_mu;
+std::unordered_map<std::string, int> g_tune;
+
+std::string model_signature(const gs_model &m) {
+    std::string s;
+    for (int i = 0; i < m.n_structures; i++)
+        s += std::to_string(m.structures[i].kind) + (m.structures[i].has_anis ? "a" : "i") + ",";
+    return s;
+}
+
+template <class Launch>
+int32_t tuned_variant(const std::string &key, hipStream_t s, Launch &&launch, int *out) {
+    if (const char *env = std::getenv("GS_SOLVE_VARIANT")) {
+        *out = std::atoi(env) ? 1 : 0;
+        return GS_OK;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            *out = it->second;
+            return GS_OK;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = -1.f;
+    int pick = 0;
+    for (int v = 0; v < gs::kVariants; v++) {
+        float ms = 0.f;
+        bool built = true;
+        for (int rep = 0; rep < 2 && built; rep++) {   // the first run warms caches and code
+            HIP_TRY(hipEventRecord(e0, s));
+            hipError_t e = launch(v);
+            if (e == hipErrorInvalidDeviceFunction) {   // variant not compiled for this class
+                (void)hipGetLastError();
+                built = false;
+                break;
+            }
+            if (e != hipSuccess) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return fail(e == hipErrorInvalidValue ? GS_ERR_INVALID_PARAMETER : GS_ERR_DEVICE,
+                            std::string("solve launch: ") + hipGetErrorString(e));
+            }
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        }
+        if (built && (best < 0.f || ms < best)) {
+            best = ms;
+            pick = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = pick;
+    *out = pick;
+    return GS_OK;
+}
+
 }  // namespace
 
 struct gs_ctx {
@@ -330,6 +399,7 @@ struct gs_kriging {
     gs_kconfig cfg{};
     gs_model model{};
     int M = 1;
+    bool lean = true;        // every structure in the fast closed-form class
     bool has_tree = false;
     int64_t k_heap = 0;
     int per_oct = 0;
@@ -525,6 +595,7 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
     HIP_TRY(hipMemcpy(K->coords.p, coords, sizeof(double) * n * dim, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(K->values.p, values, sizeof(double) * n, hipMemcpyHostToDevice));
     DevModel dm = resolve_model(*model, dim, ctx->nodes.as<gs::BesselNodes>());
+    K->lean = gs::model_is_lean(dm);
     HIP_TRY(K->dmodel.ensure(sizeof(DevModel)));
     HIP_TRY(hipMemcpy(K->dmodel.p, &dm, sizeof(DevModel), hipMemcpyHostToDevice));
     if (K->has_tree) {
@@ -670,9 +741,25 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
     p.variance = d_var;
     p.lagrange = d_lag;
     p.status = d_status;
-    auto launch = [&](int NB, const gs::SolveParams &pp, int64_t groups) -> hipError_t {
-        return precision == GS_PREC_FP64 ? gs::launch_krige_fp64(K->D, NB, K->M, pp, groups, s)
-                                         : gs::launch_krige_fp32r(K->D, NB, K->M, pp, groups, s);
+    if (precision != GS_PREC_FP64)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
+    int device = K->ctx->device;
+    auto solve = [&](int NB, const gs::SolveParams &pp, int64_t groups) -> int32_t {
+        std::string key = "krige:" + std::to_string(device) + ":" + std::to_string(K->D) + ":" +
+                          std::to_string(NB) + ":" + std::to_string(K->M) + ":" +
+                          model_signature(K->model) + (pp.list ? "list" : "");
+        gs::SolveParams tp = pp;
+        if (!pp.list) tp.T = std::min<int64_t>(pp.T, 131072);
+        int variant = 0;
+        int32_t st = tuned_variant(key, s, [&](int v) {
+            return gs::launch_krige_fp64(K->D, NB, K->M, v, K->lean, tp, pp.list ? groups : tp.T, s);
+        }, &variant);
+        if (st != GS_OK) return st;
+        hipError_t e = gs::launch_krige_fp64(K->D, NB, K->M, variant, K->lean, pp, groups, s);
+        if (e == hipErrorInvalidValue)
+            return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size");
+        HIP_TRY(e);
+        return GS_OK;
     };
     if (K->per_oct > 0 && K->nb_bound > 16) {
         // Uneven sizes: bucket by padded system size, one launch per bucket.
@@ -689,18 +776,12 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
             pb.list_len = K->lens.as<int32_t>() + b;
             // persistent grid: enough groups to fill the device, grid-strided
             int64_t groups = std::min<int64_t>(T, (int64_t)256 * 64 * 4);
-            hipError_t e = launch(nbs[b], pb, groups);
-            if (e == hipErrorInvalidValue)
-                return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size/precision");
-            HIP_TRY(e);
+            int32_t st = solve(nbs[b], pb, groups);
+            if (st != GS_OK) return st;
         }
         return GS_OK;
     }
-    hipError_t e = launch(pad_nb(K->nb_bound), p, T);
-    if (e == hipErrorInvalidValue)
-        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size/precision");
-    HIP_TRY(e);
-    return GS_OK;
+    return solve(pad_nb(K->nb_bound), p, T);
 }
 
 }  // namespace
@@ -1233,6 +1314,8 @@ int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_mode
     HIP_TRY(gs::launch_zero_i32(p->status.as<int32_t>(), n_models, s));
     for (int32_t b0 = 0; b0 < n_models; b0 += gs::kMaxModelsPerLaunch) {
         int32_t nm = std::min<int32_t>(gs::kMaxModelsPerLaunch, n_models - b0);
+        bool lean = true;
+        for (int32_t i = b0; i < b0 + nm; i++) lean = lean && gs::model_is_lean(dm[(size_t)i]);
         gs::VecchiaParams vp;
         std::memset(&vp, 0, sizeof(vp));
         vp.coords = p->coords.as<double>();
@@ -1247,7 +1330,16 @@ int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_mode
         vp.partial = p->partial.as<double>();
         vp.status = p->status.as<int32_t>() + b0;
         vp.points_per_block = ppb;
-        hipError_t e = gs::launch_vecchia(p->D, NB, 0, vp, n_blocks, s);
+        std::string key = "vecchia:" + std::to_string(p->ctx->device) + ":" + std::to_string(p->D) +
+                          ":" + std::to_string(NB) + ":" + std::to_string(nm) + ":" +
+                          model_signature(models[b0]) + (lean ? "lean" : "gen");
+        int variant = 0;
+        const int64_t tune_blocks = std::min<int64_t>(n_blocks, 4096);
+        int32_t stt = tuned_variant(key, s, [&](int v) {
+            return gs::launch_vecchia(p->D, NB, v, lean, vp, tune_blocks, s);
+        }, &variant);
+        if (stt != GS_OK) return stt;
+        hipError_t e = gs::launch_vecchia(p->D, NB, variant, lean, vp, n_blocks, s);
         if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
         HIP_TRY(e);
         HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm, p->out.as<double>() + b0, s));

@@ -78,8 +78,12 @@ struct VecchiaParams {
 
 hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s);
 // Returns hipErrorInvalidValue when (D, NB, M) has no instantiation.
-hipError_t launch_krige_fp64(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
-                             hipStream_t s);
+// variant: group-size choice (0/1); lean: every structure is spherical,
+// exponential, Gaussian or closed-form Matern.  Returns
+// hipErrorInvalidDeviceFunction for a (variant, class) pair that is not
+// compiled (general-class kernels exist for variant 0 only).
+hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, const SolveParams &p,
+                             int64_t grid_groups, hipStream_t s);
 hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
                               hipStream_t s);
 hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
@@ -87,24 +91,26 @@ hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, i
 hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
                          int64_t list_cap, hipStream_t s);
-hipError_t launch_vecchia(int D, int NB, int fp32, const VecchiaParams &p, int64_t n_blocks,
-                          hipStream_t s);
+hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaParams &p,
+                          int64_t n_blocks, hipStream_t s);
 hipError_t launch_vecchia_reduce(const double *partial, int64_t n_blocks, int32_t n_models,
                                  double *out, hipStream_t s);
 
 // Per-instantiation launchers (explicit instantiations live in generated
 // translation units, one per template argument set, built in parallel).
-template <int D, int NB, int M>
+template <int D, int NB, int M, int G, bool LEAN>
 hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
-template <int D, int NB, int M>
-hipError_t launch_krige32_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
-template <int D, int NB>
+template <int D, int NB, int G, bool LEAN>
 hipError_t launch_vecchia_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
 
 // Models evaluated by one Vecchia launch (host splits larger batches).
 constexpr int kMaxModelsPerLaunch = 64;
 
-// Group size (lanes per system) for a padded system size NB.
-constexpr int group_size(int NB) { return NB == 16 ? 2 : (NB == 32 ? 8 : 16); }
+// Lanes per system (group size G) for a padded system size NB: two
+// compiled variants each, picked per configuration by the runtime tuner.
+constexpr int kVariants = 2;
+constexpr int group_size(int NB, int variant) {
+    return NB == 16 ? (variant ? 4 : 2) : NB == 32 ? (variant ? 16 : 8) : (variant ? 64 : 32);
+}
 
 }  // namespace gs

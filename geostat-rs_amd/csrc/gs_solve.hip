@@ -67,32 +67,40 @@ hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int D, int M>
+template <int D, int M, int V, bool LEAN>
 static hipError_t krige_nb(int NB, const SolveParams &p, int64_t gg, hipStream_t s) {
     switch (NB) {
-    case 16: return launch_krige_inst<D, 16, M>(p, gg, s);
-    case 32: return launch_krige_inst<D, 32, M>(p, gg, s);
-    case 64: return launch_krige_inst<D, 64, M>(p, gg, s);
+    case 16: return launch_krige_inst<D, 16, M, group_size(16, V), LEAN>(p, gg, s);
+    case 32: return launch_krige_inst<D, 32, M, group_size(32, V), LEAN>(p, gg, s);
+    case 64: return launch_krige_inst<D, 64, M, group_size(64, V), LEAN>(p, gg, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_krige_fp64(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
-                             hipStream_t s) {
+template <int D, int M>
+static hipError_t krige_v(int NB, int variant, bool lean, const SolveParams &p, int64_t gg,
+                          hipStream_t s) {
+    if (!lean) return variant ? hipErrorInvalidDeviceFunction : krige_nb<D, M, 0, false>(NB, p, gg, s);
+    return variant ? krige_nb<D, M, 1, true>(NB, p, gg, s) : krige_nb<D, M, 0, true>(NB, p, gg, s);
+}
+
+hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, const SolveParams &p,
+                             int64_t grid_groups, hipStream_t s) {
+    if (variant < 0 || variant >= kVariants) return hipErrorInvalidValue;
     if (D == 2) {
         switch (M) {
-        case 0: return krige_nb<2, 0>(NB, p, grid_groups, s);
-        case 1: return krige_nb<2, 1>(NB, p, grid_groups, s);
-        case 3: return krige_nb<2, 3>(NB, p, grid_groups, s);
-        case 6: return krige_nb<2, 6>(NB, p, grid_groups, s);
+        case 0: return krige_v<2, 0>(NB, variant, lean, p, grid_groups, s);
+        case 1: return krige_v<2, 1>(NB, variant, lean, p, grid_groups, s);
+        case 3: return krige_v<2, 3>(NB, variant, lean, p, grid_groups, s);
+        case 6: return krige_v<2, 6>(NB, variant, lean, p, grid_groups, s);
         default: return hipErrorInvalidValue;
         }
     } else if (D == 3) {
         switch (M) {
-        case 0: return krige_nb<3, 0>(NB, p, grid_groups, s);
-        case 1: return krige_nb<3, 1>(NB, p, grid_groups, s);
-        case 4: return krige_nb<3, 4>(NB, p, grid_groups, s);
-        case 10: return krige_nb<3, 10>(NB, p, grid_groups, s);
+        case 0: return krige_v<3, 0>(NB, variant, lean, p, grid_groups, s);
+        case 1: return krige_v<3, 1>(NB, variant, lean, p, grid_groups, s);
+        case 4: return krige_v<3, 4>(NB, variant, lean, p, grid_groups, s);
+        case 10: return krige_v<3, 10>(NB, variant, lean, p, grid_groups, s);
         default: return hipErrorInvalidValue;
         }
     }
@@ -113,22 +121,28 @@ hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_
     return hipGetLastError();
 }
 
-hipError_t launch_vecchia(int D, int NB, int fp32, const VecchiaParams &p, int64_t n_blocks,
-                          hipStream_t s) {
-    (void)fp32;
-    if (D == 2) {
-        switch (NB) {
-        case 16: return launch_vecchia_inst<2, 16>(p, n_blocks, s);
-        case 32: return launch_vecchia_inst<2, 32>(p, n_blocks, s);
-        case 64: return launch_vecchia_inst<2, 64>(p, n_blocks, s);
-        }
-    } else if (D == 3) {
-        switch (NB) {
-        case 16: return launch_vecchia_inst<3, 16>(p, n_blocks, s);
-        case 32: return launch_vecchia_inst<3, 32>(p, n_blocks, s);
-        case 64: return launch_vecchia_inst<3, 64>(p, n_blocks, s);
-        }
+template <int D, int V, bool LEAN>
+static hipError_t vecchia_nb(int NB, const VecchiaParams &p, int64_t nb, hipStream_t s) {
+    switch (NB) {
+    case 16: return launch_vecchia_inst<D, 16, group_size(16, V), LEAN>(p, nb, s);
+    case 32: return launch_vecchia_inst<D, 32, group_size(32, V), LEAN>(p, nb, s);
+    case 64: return launch_vecchia_inst<D, 64, group_size(64, V), LEAN>(p, nb, s);
+    default: return hipErrorInvalidValue;
     }
+}
+
+template <int D>
+static hipError_t vecchia_v(int NB, int variant, bool lean, const VecchiaParams &p, int64_t nb,
+                            hipStream_t s) {
+    if (!lean) return variant ? hipErrorInvalidDeviceFunction : vecchia_nb<D, 0, false>(NB, p, nb, s);
+    return variant ? vecchia_nb<D, 1, true>(NB, p, nb, s) : vecchia_nb<D, 0, true>(NB, p, nb, s);
+}
+
+hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaParams &p,
+                          int64_t n_blocks, hipStream_t s) {
+    if (variant < 0 || variant >= kVariants) return hipErrorInvalidValue;
+    if (D == 2) return vecchia_v<2>(NB, variant, lean, p, n_blocks, s);
+    if (D == 3) return vecchia_v<3>(NB, variant, lean, p, n_blocks, s);
     return hipErrorInvalidValue;
 }
 

@@ -53,9 +53,14 @@ __device__ __forceinline__ int bcast32(int v) {
     } else if constexpr (G == 8) {
         // ds_swizzle bit mode inside 32 lanes: src = (lane & 0x18) | SRC
         return __builtin_amdgcn_ds_swizzle(v, 0x18 | (SRC << 5));
-    } else {
-        static_assert(G == 16, "group size");
+    } else if constexpr (G == 16) {
         return __builtin_amdgcn_mov_dpp(v, 0x150 + SRC, 0xF, 0xF, false);  // row_share
+    } else if constexpr (G == 32) {
+        // ds_swizzle bit mode: every lane of each 32-lane half reads lane SRC
+        return __builtin_amdgcn_ds_swizzle(v, SRC << 5);
+    } else {
+        static_assert(G == 64, "group size");
+        return __builtin_amdgcn_readlane(v, SRC);   // one system per wavefront
     }
 }
 
@@ -88,6 +93,15 @@ __device__ __forceinline__ double gsum(double v) {
     if constexpr (G >= 4) v += dpp_d<0x4E>(v);                  // quad_perm(2,3,0,1)
     if constexpr (G >= 8) v += swz_d<0x1F | (4 << 10)>(v);      // lane ^ 4
     if constexpr (G >= 16) v += swz_d<0x1F | (8 << 10)>(v);     // lane ^ 8
+    if constexpr (G >= 32) v += swz_d<0x1F | (16 << 10)>(v);    // lane ^ 16
+    if constexpr (G >= 64) {
+        // the two 32-lane halves: same order in every lane
+        const double lo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 0));
+        const double hi = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 32),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 32));
+        v = lo + hi;
+    }
     return v;
 }
 
@@ -120,18 +134,37 @@ __device__ __forceinline__ void basis(const SolveParams &P, const double *x, dou
 
 constexpr int kWave = 64;
 
+// 1/sqrt(d): hardware estimate plus one Newton step (relative error ~1e-16
+// for normal d > 0; d <= 0 gives NaN/inf and is caught by the pivot test).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    const double y = __builtin_amdgcn_rsq(d);
+    return y * (1.5 - 0.5 * d * y * y);
+}
+
+// Per-group LDS column buffer: NB column entries + one slot per lane for
+// each of the Q right-hand sides (every lane writes its own slot, readers
+// take the owner's: no divergent branch inside the unrolled factorization),
+// padded to an odd number of 16-byte units so the groups of one wavefront
+// start on different banks.
+template <int NB, int Q, int G>
+struct ColStride {
+    static constexpr int raw = NB + Q * G;
+    static constexpr int value = ((raw + 1) / 2 % 2 == 0) ? ((raw + 1) / 2 + 1) * 2 : ((raw + 1) / 2) * 2;
+};
+
+
 // Core: builds C (rows owned by this lane), factors it and forward-solves
 // the Q right-hand sides held in y.  `rowx` are the coordinates of the
 // owned rows, `xy` the group's neighbour coordinates in LDS, `tg` the
 // target; y[0] is filled here with the covariance to the target.  Returns
 // false when a pivot falls below 1e-12 * c0 (the reference's singular
 // threshold relative to the matrix scale, linalg.rs:36-57, 135-140).
-template <int D, int NB, int G, int Q>
+template <int D, int NB, int G, int Q, bool LEAN>
 __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const int cnt,
                                                    const double (&rowx)[NB / G][D],
                                                    const double *xy, const double *tg,
-                                                   double *stage, const int lane, const int g,
-                                                   double (&y)[Q][NB / G]) {
+                                                   double *stage, double *col, const int lane,
+                                                   const int g, double (&y)[Q][NB / G]) {
     constexpr int R = NB / G;
     const double c0 = Mdl.c0;
     double a[R][NB];
@@ -139,6 +172,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         constexpr int r = decltype(RR)::value;
         constexpr int ncol = G * (r + 1);
         const int i = G * r + g;
+#pragma unroll 2
         for (int c = 0; c <= ncol; c++) {
             double v;
             double dh[D];
@@ -148,7 +182,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
 #pragma unroll
                 for (int d = 0; d < D; d++)
                     dh[d] = rowx[r][d] - (tgt ? tg[d] : xy[(c < NB ? c : NB - 1) * D + d]);
-                v = cov_offdiag<D>(Mdl, dh);
+                v = cov_offdiag<D, LEAN>(Mdl, dh);
             } else {
                 v = (c == i) ? c0 : 0.0;
             }
@@ -167,30 +201,33 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         constexpr int rj = j / G;
         const double d = bcast<G, oj>(a[rj][j]);
         ok = ok && (d > thresh);
-        const double inv = 1.0 / sqrt(d);
+        const double inv = rsqrt_nr(d);
         double l[R];
 #pragma unroll
         for (int r = rj; r < R; r++) l[r] = a[r][j] * inv;
         l[rj] = (g > oj) ? l[rj] : 0.0;
-        static_for<0, Q>([&](auto QQ) {
-            constexpr int qq = decltype(QQ)::value;
-            const double yj = bcast<G, oj>(y[qq][rj]) * inv;
+        // Publish column j (rows > j) and the scaled right-hand sides of
+        // row j through the group's LDS buffer; every lane then reads the
+        // column back with wide broadcast loads.
+#pragma unroll
+        for (int r = rj; r < R; r++) col[G * r + g] = l[r];
+#pragma unroll
+        for (int qq = 0; qq < Q; qq++) col[NB + qq * G + g] = y[qq][rj] * inv;
+        wave_sync();
+#pragma unroll
+        for (int qq = 0; qq < Q; qq++) {
+            const double yj = col[NB + qq * G + oj];
 #pragma unroll
             for (int r = rj; r < R; r++) y[qq][r] -= l[r] * yj;
             y[qq][rj] = (g == oj) ? yj : y[qq][rj];
-        });
-        static_for<rj, R>([&](auto RK) {
-            constexpr int rk = decltype(RK)::value;
-            static_for<0, G>([&](auto GK) {
-                constexpr int gk = decltype(GK)::value;
-                constexpr int k = G * rk + gk;
-                if constexpr (k > j) {
-                    const double lk = bcast<G, gk>(l[rk]);
+        }
+        static_for<j + 1, NB>([&](auto KK) {
+            constexpr int k = decltype(KK)::value;
+            const double lk = col[k];
 #pragma unroll
-                    for (int r = rk; r < R; r++) a[r][k] -= l[r] * lk;
-                }
-            });
+            for (int r = k / G; r < R; r++) a[r][k] -= l[r] * lk;
         });
+        wave_sync();
     });
     return ok;
 }
@@ -290,17 +327,19 @@ __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&
     return value;
 }
 
-template <int D, int NB, int G, int M>
+template <int D, int NB, int G, int M, bool LEAN>
 __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
     constexpr int R = NB / G;
     constexpr int Q = 2 + M;
     constexpr int GPW = kWave / G;
     __shared__ double s_xy[GPW * NB * D];
     __shared__ double s_stage[(NB + 1) * kWave];
+    __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, Q, G>::value];
     const int lane = threadIdx.x;
     const int g = lane % G;
     const int grp = lane / G;
     double *xy = s_xy + grp * NB * D;
+    double *colbuf = s_col + grp * ColStride<NB, Q, G>::value;
     const DevModel &Mdl = *P.model;
     const double c0 = Mdl.c0;
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
@@ -355,7 +394,7 @@ __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
                 for (int q = 0; q < M; q++) y[2 + q][r] = live ? f[q] : 0.0;
             }
         }
-        bool ok = build_factor_solve<D, NB, G, Q>(Mdl, cnt, rowx, xy, tg, s_stage, lane, g, y);
+        bool ok = build_factor_solve<D, NB, G, Q, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, y);
         wave_sync();  // the group's LDS rows are rewritten by its next system
 
         double value, var, lag;
@@ -398,17 +437,19 @@ __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
 // then blocks in index order by vecchia_reduce), so the result does not
 // depend on scheduling.
 
-template <int D, int NB, int G>
+template <int D, int NB, int G, bool LEAN>
 __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
     constexpr int R = NB / G;
     constexpr int GPW = kWave / G;
     __shared__ double s_xy[GPW * NB * D];
     __shared__ double s_stage[(NB + 1) * kWave];
     __shared__ double s_sum[GPW * kMaxModelsPerLaunch];
+    __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, 2, G>::value];
     const int lane = threadIdx.x;
     const int g = lane % G;
     const int grp = lane / G;
     double *xy = s_xy + grp * NB * D;
+    double *colbuf = s_col + grp * ColStride<NB, 2, G>::value;
     const int nm = P.n_models;
     for (int p = g; p < nm; p += G) s_sum[grp * kMaxModelsPerLaunch + p] = 0.0;
     wave_sync();
@@ -453,7 +494,7 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
                 double y[2][R];
 #pragma unroll
                 for (int r = 0; r < R; r++) y[1][r] = zr[r];
-                ok = build_factor_solve<D, NB, G, 2>(Mdl, cnt, rowx, xy, tg, s_stage, lane, g, y);
+                ok = build_factor_solve<D, NB, G, 2, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, y);
                 double s01 = 0.0, s00 = 0.0;
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -482,22 +523,20 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
 }
 
 // ---- launchers (instantiated per template argument set) -------------------
-template <int D, int NB, int M>
+template <int D, int NB, int M, int G, bool LEAN>
 hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s) {
-    constexpr int G = group_size(NB);
     constexpr int GPW = kWave / G;
     int64_t blocks = (grid_groups + GPW - 1) / GPW;
     if (blocks < 1) blocks = 1;
     if (blocks > (1 << 30)) blocks = 1 << 30;
-    hipLaunchKernelGGL((krige_fp64_kernel<D, NB, G, M>), dim3((unsigned)blocks), dim3(kWave), 0,
+    hipLaunchKernelGGL((krige_fp64_kernel<D, NB, G, M, LEAN>), dim3((unsigned)blocks), dim3(kWave), 0,
                        s, p);
     return hipGetLastError();
 }
 
-template <int D, int NB>
+template <int D, int NB, int G, bool LEAN>
 hipError_t launch_vecchia_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s) {
-    constexpr int G = group_size(NB);
-    hipLaunchKernelGGL((vecchia_kernel<D, NB, G>), dim3((unsigned)n_blocks), dim3(kWave), 0, s, p);
+    hipLaunchKernelGGL((vecchia_kernel<D, NB, G, LEAN>), dim3((unsigned)n_blocks), dim3(kWave), 0, s, p);
     return hipGetLastError();
 }
 
