@@ -72,13 +72,14 @@ def _gen_sources() -> list[Path]:
 
 
 def _deps_mtime() -> float:
-    return max(p.stat().st_mtime for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".hip", ".cpp"))
+    hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h")]
+    hdrs.append(HERE.parent / "include" / "geostat_hip.h")
+    return max(p.stat().st_mtime for p in hdrs)
 
 
 def _compile(src: Path, extra: list[str], incremental: bool = False) -> Path:
     obj = OBJ / (src.stem + ".o")
-    if incremental and obj.exists() and obj.stat().st_mtime > max(_deps_mtime(), src.stat().st_mtime,
-                                                                   (HERE.parent / "include" / "geostat_hip.h").stat().st_mtime):
+    if incremental and obj.exists() and obj.stat().st_mtime > max(_deps_mtime(), src.stat().st_mtime):
         return obj
     cmd = [HIPCC, *COMMON, *extra, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -93,6 +94,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
     OBJ.mkdir(parents=True, exist_ok=True)
     units: list[tuple[Path, list[str]]] = [
         (CSRC / "gs_search.hip", EXACT),
+        (CSRC / "gs_tile_search.hip", EXACT),
         (CSRC / "gs_solve.hip", []),
         (CSRC / "gs_host.cpp", EXACT),
     ]

@@ -412,8 +412,12 @@ struct gs_kriging {
     double r2 = std::numeric_limits<double>::infinity();
     gs::SearchXf xf{};
     DBuf coords, values, tcoords, tidx, dmodel;
+    // cell grid of the search-frame coordinates (tile search)
+    DBuf cg_start, cg_pts, cg_idx;
+    gs::CellGridView cg{};
     // per-call workspace (grows)
     DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
+    DBuf keys, keys_out, vals, perm, sort_tmp, flags;
     std::mutex mu;
     // measurement hook: (search start, search end / solve start, solve end)
     bool timing = false;
@@ -444,6 +448,88 @@ struct gs_vplan {
     DBuf coords, d_order, d_nb, d_count, zc, models, partial, out, status;
     std::mutex mu;
 };
+
+namespace {
+
+// Uniform grid over the search-frame coordinates, about two points per cell
+// over the bounding box (axes with ~zero extent get one cell), capped at
+// 4n + 64 cells; points stored cell by cell (x fastest).
+int32_t build_cell_grid(gs_kriging *K, const std::vector<double> &tc) {
+    const int D = K->D;
+    const int64_t n = K->n;
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) { mn[d] = INFINITY; mx[d] = -INFINITY; }
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < D; d++) {
+            mn[d] = std::fmin(mn[d], tc[(size_t)(i * D + d)]);
+            mx[d] = std::fmax(mx[d], tc[(size_t)(i * D + d)]);
+        }
+    double ext[3] = {0, 0, 0}, max_ext = 0.0;
+    for (int d = 0; d < D; d++) { ext[d] = mx[d] - mn[d]; max_ext = std::fmax(max_ext, ext[d]); }
+    bool active[3] = {false, false, false};
+    int n_active = 0;
+    double volume = 1.0;
+    for (int d = 0; d < D; d++) {
+        active[d] = ext[d] > max_ext * 1e-9 && ext[d] > 0.0;
+        if (active[d]) { n_active++; volume *= ext[d]; }
+    }
+    double cell = 1.0;
+    if (n_active > 0) cell = std::pow(volume / std::max(0.5 * (double)n, 1.0), 1.0 / n_active);
+    if (!(cell > 0.0) || !std::isfinite(cell)) cell = std::fmax(max_ext, 1.0);
+    int64_t nc[3] = {1, 1, 1};
+    for (;;) {
+        int64_t total = 1;
+        for (int d = 0; d < D; d++) {
+            nc[d] = active[d] ? std::max<int64_t>(1, (int64_t)std::ceil(ext[d] / cell)) : 1;
+            total *= nc[d];
+        }
+        if (total <= 4 * n + 64 && total < (int64_t)1 << 30) break;
+        cell *= 1.25;
+    }
+    gs::CellGridView &G = K->cg;
+    std::memset(&G, 0, sizeof(G));
+    for (int d = 0; d < 3; d++) {
+        G.origin[d] = d < D ? mn[d] : 0.0;
+        G.nc[d] = (int32_t)(d < D ? nc[d] : 1);
+    }
+    G.cell = cell;
+    G.inv_cell = 1.0 / cell;
+    const int64_t ncells = nc[0] * nc[1] * nc[2];
+    std::vector<int32_t> start((size_t)ncells + 1, 0);
+    std::vector<int64_t> cid((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t id = 0, mul = 1;
+        for (int d = 0; d < D; d++) {
+            const double f = (tc[(size_t)(i * D + d)] - G.origin[d]) * G.inv_cell;
+            int64_t v = f <= 0.0 ? 0 : (f >= (double)(nc[d] - 1) ? nc[d] - 1 : (int64_t)f);
+            id += v * mul;
+            mul *= nc[d];
+        }
+        cid[(size_t)i] = id;
+        start[(size_t)id + 1]++;
+    }
+    for (int64_t c = 0; c < ncells; c++) start[(size_t)c + 1] += start[(size_t)c];
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    std::vector<double> pts((size_t)(n * D));
+    std::vector<uint32_t> idx((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t at = fill[(size_t)cid[(size_t)i]]++;
+        idx[(size_t)at] = (uint32_t)i;
+        for (int d = 0; d < D; d++) pts[(size_t)at * D + d] = tc[(size_t)(i * D + d)];
+    }
+    HIP_TRY(K->cg_start.ensure(sizeof(int32_t) * (ncells + 1)));
+    HIP_TRY(K->cg_pts.ensure(sizeof(double) * n * D));
+    HIP_TRY(K->cg_idx.ensure(sizeof(uint32_t) * n));
+    HIP_TRY(hipMemcpy(K->cg_start.p, start.data(), sizeof(int32_t) * (ncells + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(K->cg_pts.p, pts.data(), sizeof(double) * n * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(K->cg_idx.p, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    G.start = K->cg_start.as<int32_t>();
+    G.pts = K->cg_pts.as<double>();
+    G.idx = K->cg_idx.as<uint32_t>();
+    return GS_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -613,6 +699,8 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
         HIP_TRY(K->tidx.ensure(sizeof(uint32_t) * n));
         HIP_TRY(hipMemcpy(K->tcoords.p, tree.data(), sizeof(double) * n * dim, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(K->tidx.p, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+        int32_t st2 = build_cell_grid(K.get(), tc);
+        if (st2 != GS_OK) return st2;
     }
     *out = K.release();
     return GS_OK;
@@ -671,8 +759,44 @@ int32_t run_neighbors(gs_kriging *K, const double *d_targets, int64_t T, hipStre
                                     stride, s));
         return GS_OK;
     }
+    // 1) Hilbert order of the targets' search-frame cells
+    const size_t tmp_bytes = gs::tile_sort_workspace(T);
+    HIP_TRY(K->keys.ensure(sizeof(uint64_t) * T));
+    HIP_TRY(K->keys_out.ensure(sizeof(uint64_t) * T));
+    HIP_TRY(K->vals.ensure(sizeof(int32_t) * T));
+    HIP_TRY(K->perm.ensure(sizeof(int32_t) * T));
+    HIP_TRY(K->sort_tmp.ensure(tmp_bytes));
+    HIP_TRY(K->flags.ensure(sizeof(int32_t) * (T + 4)));
+    HIP_TRY(gs::launch_tile_order(K->D, d_targets, T, K->xf, K->cg, K->keys.as<uint64_t>(),
+                                  K->keys_out.as<uint64_t>(), K->vals.as<int32_t>(),
+                                  K->perm.as<int32_t>(), K->sort_tmp.p, tmp_bytes, s));
+    // 2) tile search: canonical lists; boundary ties and scattered
+    //    wavefronts flagged
+    int32_t *flag_len = K->flags.as<int32_t>() + T;
+    HIP_TRY(gs::launch_zero_i32(flag_len, 1, s));
+    gs::TileParams tp;
+    std::memset(&tp, 0, sizeof(tp));
+    tp.G = K->cg;
+    tp.targets = d_targets;
+    tp.T = T;
+    tp.perm = K->perm.as<int32_t>();
+    tp.k = (int32_t)K->k_heap;
+    tp.per_octant = K->per_oct;
+    tp.max_nb = (int32_t)K->stride;
+    tp.stride = stride;
+    tp.r2 = K->r2;
+    tp.xf = K->xf;
+    tp.out_idx = K->nbr.as<int32_t>();
+    tp.out_count = K->count.as<int32_t>();
+    tp.flag_list = K->flags.as<int32_t>();
+    tp.flag_len = flag_len;
+    HIP_TRY(gs::launch_tile_knn(K->D, tp, s));
+    // 3) exact kd-tree replay (the reference's traversal order) of the
+    //    flagged targets
     gs::KnnParams p;
     std::memset(&p, 0, sizeof(p));
+    p.list = K->flags.as<int32_t>();
+    p.list_len = flag_len;
     p.tcoords = K->tcoords.as<double>();
     p.tidx = K->tidx.as<uint32_t>();
     p.n = K->n;
@@ -780,6 +904,14 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
             if (st != GS_OK) return st;
         }
         return GS_OK;
+    }
+    if (K->has_tree) {
+        // process targets in the search's Hilbert order: neighbouring systems
+        // share data points, so the coordinate gathers hit cache
+        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 4));
+        HIP_TRY(gs::launch_fill_i32(K->lens.as<int32_t>() + 3, 1, (int32_t)T, s));
+        p.list = K->perm.as<int32_t>();
+        p.list_len = K->lens.as<int32_t>() + 3;
     }
     return solve(pad_nb(K->nb_bound), p, T);
 }

@@ -21,7 +21,39 @@ struct SearchXf {
     double r[9];
 };
 
+// Uniform cell grid over the search-frame data coordinates (CSR by cell,
+// x fastest), used by the tile search.
+struct CellGridView {
+    double origin[3];
+    double cell;
+    double inv_cell;
+    int32_t nc[3];
+    int32_t pad;
+    const int32_t *start;    // [ncells + 1]
+    const double *pts;       // [n][D] search-frame coords in cell order
+    const uint32_t *idx;     // [n] original indices in cell order
+};
+
+struct TileParams {
+    CellGridView G;
+    const double *targets;   // [T][D] original frame
+    int64_t T;
+    const int32_t *perm;     // [T] targets in Hilbert order of their cell
+    int32_t k;               // heap mode list length
+    int32_t per_octant;      // > 0: octant mode
+    int32_t max_nb;          // octant mode truncation
+    int32_t stride;
+    double r2;
+    SearchXf xf;
+    int32_t *out_idx;
+    int32_t *out_count;
+    int32_t *flag_list;      // targets for the kd-tree replay (ties, scattered tiles)
+    int32_t *flag_len;
+};
+
 struct KnnParams {
+    const int32_t *list;     // optional target ids (exact replay of flagged targets)
+    const int32_t *list_len; // device counter for list
     const double *tcoords;   // [n][D] search-frame coords in kd-tree layout order
     const uint32_t *tidx;    // kd-tree layout position -> original index
     int64_t n;
@@ -77,6 +109,13 @@ struct VecchiaParams {
 };
 
 hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s);
+size_t tile_sort_workspace(int64_t T);
+hipError_t launch_tile_order(int D, const double *targets, int64_t T, const SearchXf &xf,
+                             const CellGridView &G, uint64_t *keys, uint64_t *keys_out,
+                             int32_t *vals, int32_t *perm, void *tmp, size_t tmp_bytes,
+                             hipStream_t s);
+hipError_t launch_tile_knn(int D, const TileParams &p, hipStream_t s);
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t s);
 // Returns hipErrorInvalidValue when (D, NB, M) has no instantiation.
 // variant: group-size choice (0/1); lean: every structure is spherical,
 // exponential, Gaussian or closed-form Matern.  Returns

@@ -15,6 +15,8 @@
 // lanes walk nearly the same tree path.  The heap lives in LDS with a
 // lane-interleaved layout (entry e of lane l at [e][l]), conflict-free.
 // The traversal stack (pending far subtrees) is a small private array.
+#include <algorithm>
+
 #include "gs_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -124,9 +126,10 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(KnnParams P) {
     H.ix = reinterpret_cast<uint32_t *>(H.d + (size_t)slots * kBlock);
     H.lane = lane;
 
-    const int64_t t = (int64_t)blockIdx.x * kBlock + lane;
-    if (t >= P.T) return;
-
+    const int64_t nt = P.list ? (int64_t)(*P.list_len) : P.T;
+    for (int64_t it = (int64_t)blockIdx.x * kBlock + lane; it < nt;
+         it += (int64_t)gridDim.x * kBlock) {
+    const int64_t t = P.list ? (int64_t)P.list[it] : it;
     double tg[D], st[D];
 #pragma unroll
     for (int d = 0; d < D; d++) tg[d] = P.targets[t * D + d];
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(KnnParams P) {
         for (int e = 0; e < len; e++) out[e] = (int32_t)H.IX(e);
         for (int e = len; e < P.stride; e++) out[e] = -1;
         P.out_count[t] = len;
-        return;
+        continue;
     }
     // Octant mode: sort each octant, then merge by (d2, idx) and truncate
     // to max_neighbors (kriging.rs:486-488).
@@ -260,6 +263,7 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(KnnParams P) {
     }
     for (int e = outn; e < P.stride; e++) out[e] = -1;
     P.out_count[t] = outn;
+    }
 }
 
 hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s) {
@@ -267,7 +271,9 @@ hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s) {
     const bool oct = p.per_octant > 0;
     const int slots = oct ? (p.per_octant << D) : p.k;
     const size_t lds = (size_t)slots * kBlock * (sizeof(double) + sizeof(uint32_t));
-    const dim3 grid((unsigned)((p.T + kBlock - 1) / kBlock));
+    int64_t blocks = (p.T + kBlock - 1) / kBlock;
+    if (p.list) blocks = std::min<int64_t>(blocks, 2048);   // grid-stride over the list
+    const dim3 grid((unsigned)blocks);
     if (D == 2) {
         if (oct) hipLaunchKernelGGL((knn_kernel<2, true>), grid, dim3(kBlock), lds, s, p);
         else hipLaunchKernelGGL((knn_kernel<2, false>), grid, dim3(kBlock), lds, s, p);

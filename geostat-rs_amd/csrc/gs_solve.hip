@@ -48,9 +48,9 @@ __global__ void iota_nbr_kernel(int32_t *nbr, int32_t *count, int64_t T, int32_t
     count[t] = n;
 }
 
-__global__ void zero_i32_kernel(int32_t *p, int64_t n) {
+__global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t value) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    if (i < n) p[i] = value;
 }
 
 hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
@@ -61,11 +61,14 @@ hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, i
     return hipGetLastError();
 }
 
-hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n,
+                       value);
     return hipGetLastError();
 }
+
+hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) { return launch_fill_i32(p, n, 0, s); }
 
 template <int D, int M, int V, bool LEAN>
 static hipError_t krige_nb(int NB, const SolveParams &p, int64_t gg, hipStream_t s) {

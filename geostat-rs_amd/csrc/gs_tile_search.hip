@@ -1,0 +1,565 @@
+// Tile neighbour search on gfx950: one wavefront per 64 spatially close
+// targets, walking cell rings of a uniform grid in lockstep.
+//
+// Result contract: the same lists as KdTree::k_nearest (search.rs:57-99).
+// The reference keeps a bounded max-heap keyed (d2 bits, idx) that accepts a
+// candidate into a full heap only if d2 < worst; its final set is every point
+// with d2 below the k-th distance W plus some points with d2 == W, chosen by
+// the traversal order.  When no excluded point has d2 == W that set is the
+// canonical "k smallest by (d2, idx)", which is what each lane computes here
+// (visiting order does not matter for a canonical selection).  A lane also
+// records the smallest distance it rejected or evicted; if that equals W the
+// target is a boundary tie and is handed to the exact kd-tree replay
+// (gs_search.hip), which reproduces the reference's traversal order.
+// Octant lists (kriging.rs:471-488 with k = n) are canonical by construction.
+//
+// Distances use the reference's expression ((0 + dx^2) + dy^2) [+ dz^2] with
+// contraction off, on search-frame coordinates transformed exactly as the
+// reference does, so every compared d2 is bit-identical.
+//
+// Ordering: targets are sorted by the Hilbert index of their search-frame
+// cell (Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 2004: axes-to-transpose, then bit interleave), so consecutive targets lie
+// in adjacent cells and a wavefront covers a small box of cells.  A
+// wavefront whose box exceeds kMaxBoxCells (the curve leaving and
+// re-entering a non-power-of-two grid, or sparse targets) hands all its
+// targets to the kd-tree replay instead.
+//
+// Walk: ring r is the box grown by r cells (clipped to the grid) minus the
+// region already visited, enumerated as row segments (x-runs).  The 64 lanes
+// resolve up to 64 rows' segments at once, prefix-sum their lengths, and
+// then load up to 64 candidates per step cooperatively; each candidate is
+// broadcast with v_readlane and offered to every lane's top-k.  After each
+// ring a lane stops when the distance from its target to the unvisited part
+// of the grid (outside the visited rectangle) exceeds its k-th distance or
+// the search radius; the wavefront stops when all lanes have.
+#include <hipcub/hipcub.hpp>
+
+#include "gs_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace gs {
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kMaxBoxCells = 64;
+
+template <int D>
+__device__ __forceinline__ void xf_target(const SearchXf &X, const double *p, double *out) {
+    if (!X.on) {
+#pragma unroll
+        for (int d = 0; d < D; d++) out[d] = p[d];
+        return;
+    }
+    if (D == 2) {
+        out[0] = p[0] * X.s + p[1] * X.c;
+        out[1] = (p[0] * X.c - p[1] * X.s) / X.ratio;
+    } else {
+        out[0] = X.r[0] * p[0] + X.r[1] * p[1] + X.r[2] * p[2];
+        out[1] = X.r[3] * p[0] + X.r[4] * p[1] + X.r[5] * p[2];
+        out[2] = X.r[6] * p[0] + X.r[7] * p[1] + X.r[8] * p[2];
+    }
+}
+
+// Cell of a search-frame position, clamped to the grid (same rule as the
+// host's point binning); NaN maps to cell 0.
+template <int D>
+__device__ __forceinline__ void cell_of(const CellGridView &G, const double *p, int *c) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const double f = (p[d] - G.origin[d]) * G.inv_cell;
+        int v;
+        if (!(f == f)) v = 0;
+        else if (f <= 0.0) v = 0;
+        else if (f >= (double)(G.nc[d] - 1)) v = G.nc[d] - 1;
+        else v = (int)f;
+        c[d] = v;
+    }
+}
+
+// Hilbert index of a cell, `bits` bits per axis (Skilling 2004).
+template <int D>
+__device__ __forceinline__ uint64_t hilbert_index(const int *c, int bits) {
+    uint32_t X[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) X[i] = (uint32_t)c[i];
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 1; i < D; i++) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[D - 1] & Q) t ^= Q - 1;
+#pragma unroll
+    for (int i = 0; i < D; i++) X[i] ^= t;
+    uint64_t key = 0;
+    for (int b = bits - 1; b >= 0; b--) {
+#pragma unroll
+        for (int i = 0; i < D; i++) key = (key << 1) | ((X[i] >> b) & 1u);
+    }
+    return key;
+}
+
+template <int D>
+__global__ void hilbert_kernel(const double *targets, int64_t T, SearchXf xf, CellGridView G,
+                               int bits, int shift, uint64_t *keys, int32_t *vals) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    double p[D], st[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) p[d] = targets[t * D + d];
+    xf_target<D>(xf, p, st);
+    int c[D];
+    cell_of<D>(G, st, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d] >>= shift;
+    keys[t] = hilbert_index<D>(c, bits);
+    vals[t] = (int32_t)t;
+}
+
+// Bounded max-heap over (d2, idx) in LDS, lane-interleaved (entry e of
+// lane l at [e][l]: conflict-free).
+struct TileHeap {
+    double *d;
+    uint32_t *ix;
+    int lane;
+    __device__ __forceinline__ double D2(int e) const { return d[e * kTile + lane]; }
+    __device__ __forceinline__ uint32_t IX(int e) const { return ix[e * kTile + lane]; }
+    __device__ __forceinline__ void set(int e, double v, uint32_t i) {
+        d[e * kTile + lane] = v;
+        ix[e * kTile + lane] = i;
+    }
+    static __device__ __forceinline__ bool gt(double ad, uint32_t ai, double bd, uint32_t bi) {
+        return ad > bd || (ad == bd && ai > bi);
+    }
+    __device__ __forceinline__ void push(int &len, double v, uint32_t i) {
+        int e = len++;
+        while (e > 0) {
+            const int p = (e - 1) >> 1;
+            const double pd = D2(p);
+            const uint32_t pi = IX(p);
+            if (!gt(v, i, pd, pi)) break;
+            set(e, pd, pi);
+            e = p;
+        }
+        set(e, v, i);
+    }
+    // Sift (v, i) down from slot e; returns the value left in slot e.
+    __device__ __forceinline__ void sift_down(int len, int e, double v, uint32_t i, double &top_d,
+                                              uint32_t &top_i) {
+        top_d = v;
+        top_i = i;
+        bool first = true;
+        for (;;) {
+            const int l = 2 * e + 1;
+            if (l >= len) break;
+            int c = l;
+            double cd = D2(l);
+            uint32_t ci = IX(l);
+            if (l + 1 < len) {
+                const double rd = D2(l + 1);
+                const uint32_t ri = IX(l + 1);
+                if (gt(rd, ri, cd, ci)) { c = l + 1; cd = rd; ci = ri; }
+            }
+            if (!gt(cd, ci, v, i)) break;
+            set(e, cd, ci);
+            if (first) { top_d = cd; top_i = ci; first = false; }
+            e = c;
+        }
+        set(e, v, i);
+    }
+    __device__ __forceinline__ void sort(int len) {
+        double td;
+        uint32_t ti;
+        for (int end = len - 1; end > 0; end--) {
+            const double rd = D2(0);
+            const uint32_t ri = IX(0);
+            const double vd = D2(end);
+            const uint32_t vi = IX(end);
+            set(end, rd, ri);
+            sift_down(end, 0, vd, vi, td, ti);
+        }
+    }
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < kTile; o <<= 1) {
+        const int y = __shfl_up(v, o);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+__device__ __forceinline__ double readlane_f64(double v, int j) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Dynamic LDS: the heaps ([slots][64] doubles, then [slots][64] indices),
+// then the row table of the ring being walked.
+struct RowTable {
+    int32_t excl[kTile];   // exclusive prefix of the rows' candidate counts
+    int32_t alen[kTile];   // length of the row's first segment
+    int32_t a0[kTile];     // first point of segment a
+    int32_t b0[kTile];     // first point of segment b
+};
+
+template <int D, bool OCT>
+__global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lane = threadIdx.x;
+    const int slots = OCT ? (P.per_octant << D) : P.k;
+    TileHeap H;
+    H.d = reinterpret_cast<double *>(smem_raw);
+    H.ix = reinterpret_cast<uint32_t *>(H.d + (size_t)slots * kTile);
+    H.lane = lane;
+    RowTable *RT = reinterpret_cast<RowTable *>(smem_raw + (size_t)slots * kTile * 12);
+    const CellGridView &G = P.G;
+
+    const int64_t pos = (int64_t)blockIdx.x * kTile + lane;
+    const bool live = pos < P.T;
+    const int64_t t = live ? (int64_t)P.perm[pos] : -1;
+    double st[D], ft[D];
+    int cl[3] = {0, 0, 0};
+    bool nan_target = false;
+    if (live) {
+        double p[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) p[d] = P.targets[t * D + d];
+        xf_target<D>(P.xf, p, st);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            nan_target = nan_target || !(st[d] == st[d]);
+            ft[d] = (st[d] - G.origin[d]) * G.inv_cell;
+        }
+        cell_of<D>(G, st, cl);
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) st[d] = ft[d] = 0.0;
+    }
+    const bool active = live && !nan_target;
+
+    // box of the active lanes' cells
+    const uint64_t active_mask = __ballot(active);
+    int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+    int64_t box_cells = 0;
+    if (active_mask != 0) {
+        box_cells = 1;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            blo[d] = wave_min(active ? cl[d] : INT32_MAX);
+            bhi[d] = wave_max(active ? cl[d] : INT32_MIN);
+            box_cells *= (int64_t)(bhi[d] - blo[d] + 1);
+        }
+    }
+    if (active_mask != 0 && box_cells > kMaxBoxCells) {
+        // scattered wavefront: every live target goes to the kd-tree replay
+        const uint64_t live_mask = __ballot(live);
+        int base = 0;
+        if (lane == 0) base = atomicAdd(P.flag_len, (int)__popcll(live_mask));
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (live) {
+            const int rank = (int)__popcll(live_mask & ((1ull << lane) - 1ull));
+            P.flag_list[base + rank] = (int32_t)t;
+        }
+        return;
+    }
+
+    const double r2 = P.r2;
+    const int k = P.k;
+    const int q = P.per_octant;
+    int len = 0;
+    double top_d = INFINITY;        // heap mode: root of the full heap
+    uint32_t top_i = 0;
+    int olen[1 << D];
+#pragma unroll
+    for (int o = 0; o < (1 << D); o++) olen[o] = 0;
+    double rej = INFINITY;
+    bool done = !active;
+
+    int vlo[3] = {0, 0, 0}, vhi[3] = {0, 0, 0};   // visited rectangle
+    bool visited_any = false;
+    for (int r = 0; active_mask != 0; r++) {
+        int elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
+        bool grew = !visited_any;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            elo[d] = max(blo[d] - r, 0);
+            ehi[d] = min(bhi[d] + r, G.nc[d] - 1);
+            grew = grew || elo[d] != vlo[d] || ehi[d] != vhi[d];
+        }
+        if (!grew) break;                           // whole grid visited
+        const int ny = D >= 2 ? ehi[1] - elo[1] + 1 : 1;
+        const int nz = D == 3 ? ehi[2] - elo[2] + 1 : 1;
+        const int nrows = ny * nz;
+        for (int rb = 0; rb < nrows; rb += kTile) {
+            const int ri = rb + lane;
+            int32_t a0 = 0, alen = 0, b0 = 0, blen = 0;
+            if (ri < nrows) {
+                const int y = D >= 2 ? elo[1] + ri % ny : 0;
+                const int z = D == 3 ? elo[2] + ri / ny : 0;
+                bool inner = visited_any;
+                if (D >= 2) inner = inner && y >= vlo[1] && y <= vhi[1];
+                if (D == 3) inner = inner && z >= vlo[2] && z <= vhi[2];
+                const int64_t row = ((int64_t)z * G.nc[1] + y) * G.nc[0];
+                if (!inner) {
+                    a0 = G.start[row + elo[0]];
+                    alen = G.start[row + ehi[0] + 1] - a0;
+                } else {
+                    if (elo[0] < vlo[0]) {
+                        a0 = G.start[row + elo[0]];
+                        alen = G.start[row + elo[0] + 1] - a0;
+                    }
+                    if (ehi[0] > vhi[0]) {
+                        b0 = G.start[row + ehi[0]];
+                        blen = G.start[row + ehi[0] + 1] - b0;
+                    }
+                }
+            }
+            const int rlen = alen + blen;
+            const int incl = wave_incl_scan(rlen, lane);
+            const int total = __builtin_amdgcn_readlane(incl, kTile - 1);
+            RT->excl[lane] = incl - rlen;
+            RT->alen[lane] = alen;
+            RT->a0[lane] = a0;
+            RT->b0[lane] = b0;
+            __syncthreads();
+            for (int cb = 0; cb < total; cb += kTile) {
+                const int g = cb + lane;
+                double pc[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) pc[d] = 0.0;
+                uint32_t pid = 0;
+                if (g < total) {
+                    int at = 0;    // last row whose exclusive prefix is <= g
+#pragma unroll
+                    for (int step = kTile / 2; step > 0; step >>= 1)
+                        if (RT->excl[at + step] <= g) at += step;
+                    const int o = g - RT->excl[at];
+                    const int al = RT->alen[at];
+                    const int64_t p = o < al ? (int64_t)RT->a0[at] + o : (int64_t)RT->b0[at] + (o - al);
+#pragma unroll
+                    for (int d = 0; d < D; d++) pc[d] = G.pts[p * D + d];
+                    pid = G.idx[p];
+                }
+                const int cnt = min(kTile, total - cb);
+                for (int j = 0; j < cnt; j++) {
+                    double cp[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) cp[d] = readlane_f64(pc[d], j);
+                    const uint32_t id = (uint32_t)__builtin_amdgcn_readlane((int)pid, j);
+                    if (done) continue;
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        const double dd = cp[d] - st[d];
+                        d2 += dd * dd;
+                    }
+                    if (!(d2 <= r2)) continue;
+                    if (!OCT) {
+                        if (len < k) {
+                            H.push(len, d2, id);
+                            if (len == k) { top_d = H.D2(0); top_i = H.IX(0); }
+                        } else if (TileHeap::gt(top_d, top_i, d2, id)) {
+                            rej = fmin(rej, top_d);
+                            H.sift_down(len, 0, d2, id, top_d, top_i);
+                        } else {
+                            rej = fmin(rej, d2);
+                        }
+                    } else {
+                        int oct = 0;
+#pragma unroll
+                        for (int d = 0; d < D; d++)
+                            if (cp[d] >= st[d]) oct |= 1 << d;
+                        TileHeap O = H;
+                        O.d = H.d + (size_t)oct * q * kTile;
+                        O.ix = H.ix + (size_t)oct * q * kTile;
+                        int ol = olen[0];
+#pragma unroll
+                        for (int o = 1; o < (1 << D); o++) if (oct == o) ol = olen[o];
+                        if (ol < q) {
+                            O.push(ol, d2, id);
+#pragma unroll
+                            for (int o = 0; o < (1 << D); o++) if (oct == o) olen[o] = ol;
+                        } else if (TileHeap::gt(O.D2(0), O.IX(0), d2, id)) {
+                            double td;
+                            uint32_t ti;
+                            O.sift_down(ol, 0, d2, id, td, ti);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) { vlo[d] = elo[d]; vhi[d] = ehi[d]; }
+        visited_any = true;
+
+        if (!done) {
+            // distance (in cells) from the target to the unvisited cells:
+            // points of a cell j have j <= f <= j + 1 on that axis
+            double lbc = INFINITY;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                if (vlo[d] > 0) lbc = fmin(lbc, ft[d] - (double)vlo[d]);
+                if (vhi[d] < G.nc[d] - 1) lbc = fmin(lbc, (double)(vhi[d] + 1) - ft[d]);
+            }
+            if (lbc == INFINITY) {
+                done = true;                        // nothing left unvisited
+            } else {
+                // slack for the rounding of f and of the compared d2
+                lbc = lbc * (1.0 - 1e-9) - 1e-6;
+                double bound = r2;
+                if (!OCT) {
+                    if (len == k) bound = fmin(bound, top_d);
+                } else {
+                    bool full = true;
+                    double worst = 0.0;
+#pragma unroll
+                    for (int o = 0; o < (1 << D); o++) {
+                        full = full && olen[o] == q;
+                        if (olen[o] > 0) worst = fmax(worst, H.d[((size_t)o * q) * kTile + lane]);
+                    }
+                    if (full) bound = fmin(bound, worst);
+                }
+                if (lbc > 0.0) {
+                    const double lb = lbc * G.cell;
+                    if (lb * lb > bound) done = true;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+    }
+    if (!live) return;
+
+    int32_t *out = P.out_idx + t * P.stride;
+    if (!OCT) {
+        const bool tie = len == k && rej == top_d;
+        H.sort(len);
+        for (int e = 0; e < len; e++) out[e] = (int32_t)H.IX(e);
+        for (int e = len; e < P.stride; e++) out[e] = -1;
+        P.out_count[t] = len;
+        if (tie) {
+            const int slot = atomicAdd(P.flag_len, 1);
+            P.flag_list[slot] = (int32_t)t;
+        }
+        return;
+    }
+    int head[1 << D];
+#pragma unroll
+    for (int o = 0; o < (1 << D); o++) {
+        TileHeap O = H;
+        O.d = H.d + (size_t)o * q * kTile;
+        O.ix = H.ix + (size_t)o * q * kTile;
+        O.sort(olen[o]);
+        head[o] = 0;
+    }
+    int total = 0;
+#pragma unroll
+    for (int o = 0; o < (1 << D); o++) total += olen[o];
+    const int outn = total < P.max_nb ? total : P.max_nb;
+    for (int e = 0; e < outn; e++) {
+        int best = -1;
+        double bd = 0.0;
+        uint32_t bi = 0;
+#pragma unroll
+        for (int o = 0; o < (1 << D); o++) {
+            if (head[o] < olen[o]) {
+                const double od = H.d[((size_t)o * q + head[o]) * kTile + lane];
+                const uint32_t oi = H.ix[((size_t)o * q + head[o]) * kTile + lane];
+                if (best < 0 || TileHeap::gt(bd, bi, od, oi)) { best = o; bd = od; bi = oi; }
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < (1 << D); o++) if (best == o) head[o]++;
+        out[e] = (int32_t)bi;
+    }
+    for (int e = outn; e < P.stride; e++) out[e] = -1;
+    P.out_count[t] = outn;
+}
+
+}  // namespace
+
+size_t tile_sort_workspace(int64_t T) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t *)nullptr,
+                                             (uint64_t *)nullptr, (const int32_t *)nullptr,
+                                             (int32_t *)nullptr, (int)T);
+    return bytes;
+}
+
+hipError_t launch_tile_order(int D, const double *targets, int64_t T, const SearchXf &xf,
+                             const CellGridView &G, uint64_t *keys, uint64_t *keys_out,
+                             int32_t *vals, int32_t *perm, void *tmp, size_t tmp_bytes,
+                             hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    if (D != 2 && D != 3) return hipErrorInvalidValue;
+    // bits per axis covering the largest cell index; coarser cells when the
+    // interleaved index would not fit in 64 bits
+    int need = 1;
+    for (int d = 0; d < D; d++)
+        while (need < 31 && (int64_t)(G.nc[d] - 1) >> need) need++;
+    const int bits = std::min(need, 64 / D);
+    const int shift = need - bits;
+    const dim3 grid((unsigned)((T + 255) / 256));
+    if (D == 2)
+        hipLaunchKernelGGL(hilbert_kernel<2>, grid, dim3(256), 0, s, targets, T, xf, G, bits, shift,
+                           keys, vals);
+    else
+        hipLaunchKernelGGL(hilbert_kernel<3>, grid, dim3(256), 0, s, targets, T, xf, G, bits, shift,
+                           keys, vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_out, vals, perm, (int)T, 0,
+                                              D * bits, s);
+}
+
+static size_t tile_lds_bytes(int D, int k, int per_octant) {
+    const int slots = per_octant > 0 ? (per_octant << D) : k;
+    return (size_t)slots * kTile * (sizeof(double) + sizeof(uint32_t)) + sizeof(RowTable);
+}
+
+hipError_t launch_tile_knn(int D, const TileParams &p, hipStream_t s) {
+    if (p.T == 0) return hipSuccess;
+    const bool oct = p.per_octant > 0;
+    const size_t lds = tile_lds_bytes(D, p.k, p.per_octant);
+    const dim3 grid((unsigned)((p.T + kTile - 1) / kTile));
+    if (D == 2) {
+        if (oct) hipLaunchKernelGGL((tile_knn_kernel<2, true>), grid, dim3(kTile), lds, s, p);
+        else hipLaunchKernelGGL((tile_knn_kernel<2, false>), grid, dim3(kTile), lds, s, p);
+    } else if (D == 3) {
+        if (oct) hipLaunchKernelGGL((tile_knn_kernel<3, true>), grid, dim3(kTile), lds, s, p);
+        else hipLaunchKernelGGL((tile_knn_kernel<3, false>), grid, dim3(kTile), lds, s, p);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gs
