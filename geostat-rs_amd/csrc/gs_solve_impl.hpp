@@ -4,9 +4,9 @@
 // data-data covariance block C (n <= NB) is distributed by rows over the
 // group: lane g owns rows i = G*r + g (r < R = NB/G) and keeps, for each
 // owned row, the columns up to the end of its row block, entirely in VGPRs.
-// A right-looking Cholesky runs with the pivot row and the column-j
-// multipliers broadcast inside the group by DPP (quad_perm / row_share) or
-// ds_swizzle, so no LDS traffic sits in the O(n^3) loop.
+// A right-looking Cholesky runs with the pivot broadcast inside the group by
+// DPP (quad_perm / row_share), ds_swizzle or readlane, and column j of L
+// published through a small per-group LDS buffer.
 //
 // The reference solves the bordered (n+m)^2 system [C F; F' 0][w; mu] =
 // [b; f] by LU with partial pivoting (kriging.rs:527-644, linalg.rs:27-111).
@@ -21,8 +21,9 @@
 // are the m = 0 case of the same routine.
 //
 // Covariance entries are evaluated by a runtime loop into a lane-private
-// LDS staging row and then moved to registers with compile-time indices, so
-// the model code appears once per row block instead of once per entry.
+// LDS staging column and then moved to registers with compile-time
+// indices, so the model code appears once per pair of row blocks instead of
+// once per entry.
 #pragma once
 
 #include <type_traits>
@@ -153,64 +154,218 @@ struct ColStride {
 };
 
 
+// The value, made opaque to the optimizer.  The lane-dependent compares of
+// the unrolled factorization then stay where they are used instead of being
+// hoisted out of the system loop into long-lived (spilled) lane masks.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Row ownership: in row block r, lane g of a group owns row G*r + g (r
+// even) or G*r + G-1-g (r odd).  Blocks r and R-1-r then hold rows summing
+// to NB-1 in every lane, so the lower-triangle entries of C split evenly
+// over the group.
+template <int G>
+__device__ __forceinline__ int row_of(int r, int g) { return G * r + ((r & 1) ? G - 1 - g : g); }
+template <int G>
+constexpr int owner_of(int j) { return ((j / G) & 1) ? G - 1 - j % G : j % G; }
+
+// One lean structure's parameters, loaded once per system into registers.
+// Exponential-family kinds share g = 1 - (1 + beta q + gam q^2) exp(-q')
+// with q = alpha d and q' = d^2 (Gaussian) or q (model.rs:122-181).
+template <int D>
+struct LeanRegs {
+    double t[D * D];    // anisotropy matrix divided by the range
+    double ir2;         // 1 / range^2 (isotropic structures)
+    double sill, alpha, beta, gam;
+    bool aniso, sph, gau;
+};
+
+template <int D>
+__device__ __forceinline__ LeanRegs<D> load_lean(const DevStructure &S) {
+    LeanRegs<D> L;
+    const double ir = S.inv_range;
+#pragma unroll
+    for (int i = 0; i < D * D; i++) L.t[i] = S.m[i] * ir;
+    L.ir2 = ir * ir;
+    L.sill = S.sill;
+    L.aniso = S.aniso != 0;
+    L.sph = S.kind == K_SPH;
+    L.gau = S.kind == K_GAU;
+    L.alpha = S.kind == K_MAT15 ? 1.7320508075688772 : (S.kind == K_MAT25 ? 2.23606797749979 : 1.0);
+    L.beta = (S.kind == K_MAT15 || S.kind == K_MAT25) ? 1.0 : 0.0;
+    L.gam = S.kind == K_MAT25 ? 1.0 / 3.0 : 0.0;
+    return L;
+}
+
+// sill * g(|T dh| / range) for a lean structure.
+template <int D>
+__device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *dh) {
+    double h2s = 0.0;
+    if (L.aniso) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            double e = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) e = fma(L.t[k * D + d], dh[d], e);
+            h2s = fma(e, e, h2s);
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) h2s = fma(dh[d], dh[d], h2s);
+        h2s *= L.ir2;
+    }
+    const double d = sqrt_fast(h2s);
+    double g;
+    if (L.sph) {
+        g = d >= 1.0 ? 1.0 : 1.5 * d - 0.5 * d * d * d;
+    } else {
+        const double q = L.alpha * d;
+        const double p = fma(fma(L.gam, q, L.beta), q, 1.0);
+        g = 1.0 - p * exp_neg(L.gau ? h2s : q);
+    }
+    return L.sill * g;
+}
+
+// Accumulates sum_s sill_s g_s over the entries of one pair of row blocks
+// into the lane's staging column: slot e < iA is (row iA, column e), slot
+// iA <= e < iA+iB is (row iB, column e-iA), slot iA+iB is (row iA,
+// target), slot iA+iB+1 is (row iB, target).  Slots past those hold
+// unused values.  The structure loop is outside the entry loop so each
+// structure's parameters are read once.
+template <int D, int NB, bool LEAN>
+__device__ __forceinline__ void accum_entries(const DevModel &Mdl, const int E, const int iA,
+                                              const int iB, const double (&xA)[D],
+                                              const double (&xB)[D], const double *xy,
+                                              const double *tg, double *stage, const int lane) {
+    const int tA = iA + iB;
+    const int ns = Mdl.ns;
+    for (int s = 0; s < ns; s++) {
+        const bool first = s == 0;
+        if constexpr (LEAN) {
+            const LeanRegs<D> L = load_lean<D>(Mdl.s[s]);
+#pragma unroll 2
+            for (int e = 0; e < E; e++) {
+                const bool inA = e < iA;
+                const bool tgt = e >= tA;
+                const bool useA = tgt ? (e == tA) : inA;
+                int c = inA ? e : e - iA;
+                c = c < NB ? c : NB - 1;
+                double dh[D];
+#pragma unroll
+                for (int d = 0; d < D; d++)
+                    dh[d] = (useA ? xA[d] : xB[d]) - (tgt ? tg[d] : xy[c * D + d]);
+                const double v = lean_term<D>(L, dh);
+                double *slot = stage + e * kWave + lane;
+                *slot = first ? v : *slot + v;
+            }
+        } else {
+            const DevStructure S = Mdl.s[s];
+            for (int e = 0; e < E; e++) {
+                const bool inA = e < iA;
+                const bool tgt = e >= tA;
+                const bool useA = tgt ? (e == tA) : inA;
+                int c = inA ? e : e - iA;
+                c = c < NB ? c : NB - 1;
+                double dh[D];
+#pragma unroll
+                for (int d = 0; d < D; d++)
+                    dh[d] = (useA ? xA[d] : xB[d]) - (tgt ? tg[d] : xy[c * D + d]);
+                const double v = S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dh));
+                double *slot = stage + e * kWave + lane;
+                *slot = first ? v : *slot + v;
+            }
+        }
+    }
+}
+
 // Core: builds C (rows owned by this lane), factors it and forward-solves
 // the Q right-hand sides held in y.  `rowx` are the coordinates of the
 // owned rows, `xy` the group's neighbour coordinates in LDS, `tg` the
-// target; y[0] is filled here with the covariance to the target.  Returns
-// false when a pivot falls below 1e-12 * c0 (the reference's singular
-// threshold relative to the matrix scale, linalg.rs:36-57, 135-140).
+// target; y[0] is filled here with the covariance to the target.  Entries
+// are c0 - gamma(dh) with gamma = nugget + sum sill g (model.rs:636-646);
+// gamma(0) = 0, so a target on a data point gets c0.  Returns false when a
+// pivot falls below 1e-12 * c0 (the reference's singular threshold relative
+// to the matrix scale, linalg.rs:36-57, 135-140).
 template <int D, int NB, int G, int Q, bool LEAN>
 __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const int cnt,
                                                    const double (&rowx)[NB / G][D],
                                                    const double *xy, const double *tg,
                                                    double *stage, double *col, const int lane,
-                                                   const int g, double (&y)[Q][NB / G]) {
+                                                   const int g_in, double (&y)[Q][NB / G]) {
     constexpr int R = NB / G;
+    const int g = opaque(g_in);
     const double c0 = Mdl.c0;
+    const double nug = Mdl.nugget;
     double a[R][NB];
-    static_for<0, R>([&](auto RR) {
-        constexpr int r = decltype(RR)::value;
-        constexpr int ncol = G * (r + 1);
-        const int i = G * r + g;
-#pragma unroll 2
-        for (int c = 0; c <= ncol; c++) {
-            double v;
-            double dh[D];
-            const bool tgt = (c == ncol);
-            const bool off = i < cnt && (tgt || c < i);
-            if (off) {
+    static_for<0, (R + 1) / 2>([&](auto PP) {
+        constexpr int rA = decltype(PP)::value;
+        constexpr int rB = R - 1 - rA;
+        constexpr bool single = rA == rB;
+        const int iA = row_of<G>(rA, g);
+        const int iB = single ? 0 : row_of<G>(rB, g);
+        const int E = single ? G * (rA + 1) : NB + 1;
+        accum_entries<D, NB, LEAN>(Mdl, E, iA, iB, rowx[rA], rowx[rB], xy, tg, stage, lane);
+        // Entries right of the diagonal receive unused values: the
+        // factorization never reads them (its multipliers for rows <= j are
+        // selected away) and the diagonal is tracked in dg below.
+        const bool liveA = iA < cnt;
 #pragma unroll
-                for (int d = 0; d < D; d++)
-                    dh[d] = rowx[r][d] - (tgt ? tg[d] : xy[(c < NB ? c : NB - 1) * D + d]);
-                v = cov_offdiag<D, LEAN>(Mdl, dh);
-            } else {
-                v = (c == i) ? c0 : 0.0;
-            }
-            stage[c * kWave + lane] = v;
+        for (int c = 0; c < G * (rA + 1); c++) {
+            const double sv = stage[c * kWave + lane];
+            a[rA][c] = liveA ? c0 - (nug + sv) : 0.0;
         }
+        bool zA = true;
 #pragma unroll
-        for (int c = 0; c < ncol; c++) a[r][c] = stage[c * kWave + lane];
-        y[0][r] = stage[ncol * kWave + lane];
+        for (int d = 0; d < D; d++) zA = zA && rowx[rA][d] == tg[d];
+        const double tAv = stage[(iA + iB) * kWave + lane];
+        y[0][rA] = liveA ? (zA ? c0 : c0 - (nug + tAv)) : 0.0;
+        if constexpr (!single) {
+            const bool liveB = iB < cnt;
+#pragma unroll
+            for (int c = 0; c < G * (rB + 1); c++) {
+                const int slot = min(c + iA, NB);
+                const double sv = stage[slot * kWave + lane];
+                a[rB][c] = liveB ? c0 - (nug + sv) : 0.0;
+            }
+            bool zB = true;
+#pragma unroll
+            for (int d = 0; d < D; d++) zB = zB && rowx[rB][d] == tg[d];
+            const double tBv = stage[(iA + iB + 1) * kWave + lane];
+            y[0][rB] = liveB ? (zB ? c0 : c0 - (nug + tBv)) : 0.0;
+        }
     });
 
+    // dg[r]: the diagonal entry of the owned row of block r, updated as
+    // the entry itself would be (d -= l^2 per column).
+    int rid[R];
+    double dg[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        rid[r] = row_of<G>(r, g);
+        dg[r] = c0;
+    }
     const double thresh = 1e-12 * c0;
     bool ok = true;
     static_for<0, NB>([&](auto JJ) {
         constexpr int j = decltype(JJ)::value;
-        constexpr int oj = j % G;
+        constexpr int oj = owner_of<G>(j);
         constexpr int rj = j / G;
-        const double d = bcast<G, oj>(a[rj][j]);
+        const double d = bcast<G, oj>(dg[rj]);
         ok = ok && (d > thresh);
         const double inv = rsqrt_nr(d);
         double l[R];
 #pragma unroll
         for (int r = rj; r < R; r++) l[r] = a[r][j] * inv;
-        l[rj] = (g > oj) ? l[rj] : 0.0;
+        l[rj] = (rid[rj] > j) ? l[rj] : 0.0;
+#pragma unroll
+        for (int r = rj; r < R; r++) dg[r] -= l[r] * l[r];
         // Publish column j (rows > j) and the scaled right-hand sides of
         // row j through the group's LDS buffer; every lane then reads the
         // column back with wide broadcast loads.
 #pragma unroll
-        for (int r = rj; r < R; r++) col[G * r + g] = l[r];
+        for (int r = rj; r < R; r++) col[rid[r]] = l[r];
 #pragma unroll
         for (int qq = 0; qq < Q; qq++) col[NB + qq * G + g] = y[qq][rj] * inv;
         wave_sync();
@@ -221,7 +376,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
             for (int r = rj; r < R; r++) y[qq][r] -= l[r] * yj;
             y[qq][rj] = (g == oj) ? yj : y[qq][rj];
         }
-        static_for<j + 1, NB>([&](auto KK) {
+        static_for<j + 1, NB - 1>([&](auto KK) {
             constexpr int k = decltype(KK)::value;
             const double lk = col[k];
 #pragma unroll
@@ -370,21 +525,21 @@ __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
         double zr[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = G * r + g;
+            const int i = row_of<G>(r, g);
             zr[r] = i < cnt ? P.values[nbl[i]] : 0.0;
         }
         wave_sync();
         double rowx[R][D];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = G * r + g;
+            const int i = row_of<G>(r, g);
 #pragma unroll
             for (int d = 0; d < D; d++) rowx[r][d] = i < cnt ? xy[i * D + d] : 0.0;
         }
         double y[Q][R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = G * r + g;
+            const int i = row_of<G>(r, g);
             const bool live = i < cnt;
             y[1][r] = live ? (M == 0 ? zr[r] - P.sk_mean : zr[r]) : 0.0;
             if constexpr (M > 0) {
@@ -474,14 +629,14 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
         double zr[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = G * r + g;
+            const int i = row_of<G>(r, g);
             zr[r] = i < cnt ? P.zc[nbl[i]] : 0.0;
         }
         wave_sync();
         double rowx[R][D];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = G * r + g;
+            const int i = row_of<G>(r, g);
 #pragma unroll
             for (int d = 0; d < D; d++) rowx[r][d] = i < cnt ? xy[i * D + d] : 0.0;
         }
