@@ -8,6 +8,14 @@ targets).  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                   [--precision fp64|fp32r] [--no-cpu-baseline]
+
+Other workloads (one JSON line each, not the headline):
+  --workload ok2d_small | uk3d_octant   the other kriging configurations
+  --workload vecchia_c3                 Vecchia log-likelihood, n = 10^6 points,
+                                        m = 30, 64 Matern-5/2 parameter vectors per
+                                        batch (plan built once, timed separately)
+  --workload serving                    request mix (build + predict per request),
+                                        p50 / p99 latency under Poisson arrivals
 """
 from __future__ import annotations
 
@@ -77,13 +85,108 @@ def cpu_baseline(w, targets, budget_s: float = 10.0):
                       f"({build_s:.2f} s)"}
 
 
+def vecchia_flops_per_point(s: int, c: int) -> float:
+    """E*c + s^3/3 + 2 s^2 + 4 s + 6 with E = s(s+1)/2 (s conditioning points)."""
+    return s * (s + 1) / 2 * c + s ** 3 / 3.0 + 2 * s * s + 4 * s + 6
+
+
+def bench_vecchia(args) -> dict:
+    import numpy as np
+    import geostat_hip as gh
+    from geostat_hip import workloads
+    m = 30
+    coords = workloads.vecchia_c3(args.points or 1_000_000)
+    n = coords.shape[0]
+    t0 = time.perf_counter()
+    plan = gh.vecchia_plan(coords, m)
+    plan_s = time.perf_counter() - t0
+    order, nb, cnt = plan.arrays()
+    vals = workloads.vecchia_field(coords, order, nb, cnt, m, threads=cpu_threads())
+    models = workloads.vecchia_models(64)
+    for _ in range(args.warmup):
+        plan.loglik_batch(vals, models)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ll, st = plan.loglik_batch(vals, models)
+    dt = (time.perf_counter() - t0) / args.steps
+    hist = np.bincount(cnt, minlength=m + 1)
+    flops = 64 * float(sum(hist[s] * vecchia_flops_per_point(s, 19) for s in range(m + 1) if hist[s]))
+    return {
+        "metric": "Vecchia conditionals/s (points x parameter vectors)",
+        "value": n * 64 / dt, "unit": "conditionals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded, sequential Vecchia draw over the plan)",
+        "config": {"workload": "vecchia_c3", "points": int(n), "m": m, "models": 64,
+                   "plan_build_s": plan_s, "ok_models": int((st == 0).sum())},
+        "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": flops / dt / 1e12 / FP64_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "vecchia_kernel", "flops_per_launch": flops},
+        "cpu_baseline": None,
+    }
+
+
+def bench_serving(args) -> dict:
+    """Each request builds its handle (host kd-tree + cell grid, upload)
+    and predicts its targets (host in, host out).  Requests are served FIFO
+    by one worker; the measured service times are replayed against Poisson
+    arrivals at a load of 0.5 (arrival rate = 0.5 / mean service time)."""
+    import numpy as np
+    import geostat_hip as gh
+    from geostat_hip import workloads
+    reqs = workloads.serving_requests(args.requests)
+    ctx = gh.Context(0)
+    # warm the kernels of every k and kind once
+    for k in (16, 32, 64):
+        for kind in workloads.SERVING_KINDS:
+            r = workloads.ServingRequest(0.0, 2000, 10_000, k, kind, 7)
+            c, v, mdl, cfg, t = workloads.serving_case(r)
+            gh.Kriging(c, v, mdl, cfg, ctx=ctx).predict_many(t)
+    service, sizes = [], []
+    for r in reqs:
+        c, v, mdl, cfg, t = workloads.serving_case(r)
+        t0 = time.perf_counter()
+        kr = gh.Kriging(c, v, mdl, cfg, ctx=ctx)
+        est = kr.predict_many(t)
+        service.append(time.perf_counter() - t0)
+        sizes.append((r.n, r.targets, r.k))
+        del kr, est
+    service = np.array(service)
+    mean_gap = service.mean() / 0.5
+    arrival = np.cumsum([r.arrival_gap * mean_gap for r in reqs])
+    finish = np.empty(len(reqs))
+    free = 0.0
+    for i in range(len(reqs)):
+        start = max(arrival[i], free)
+        finish[i] = start + service[i]
+        free = finish[i]
+    lat = (finish - arrival) * 1e3
+    total_targets = sum(s[1] for s in sizes)
+    return {
+        "metric": "serving request latency p50/p99 (build + predict per request)",
+        "value": float(np.percentile(lat, 50)), "unit": "ms", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 12, "ms_per_step": float(service.mean() * 1e3),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic request mix (seeded)",
+        "config": {"workload": "serving", "requests": len(reqs), "load": 0.5,
+                   "p50_ms": float(np.percentile(lat, 50)),
+                   "p99_ms": float(np.percentile(lat, 99)),
+                   "service_p50_ms": float(np.percentile(service, 50) * 1e3),
+                   "service_p99_ms": float(np.percentile(service, 99) * 1e3),
+                   "targets_per_s_in_service": float(total_targets / service.sum())},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ok2d_large",
-                    choices=["ok2d_large", "ok2d_small", "uk3d_octant"])
+                    choices=["ok2d_large", "ok2d_small", "uk3d_octant", "vecchia_c3", "serving"])
+    ap.add_argument("--points", type=int, default=0, help="vecchia_c3: number of points")
+    ap.add_argument("--requests", type=int, default=40, help="serving: number of requests")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32r"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -104,6 +207,11 @@ def main() -> int:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(targets shard trivially; see DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.workload in ("vecchia_c3", "serving"):
+        fn = bench_vecchia if args.workload == "vecchia_c3" else bench_serving
+        print(json.dumps(fn(args)), flush=True)
+        return 0
 
     import numpy as np
     import torch

@@ -126,3 +126,53 @@ def vecchia_field(coords, order, nb, nb_count, m, nugget=0.01, sill=1.0, rng=0.0
                                         _abi.i32ptr(np.ascontiguousarray(nb_count, dtype=np.int32)),
                                         m, nugget, sill, rng, threads, _abi.dptr(vals))
     return vals
+
+
+def vecchia_c3(n: int = 1_000_000, seed: int = SEED_BASE + 3):
+    """Coordinates of the Vecchia likelihood workload: n points uniform on
+    [0, 1]^2 (values are drawn over the plan by vecchia_field)."""
+    c, _ = points(seed, n, [1.0, 1.0], recipe=2)
+    return c
+
+
+@dataclass
+class ServingRequest:
+    arrival_gap: float      # in units of the mean gap
+    n: int
+    targets: int
+    k: int
+    kind: int
+    seed: int
+
+
+SERVING_KINDS = (ModelKind.SPHERICAL, ModelKind.EXPONENTIAL, ModelKind.MATERN15,
+                 ModelKind.MATERN25)
+
+
+def serving_requests(count: int = 40, seed: int = SEED_BASE + 4, n_max: int = 200_000,
+                     t_max: int = 1_000_000):
+    """A request mix: exponential inter-arrival gaps (Poisson arrivals),
+    n log-uniform in [10^3, n_max], targets log-uniform in [10^4, t_max],
+    k in {16, 32, 64}, model kind in {Sph, Exp, Mat15, Mat25}."""
+    u = uniforms(seed, 5 * count).reshape(count, 5)
+    out = []
+    for i, (a, b, c, d, e) in enumerate(u):
+        out.append(ServingRequest(
+            arrival_gap=-np.log(max(1.0 - a, 1e-300)),
+            n=int(round(10.0 ** (3.0 + b * (np.log10(n_max) - 3.0)))),
+            targets=int(round(10.0 ** (4.0 + c * (np.log10(t_max) - 4.0)))),
+            k=(16, 32, 64)[min(int(d * 3), 2)],
+            kind=SERVING_KINDS[min(int(e * 4), 3)],
+            seed=seed + 1 + i))
+    return out
+
+
+def serving_case(req: ServingRequest, ext: float = 10_000.0):
+    """(coords, values, model, config, targets) of one serving request:
+    data by the reference bench recipe on [0, ext]^2, targets uniform in
+    the same square, OK with max_neighbors = k."""
+    c, v = points(req.seed, req.n, [ext, ext], recipe=0)
+    t = uniforms(req.seed + 100_000, 2 * req.targets).reshape(req.targets, 2) * ext
+    model = VariogramModel(0.05, [Structure(req.kind, 1.0, 800.0)])
+    cfg = KrigingConfig(max_neighbors=req.k)
+    return c, v, model, cfg, t
