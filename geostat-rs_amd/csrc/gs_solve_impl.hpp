@@ -199,11 +199,12 @@ __device__ __forceinline__ LeanRegs<D> load_lean(const DevStructure &S) {
     return L;
 }
 
-// sill * g(|T dh| / range) for a lean structure.
-template <int D>
+// sill * g(|T dh| / range) for a lean structure (ANISO: T is the
+// structure's rotation/scaling, else the identity).
+template <int D, bool ANISO, bool SPH>
 __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *dh) {
     double h2s = 0.0;
-    if (L.aniso) {
+    if constexpr (ANISO) {
 #pragma unroll
         for (int k = 0; k < D; k++) {
             double e = 0.0;
@@ -218,7 +219,7 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
     }
     const double d = sqrt_fast(h2s);
     double g;
-    if (L.sph) {
+    if constexpr (SPH) {
         g = d >= 1.0 ? 1.0 : 1.5 * d - 0.5 * d * d * d;
     } else {
         const double q = L.alpha * d;
@@ -228,54 +229,79 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
     return L.sill * g;
 }
 
-// Accumulates sum_s sill_s g_s over the entries of one pair of row blocks
-// into the lane's staging column: slot e < iA is (row iA, column e), slot
-// iA <= e < iA+iB is (row iB, column e-iA), slot iA+iB is (row iA,
-// target), slot iA+iB+1 is (row iB, target).  Slots past those hold
-// unused values.  The structure loop is outside the entry loop so each
-// structure's parameters are read once.
+// Data-data entries of one pair of row blocks into the lane's LDS staging
+// column: slot e < iA is (row iA, column e), slot iA <= e < NB-1 is (row
+// NB-1-iA, column e-iA) (with one block, slots past iA hold unused
+// values).  FIRST writes, otherwise the slot accumulates.
+template <int D, bool ANISO, bool SPH, bool FIRST>
+__device__ __forceinline__ void lean_slots(const LeanRegs<D> &L, const int E, const int iA,
+                                           const double (&xA)[D], const double (&xB)[D],
+                                           const double *xy, double *stage, const int lane) {
+    for (int e = 0; e < E; e++) {
+        const bool inA = e < iA;
+        const int c = inA ? e : e - iA;
+        double dh[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
+        const double v = lean_term<D, ANISO, SPH>(L, dh);
+        double *slot = stage + e * kWave + lane;
+        *slot = FIRST ? v : *slot + v;
+    }
+}
+
+// Staged evaluation of one pair of row blocks (rows iA and NB-1-iA of the
+// lane; one block when R = 1): the data-data entries go to the staging
+// column, the two target entries accumulate in tA / tB.  The structure
+// loop is outside the entry loop, so each structure's parameters are read
+// once, and the kind / anisotropy choice is made once per structure.
 template <int D, int NB, bool LEAN>
-__device__ __forceinline__ void accum_entries(const DevModel &Mdl, const int E, const int iA,
-                                              const int iB, const double (&xA)[D],
-                                              const double (&xB)[D], const double *xy,
-                                              const double *tg, double *stage, const int lane) {
-    const int tA = iA + iB;
+__device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int E, const int iA,
+                                               const double (&xA)[D], const double (&xB)[D],
+                                               const double *xy, const double *tg,
+                                               double *stage, const int lane, double &tA,
+                                               double &tB) {
+    tA = 0.0;
+    tB = 0.0;
+    double dA[D], dB[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        dA[d] = xA[d] - tg[d];
+        dB[d] = xB[d] - tg[d];
+    }
     const int ns = Mdl.ns;
     for (int s = 0; s < ns; s++) {
-        const bool first = s == 0;
         if constexpr (LEAN) {
             const LeanRegs<D> L = load_lean<D>(Mdl.s[s]);
-#pragma unroll 2
-            for (int e = 0; e < E; e++) {
-                const bool inA = e < iA;
-                const bool tgt = e >= tA;
-                const bool useA = tgt ? (e == tA) : inA;
-                int c = inA ? e : e - iA;
-                c = c < NB ? c : NB - 1;
-                double dh[D];
-#pragma unroll
-                for (int d = 0; d < D; d++)
-                    dh[d] = (useA ? xA[d] : xB[d]) - (tgt ? tg[d] : xy[c * D + d]);
-                const double v = lean_term<D>(L, dh);
-                double *slot = stage + e * kWave + lane;
-                *slot = first ? v : *slot + v;
+#define GS_LEAN_CASE(AN, SP)                                                                    \
+    do {                                                                                        \
+        if (s == 0) lean_slots<D, AN, SP, true>(L, E, iA, xA, xB, xy, stage, lane);             \
+        else lean_slots<D, AN, SP, false>(L, E, iA, xA, xB, xy, stage, lane);                   \
+        tA += lean_term<D, AN, SP>(L, dA);                                                      \
+        tB += lean_term<D, AN, SP>(L, dB);                                                      \
+    } while (0)
+            if (L.aniso) {
+                if (L.sph) GS_LEAN_CASE(true, true);
+                else GS_LEAN_CASE(true, false);
+            } else {
+                if (L.sph) GS_LEAN_CASE(false, true);
+                else GS_LEAN_CASE(false, false);
             }
+#undef GS_LEAN_CASE
         } else {
             const DevStructure S = Mdl.s[s];
+            const bool first = s == 0;
             for (int e = 0; e < E; e++) {
                 const bool inA = e < iA;
-                const bool tgt = e >= tA;
-                const bool useA = tgt ? (e == tA) : inA;
-                int c = inA ? e : e - iA;
-                c = c < NB ? c : NB - 1;
+                const int c = inA ? e : e - iA;
                 double dh[D];
 #pragma unroll
-                for (int d = 0; d < D; d++)
-                    dh[d] = (useA ? xA[d] : xB[d]) - (tgt ? tg[d] : xy[c * D + d]);
+                for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
                 const double v = S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dh));
                 double *slot = stage + e * kWave + lane;
                 *slot = first ? v : *slot + v;
             }
+            tA += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dA));
+            tB += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dB));
         }
     }
 }
@@ -304,9 +330,9 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         constexpr int rB = R - 1 - rA;
         constexpr bool single = rA == rB;
         const int iA = row_of<G>(rA, g);
-        const int iB = single ? 0 : row_of<G>(rB, g);
-        const int E = single ? G * (rA + 1) : NB + 1;
-        accum_entries<D, NB, LEAN>(Mdl, E, iA, iB, rowx[rA], rowx[rB], xy, tg, stage, lane);
+        double tA, tB;
+        staged_entries<D, NB, LEAN>(Mdl, NB - 1, iA, rowx[rA], rowx[rB], xy, tg, stage, lane, tA,
+                                    tB);
         // Entries right of the diagonal receive unused values: the
         // factorization never reads them (its multipliers for rows <= j are
         // selected away) and the diagonal is tracked in dg below.
@@ -319,32 +345,35 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         bool zA = true;
 #pragma unroll
         for (int d = 0; d < D; d++) zA = zA && rowx[rA][d] == tg[d];
-        const double tAv = stage[(iA + iB) * kWave + lane];
-        y[0][rA] = liveA ? (zA ? c0 : c0 - (nug + tAv)) : 0.0;
+        y[0][rA] = liveA ? (zA ? c0 : c0 - (nug + tA)) : 0.0;
         if constexpr (!single) {
+            const int iB = NB - 1 - iA;
             const bool liveB = iB < cnt;
 #pragma unroll
             for (int c = 0; c < G * (rB + 1); c++) {
-                const int slot = min(c + iA, NB);
+                const int slot = min(c + iA, NB - 2);
                 const double sv = stage[slot * kWave + lane];
                 a[rB][c] = liveB ? c0 - (nug + sv) : 0.0;
             }
             bool zB = true;
 #pragma unroll
             for (int d = 0; d < D; d++) zB = zB && rowx[rB][d] == tg[d];
-            const double tBv = stage[(iA + iB + 1) * kWave + lane];
-            y[0][rB] = liveB ? (zB ? c0 : c0 - (nug + tBv)) : 0.0;
+            y[0][rB] = liveB ? (zB ? c0 : c0 - (nug + tB)) : 0.0;
         }
     });
 
     // dg[r]: the diagonal entry of the owned row of block r, updated as
     // the entry itself would be (d -= l^2 per column).
+    // invd[r]: 1/L_ii of the owned row of block r.  The right-hand sides of
+    // a row are scaled once at the end instead of being overwritten by the
+    // owner at its step.
     int rid[R];
-    double dg[R];
+    double dg[R], invd[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         rid[r] = row_of<G>(r, g);
         dg[r] = c0;
+        invd[r] = 0.0;
     }
     const double thresh = 1e-12 * c0;
     bool ok = true;
@@ -355,6 +384,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         const double d = bcast<G, oj>(dg[rj]);
         ok = ok && (d > thresh);
         const double inv = rsqrt_nr(d);
+        invd[rj] = (g == oj) ? inv : invd[rj];
         double l[R];
 #pragma unroll
         for (int r = rj; r < R; r++) l[r] = a[r][j] * inv;
@@ -374,7 +404,6 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
             const double yj = col[NB + qq * G + oj];
 #pragma unroll
             for (int r = rj; r < R; r++) y[qq][r] -= l[r] * yj;
-            y[qq][rj] = (g == oj) ? yj : y[qq][rj];
         }
         static_for<j + 1, NB - 1>([&](auto KK) {
             constexpr int k = decltype(KK)::value;
@@ -384,6 +413,10 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         });
         wave_sync();
     });
+#pragma unroll
+    for (int qq = 0; qq < Q; qq++)
+#pragma unroll
+        for (int r = 0; r < R; r++) y[qq][r] *= invd[r];
     return ok;
 }
 
@@ -488,7 +521,7 @@ __global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
     constexpr int Q = 2 + M;
     constexpr int GPW = kWave / G;
     __shared__ double s_xy[GPW * NB * D];
-    __shared__ double s_stage[(NB + 1) * kWave];
+    __shared__ double s_stage[(NB - 1) * kWave];
     __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, Q, G>::value];
     const int lane = threadIdx.x;
     const int g = lane % G;
@@ -597,7 +630,7 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
     constexpr int R = NB / G;
     constexpr int GPW = kWave / G;
     __shared__ double s_xy[GPW * NB * D];
-    __shared__ double s_stage[(NB + 1) * kWave];
+    __shared__ double s_stage[(NB - 1) * kWave];
     __shared__ double s_sum[GPW * kMaxModelsPerLaunch];
     __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, 2, G>::value];
     const int lane = threadIdx.x;
