@@ -229,39 +229,39 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
     return L.sill * g;
 }
 
-// Data-data entries of one pair of row blocks into the lane's LDS staging
-// column: slot e < iA is (row iA, column e), slot iA <= e < NB-1 is (row
-// NB-1-iA, column e-iA) (with one block, slots past iA hold unused
-// values).  FIRST writes, otherwise the slot accumulates.
-template <int D, bool ANISO, bool SPH, bool FIRST>
-__device__ __forceinline__ void lean_slots(const LeanRegs<D> &L, const int E, const int iA,
-                                           const double (&xA)[D], const double (&xB)[D],
-                                           const double *xy, double *stage, const int lane) {
-    for (int e = 0; e < E; e++) {
+// Data-data entries e0 <= e < e1 of one pair of row blocks into the lane's
+// LDS staging column (slot e - e0).  Entry e < iA is (row iA, column e);
+// entry e >= iA is (row NB-1-iA, column NB-2-e), so every register of the
+// pair has a compile-time entry number (with one block, entries past iA
+// are unused).  FIRST writes, otherwise the slot accumulates.
+template <int D, int NB, bool ANISO, bool SPH, bool FIRST>
+__device__ __forceinline__ void lean_slots(const LeanRegs<D> &L, const int e0, const int e1,
+                                           const int iA, const double (&xA)[D],
+                                           const double (&xB)[D], const double *xy,
+                                           double *stage, const int lane) {
+    for (int e = e0; e < e1; e++) {
         const bool inA = e < iA;
-        const int c = inA ? e : e - iA;
+        const int c = inA ? e : NB - 2 - e;
         double dh[D];
 #pragma unroll
         for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
         const double v = lean_term<D, ANISO, SPH>(L, dh);
-        double *slot = stage + e * kWave + lane;
+        double *slot = stage + (e - e0) * kWave + lane;
         *slot = FIRST ? v : *slot + v;
     }
 }
 
-// Staged evaluation of one pair of row blocks (rows iA and NB-1-iA of the
-// lane; one block when R = 1): the data-data entries go to the staging
-// column, the two target entries accumulate in tA / tB.  The structure
-// loop is outside the entry loop, so each structure's parameters are read
-// once, and the kind / anisotropy choice is made once per structure.
-template <int D, int NB, bool LEAN>
-__device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int E, const int iA,
-                                               const double (&xA)[D], const double (&xB)[D],
-                                               const double *xy, const double *tg,
-                                               double *stage, const int lane, double &tA,
-                                               double &tB) {
-    tA = 0.0;
-    tB = 0.0;
+// Staged evaluation of entries e0 <= e < e1 of one pair of row blocks (rows
+// iA and NB-1-iA of the lane; one block when R = 1); with TGT the two
+// target entries are accumulated into tA / tB as well.  The structure loop
+// is outside the entry loop, so each structure's parameters are read once,
+// and the kind / anisotropy choice is made once per structure.
+template <int D, int NB, bool LEAN, bool TGT>
+__device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int e0, const int e1,
+                                               const int iA, const double (&xA)[D],
+                                               const double (&xB)[D], const double *xy,
+                                               const double *tg, double *stage, const int lane,
+                                               double &tA, double &tB) {
     double dA[D], dB[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -274,10 +274,12 @@ __device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int E,
             const LeanRegs<D> L = load_lean<D>(Mdl.s[s]);
 #define GS_LEAN_CASE(AN, SP)                                                                    \
     do {                                                                                        \
-        if (s == 0) lean_slots<D, AN, SP, true>(L, E, iA, xA, xB, xy, stage, lane);             \
-        else lean_slots<D, AN, SP, false>(L, E, iA, xA, xB, xy, stage, lane);                   \
-        tA += lean_term<D, AN, SP>(L, dA);                                                      \
-        tB += lean_term<D, AN, SP>(L, dB);                                                      \
+        if (s == 0) lean_slots<D, NB, AN, SP, true>(L, e0, e1, iA, xA, xB, xy, stage, lane);    \
+        else lean_slots<D, NB, AN, SP, false>(L, e0, e1, iA, xA, xB, xy, stage, lane);          \
+        if constexpr (TGT) {                                                                    \
+            tA += lean_term<D, AN, SP>(L, dA);                                                  \
+            tB += lean_term<D, AN, SP>(L, dB);                                                  \
+        }                                                                                       \
     } while (0)
             if (L.aniso) {
                 if (L.sph) GS_LEAN_CASE(true, true);
@@ -290,21 +292,28 @@ __device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int E,
         } else {
             const DevStructure S = Mdl.s[s];
             const bool first = s == 0;
-            for (int e = 0; e < E; e++) {
+            for (int e = e0; e < e1; e++) {
                 const bool inA = e < iA;
-                const int c = inA ? e : e - iA;
+                const int c = inA ? e : NB - 2 - e;
                 double dh[D];
 #pragma unroll
                 for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
                 const double v = S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dh));
-                double *slot = stage + e * kWave + lane;
+                double *slot = stage + (e - e0) * kWave + lane;
                 *slot = first ? v : *slot + v;
             }
-            tA += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dA));
-            tB += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dB));
+            if constexpr (TGT) {
+                tA += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dA));
+                tB += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dB));
+            }
         }
     }
 }
+
+// Staging slots per lane: the NB-1 entries of a block pair go through the
+// staging column in two passes.
+template <int NB>
+constexpr int stage_slots() { return NB / 2; }
 
 // Core: builds C (rows owned by this lane), factors it and forward-solves
 // the Q right-hand sides held in y.  `rowx` are the coordinates of the
@@ -325,36 +334,50 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
     const double c0 = Mdl.c0;
     const double nug = Mdl.nugget;
     double a[R][NB];
+    constexpr int NE = NB - 1;              // entries per lane and block pair
+    constexpr int HS = stage_slots<NB>();   // slots per staging pass
     static_for<0, (R + 1) / 2>([&](auto PP) {
         constexpr int rA = decltype(PP)::value;
         constexpr int rB = R - 1 - rA;
         constexpr bool single = rA == rB;
         const int iA = row_of<G>(rA, g);
-        double tA, tB;
-        staged_entries<D, NB, LEAN>(Mdl, NB - 1, iA, rowx[rA], rowx[rB], xy, tg, stage, lane, tA,
-                                    tB);
+        const int iB = NB - 1 - iA;
+        const bool liveA = iA < cnt;
+        const bool liveB = iB < cnt;
+        double tA = 0.0, tB = 0.0;
         // Entries right of the diagonal receive unused values: the
         // factorization never reads them (its multipliers for rows <= j are
         // selected away) and the diagonal is tracked in dg below.
-        const bool liveA = iA < cnt;
-#pragma unroll
-        for (int c = 0; c < G * (rA + 1); c++) {
-            const double sv = stage[c * kWave + lane];
-            a[rA][c] = liveA ? c0 - (nug + sv) : 0.0;
-        }
+        static_for<0, 2>([&](auto PS) {
+            constexpr int e0 = decltype(PS)::value * HS;
+            constexpr int e1 = e0 + HS < NE ? e0 + HS : NE;
+            staged_entries<D, NB, LEAN, decltype(PS)::value == 0>(Mdl, e0, e1, iA, rowx[rA],
+                                                                  rowx[rB], xy, tg, stage, lane,
+                                                                  tA, tB);
+            static_for<0, G * (rA + 1)>([&](auto CC) {
+                constexpr int c = decltype(CC)::value;
+                if constexpr (c >= e0 && c < e1) {
+                    const double sv = stage[(c - e0) * kWave + lane];
+                    a[rA][c] = liveA ? c0 - (nug + sv) : 0.0;
+                }
+            });
+            if constexpr (!single) {
+                static_for<0, G * (rB + 1)>([&](auto CC) {
+                    constexpr int c = decltype(CC)::value;
+                    constexpr int e = NB - 2 - c;
+                    if constexpr (e >= e0 && e < e1) {
+                        const double sv = stage[(e - e0) * kWave + lane];
+                        a[rB][c] = liveB ? c0 - (nug + sv) : 0.0;
+                    }
+                });
+            }
+        });
+        if constexpr (G * (rB + 1) == NB) a[rB][NB - 1] = 0.0;   // no entry maps here
         bool zA = true;
 #pragma unroll
         for (int d = 0; d < D; d++) zA = zA && rowx[rA][d] == tg[d];
         y[0][rA] = liveA ? (zA ? c0 : c0 - (nug + tA)) : 0.0;
         if constexpr (!single) {
-            const int iB = NB - 1 - iA;
-            const bool liveB = iB < cnt;
-#pragma unroll
-            for (int c = 0; c < G * (rB + 1); c++) {
-                const int slot = min(c + iA, NB - 2);
-                const double sv = stage[slot * kWave + lane];
-                a[rB][c] = liveB ? c0 - (nug + sv) : 0.0;
-            }
             bool zB = true;
 #pragma unroll
             for (int d = 0; d < D; d++) zB = zB && rowx[rB][d] == tg[d];
@@ -515,13 +538,20 @@ __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&
     return value;
 }
 
+// Occupancy target handed to the register allocator: three wavefronts per
+// SIMD (168 VGPRs per lane) for 32-point systems on 16-lane groups with at
+// most one drift term, where that costs only a few spills outside the
+// factorization; the other shapes keep the allocator's choice.
+template <int NB, int G, int M>
+constexpr int krige_waves() { return (NB == 32 && G == 16 && M <= 1) ? 3 : 1; }
+
 template <int D, int NB, int G, int M, bool LEAN>
-__global__ __launch_bounds__(kWave) void krige_fp64_kernel(SolveParams P) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_waves<NB, G, M>()))) void krige_fp64_kernel(SolveParams P) {
     constexpr int R = NB / G;
     constexpr int Q = 2 + M;
     constexpr int GPW = kWave / G;
     __shared__ double s_xy[GPW * NB * D];
-    __shared__ double s_stage[(NB - 1) * kWave];
+    __shared__ double s_stage[stage_slots<NB>() * kWave];
     __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, Q, G>::value];
     const int lane = threadIdx.x;
     const int g = lane % G;
@@ -630,7 +660,7 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
     constexpr int R = NB / G;
     constexpr int GPW = kWave / G;
     __shared__ double s_xy[GPW * NB * D];
-    __shared__ double s_stage[(NB - 1) * kWave];
+    __shared__ double s_stage[stage_slots<NB>() * kWave];
     __shared__ double s_sum[GPW * kMaxModelsPerLaunch];
     __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, 2, G>::value];
     const int lane = threadIdx.x;
