@@ -872,11 +872,22 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
         std::string key = "krige:" + std::to_string(device) + ":" + std::to_string(K->D) + ":" +
                           std::to_string(NB) + ":" + std::to_string(K->M) + ":" +
                           model_signature(K->model) + (pp.list ? "list" : "");
+        // the tuner times a bounded prefix of the systems (their outputs
+        // are rewritten by the real launch below)
+        constexpr int32_t kTuneSystems = 131072;
         gs::SolveParams tp = pp;
-        if (!pp.list) tp.T = std::min<int64_t>(pp.T, 131072);
         int variant = 0;
         int32_t st = tuned_variant(key, s, [&](int v) {
-            return gs::launch_krige_fp64(K->D, NB, K->M, v, K->lean, tp, pp.list ? groups : tp.T, s);
+            if (pp.list) {
+                int32_t *tune_len = K->lens.as<int32_t>() + 4;
+                hipError_t e = gs::launch_min_i32(pp.list_len, kTuneSystems, tune_len, s);
+                if (e != hipSuccess) return e;
+                tp.list_len = tune_len;
+            } else {
+                tp.T = std::min<int64_t>(pp.T, kTuneSystems);
+            }
+            return gs::launch_krige_fp64(K->D, NB, K->M, v, K->lean, tp,
+                                         std::min<int64_t>(groups, kTuneSystems), s);
         }, &variant);
         if (st != GS_OK) return st;
         hipError_t e = gs::launch_krige_fp64(K->D, NB, K->M, variant, K->lean, pp, groups, s);
@@ -888,7 +899,7 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
     if (K->per_oct > 0 && K->nb_bound > 16) {
         // Uneven sizes: bucket by padded system size, one launch per bucket.
         HIP_TRY(K->lists.ensure(sizeof(int32_t) * 3 * T));
-        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 4));
+        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 8));
         HIP_TRY(gs::launch_zero_i32(K->lens.as<int32_t>(), 4, s));
         HIP_TRY(gs::launch_bucket(K->count.as<int32_t>(), T, K->lists.as<int32_t>(),
                                   K->lens.as<int32_t>(), T, s));
@@ -908,7 +919,7 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
     if (K->has_tree) {
         // process targets in the search's Hilbert order: neighbouring systems
         // share data points, so the coordinate gathers hit cache
-        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 4));
+        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 8));
         HIP_TRY(gs::launch_fill_i32(K->lens.as<int32_t>() + 3, 1, (int32_t)T, s));
         p.list = K->perm.as<int32_t>();
         p.list_len = K->lens.as<int32_t>() + 3;

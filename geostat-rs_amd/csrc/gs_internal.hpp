@@ -128,6 +128,8 @@ hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_
 hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
                            hipStream_t s);
 hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
+// *dst = min(*src, cap) on the device (stream-ordered)
+hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStream_t s);
 hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
                          int64_t list_cap, hipStream_t s);
 hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaParams &p,

@@ -70,6 +70,15 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t s) 
 
 hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) { return launch_fill_i32(p, n, 0, s); }
 
+__global__ void min_i32_kernel(const int32_t *src, int32_t cap, int32_t *dst) {
+    if (threadIdx.x == 0) dst[0] = min(src[0], cap);
+}
+
+hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStream_t s) {
+    hipLaunchKernelGGL(min_i32_kernel, dim3(1), dim3(64), 0, s, src, cap, dst);
+    return hipGetLastError();
+}
+
 template <int D, int M, int V, bool LEAN>
 static hipError_t krige_nb(int NB, const SolveParams &p, int64_t gg, hipStream_t s) {
     switch (NB) {

@@ -433,6 +433,9 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
             const double lk = col[k];
 #pragma unroll
             for (int r = k / G; r < R; r++) a[r][k] -= l[r] * lk;
+            // at most 8 column values in flight: keeps the scheduler from
+            // loading the whole column ahead (register pressure)
+            if constexpr (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
         });
         wave_sync();
     });
