@@ -189,6 +189,9 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=40, help="serving: number of requests")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32r"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-io", action="store_true",
+                    help="also time predict_many on host arrays (PCIe-inclusive rate, reported "
+                         "as config.host_io_targets_per_s; never the headline value)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times "
                          "(scales rocprof FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -292,6 +295,12 @@ def main() -> int:
                      "flops_per_launch": flops_launch},
         "cpu_baseline": None,
     }
+    if args.host_io:
+        k.predict_many(targets)                      # warm the host path
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            k.predict_many(targets)
+        out["config"]["host_io_targets_per_s"] = T * args.steps / (time.perf_counter() - t0)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, targets)
     print(json.dumps(out), flush=True)
