@@ -1194,9 +1194,14 @@ void grid_knn(const BGrid &G, const double *target, int64_t k, int64_t limit, KH
 }
 
 // maxmin_order (vecchia.rs:144-215): exact farthest-point traversal, ties
-// to the larger index.  Lazy max-heap of stale upper bounds; a popped key
-// is refreshed against the selected set (brute force while it has <= 48
-// points, else a 1-NN query on a bucket grid of the selected points).
+// to the larger index.  The reference refreshes stale heap keys lazily;
+// here keys are kept exact instead: when p is selected with key l2 (the
+// largest current min-distance), only points closer to p than sqrt(l2) can
+// lower their key, so one range query on a grid of all points and a
+// decrease-key per improved point keep every key equal to the min squared
+// distance to the selected set.  The order is the same (argmax of
+// (total_key(min d2), index) at every step); the work is ~n log n instead
+// of the lazy refreshes' 1-NN queries on a sparse grid.
 void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &order) {
     order.clear();
     if (n == 0) return;
@@ -1209,6 +1214,9 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
         double dd = dist2(coords + i * D, c, D);
         if (total_key(dd) < total_key(bd)) { bd = dd; first = i; }
     }
+    order.reserve((size_t)n);
+    order.push_back(first);
+    if (n == 1) return;
     double mn[3], mx[3];
     for (int d = 0; d < D; d++) { mn[d] = coords[d]; mx[d] = coords[d]; }
     for (int64_t i = 0; i < n; i++)
@@ -1216,94 +1224,87 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
             mn[d] = std::fmin(mn[d], coords[i * D + d]);
             mx[d] = std::fmax(mx[d], coords[i * D + d]);
         }
-    // Growable grid of selected points (sized as the reference: ~1 per bucket at n).
-    BGrid G;
+    BGrid G;                               // ~1 point per bucket
     G.init(D, mn, mx, n);
-    const int64_t nb = G.nbuckets();
-    std::vector<std::vector<uint32_t>> buckets((size_t)nb);
-    std::vector<double> sel;
-    sel.reserve((size_t)(n * D));
-    auto add_sel = [&](int64_t i) {
-        int64_t b = G.bucket_of(coords + i * D);
-        buckets[(size_t)b].push_back((uint32_t)(sel.size() / D));
-        for (int d = 0; d < D; d++) sel.push_back(coords[i * D + d]);
-        order.push_back(i);
-    };
-    add_sel(first);
-    struct Key { double d2; int64_t idx; };
-    auto kgt = [](const Key &a, const Key &b) {   // MaxminKey order (vecchia.rs:114-125)
-        int64_t ka = total_key(a.d2), kb = total_key(b.d2);
-        return ka > kb || (ka == kb && a.idx > b.idx);
-    };
-    auto cmp = [&](const Key &a, const Key &b) { return kgt(b, a); };   // max-heap
-    std::vector<Key> heap;
+    G.build_all(coords, n);
+    std::vector<double> key((size_t)n);    // min squared distance to the selected set
+    std::vector<char> taken((size_t)n, 0);
+    taken[(size_t)first] = 1;
+    for (int64_t i = 0; i < n; i++) key[(size_t)i] = dist2(coords + i * D, coords + first * D, D);
+    // indexed binary max-heap over the remaining points, ordered by
+    // (total_key(key), index) as MaxminKey (vecchia.rs:114-125)
+    std::vector<uint32_t> heap;
+    std::vector<int64_t> where((size_t)n, -1);
     heap.reserve((size_t)n);
-    for (int64_t i = 0; i < n; i++)
-        if (i != first) heap.push_back({dist2(coords + i * D, coords + first * D, D), i});
-    std::make_heap(heap.begin(), heap.end(), cmp);
-    int64_t max_ring = G.nn[0];
-    for (int d = 1; d < D; d++) max_ring = std::max(max_ring, G.nn[d]);
-    while (!heap.empty()) {
-        std::pop_heap(heap.begin(), heap.end(), cmp);
-        Key top = heap.back();
-        heap.pop_back();
-        const double *pi = coords + top.idx * D;
-        const int64_t nsel = (int64_t)(sel.size() / D);
-        double current = INFINITY;
-        if (nsel <= 48) {
-            for (int64_t s = 0; s < nsel; s++) current = std::fmin(current, dist2(pi, &sel[(size_t)(s * D)], D));
-        } else {
-            // 1-NN over the selected set: the minimum distance is what matters
-            // and it does not depend on which of several tied points is kept.
-            int64_t tc[3];
-            G.cell_of(pi, tc);
-            double best = INFINITY;
-            for (int64_t ring = 0; ring <= max_ring; ring++) {
-                double rm = std::fmax((double)ring - 1.0, 0.0) * G.cell;
-                if (rm * rm > best) break;
-                auto visit = [&](const int64_t *cur) {
-                    for (int d = 0; d < D; d++) if (cur[d] < 0 || cur[d] >= G.nn[d]) return;
-                    int64_t b = 0;
-                    for (int d = 0; d < D; d++) b += cur[d] * G.strides[d];
-                    for (uint32_t s : buckets[(size_t)b]) {
-                        double d2 = dist2(pi, &sel[(size_t)s * D], D);
-                        if (d2 < best) best = d2;
-                    }
-                };
-                if (ring == 0) { visit(tc); continue; }
-                int64_t r = ring;
-                for (int fd = 0; fd < D; fd++)
-                    for (int sd = 0; sd < 2; sd++) {
-                        int64_t side = sd == 0 ? -r : r;
-                        int64_t lo[3], hi[3];
-                        bool empty = false;
-                        for (int d = 0; d < D; d++) {
-                            if (d < fd) { lo[d] = tc[d] - r + 1; hi[d] = tc[d] + r - 1; }
-                            else if (d == fd) { lo[d] = tc[d] + side; hi[d] = tc[d] + side; }
-                            else { lo[d] = tc[d] - r; hi[d] = tc[d] + r; }
-                            if (lo[d] > hi[d]) empty = true;
-                        }
-                        if (empty) continue;
-                        int64_t cur[3] = {lo[0], lo[1], lo[2]};
-                        for (;;) {
-                            visit(cur);
-                            int d = 0;
-                            for (; d < D; d++) {
-                                if (++cur[d] <= hi[d]) break;
-                                cur[d] = lo[d];
-                            }
-                            if (d == D) break;
-                        }
-                    }
-            }
-            current = best;
+    auto above = [&](uint32_t a, uint32_t b) {
+        const int64_t ka = total_key(key[a]), kb = total_key(key[b]);
+        return ka > kb || (ka == kb && a > b);
+    };
+    auto place = [&](size_t e, uint32_t id) {
+        heap[e] = id;
+        where[id] = (int64_t)e;
+    };
+    auto sift_down = [&](size_t e) {
+        const uint32_t id = heap[e];
+        const size_t sz = heap.size();
+        for (;;) {
+            size_t l = 2 * e + 1;
+            if (l >= sz) break;
+            size_t b = l;
+            if (l + 1 < sz && above(heap[l + 1], heap[l])) b = l + 1;
+            if (!above(heap[b], id)) break;
+            place(e, heap[b]);
+            e = b;
         }
-        Key refreshed{current, top.idx};
-        if (heap.empty() || !kgt(heap.front(), refreshed)) {
-            add_sel(top.idx);
-        } else {
-            heap.push_back(refreshed);
-            std::push_heap(heap.begin(), heap.end(), cmp);
+        place(e, id);
+    };
+    for (int64_t i = 0; i < n; i++)
+        if (i != first) { where[(size_t)i] = (int64_t)heap.size(); heap.push_back((uint32_t)i); }
+    for (size_t e = heap.size() / 2; e-- > 0;) sift_down(e);
+    while (!heap.empty()) {
+        const uint32_t p = heap[0];
+        const uint32_t last = heap.back();
+        heap.pop_back();
+        where[p] = -1;
+        if (!heap.empty()) {
+            place(0, last);
+            sift_down(0);
+        }
+        taken[p] = 1;
+        order.push_back((int64_t)p);
+        if (heap.empty()) break;
+        // points whose key can drop lie within sqrt(key[p]) of p (every
+        // remaining key is <= key[p]); the box is widened for rounding
+        const double *pp = coords + (int64_t)p * D;
+        const double rad = std::sqrt(key[p]) * (1.0 + 1e-9) + 1e-300;
+        int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int d = 0; d < D; d++) {
+            lo[d] = std::min(std::max<int64_t>(BGrid::sat((pp[d] - rad - G.origin[d]) / G.cell), 0),
+                             G.nn[d] - 1);
+            hi[d] = std::min(std::max<int64_t>(BGrid::sat((pp[d] + rad - G.origin[d]) / G.cell), 0),
+                             G.nn[d] - 1);
+        }
+        int64_t cur[3] = {lo[0], lo[1], lo[2]};
+        for (;;) {
+            // one x-run of buckets is one contiguous CSR range
+            int64_t b0 = 0;
+            for (int d = 1; d < D; d++) b0 += cur[d] * G.strides[d];
+            const int64_t q0 = G.start[(size_t)(b0 + lo[0])], q1 = G.start[(size_t)(b0 + hi[0]) + 1];
+            for (int64_t q = q0; q < q1; q++) {
+                const uint32_t i = G.items[(size_t)q];
+                if (taken[i]) continue;
+                const double d2 = dist2(coords + (int64_t)i * D, pp, D);
+                if (total_key(d2) < total_key(key[i])) {
+                    key[i] = d2;
+                    sift_down((size_t)where[i]);
+                }
+            }
+            int d = 1;
+            for (; d < D; d++) {
+                if (++cur[d] <= hi[d]) break;
+                cur[d] = lo[d];
+            }
+            if (d >= D) break;
         }
     }
 }

@@ -182,6 +182,12 @@ struct TileHeap {
         }
         set(e, v, i);
     }
+    // Floyd's bottom-up construction over slots [0, len).
+    __device__ __forceinline__ void heapify(int len) {
+        double td;
+        uint32_t ti;
+        for (int e = len / 2 - 1; e >= 0; e--) sift_down(len, e, D2(e), IX(e), td, ti);
+    }
     __device__ __forceinline__ void sort(int len) {
         double td;
         uint32_t ti;
@@ -384,8 +390,16 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
                     if (!(d2 <= r2)) continue;
                     if (!OCT) {
                         if (len < k) {
-                            H.push(len, d2, id);
-                            if (len == k) { top_d = H.D2(0); top_i = H.IX(0); }
+                            // fill unordered; one bottom-up heapify when full
+                            // (candidates arrive roughly by increasing
+                            // distance, so sift-up pushes would climb to the
+                            // root every time)
+                            H.set(len, d2, id);
+                            if (++len == k) {
+                                H.heapify(k);
+                                top_d = H.D2(0);
+                                top_i = H.IX(0);
+                            }
                         } else if (TileHeap::gt(top_d, top_i, d2, id)) {
                             rej = fmin(rej, top_d);
                             H.sift_down(len, 0, d2, id, top_d, top_i);
@@ -461,6 +475,7 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
     int32_t *out = P.out_idx + t * P.stride;
     if (!OCT) {
         const bool tie = len == k && rej == top_d;
+        if (len < k) H.heapify(len);     // a partial fill is still unordered
         H.sort(len);
         for (int e = 0; e < len; e++) out[e] = (int32_t)H.IX(e);
         for (int e = len; e < P.stride; e++) out[e] = -1;
