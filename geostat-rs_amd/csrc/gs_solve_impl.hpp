@@ -199,8 +199,19 @@ __device__ __forceinline__ LeanRegs<D> load_lean(const DevStructure &S) {
     return L;
 }
 
-// sill * g(|T dh| / range) for a lean structure (ANISO: T is the
-// structure's rotation/scaling, else the identity).
+// sqrt(q) for q >= 0: v_rsq on max(q, 1e-300) (so q = 0 gives 0, not
+// 0 * inf) and one Markstein correction (about 1 ulp for normal q).
+__device__ __forceinline__ double sqrt_lean(double q) {
+    const double y = __builtin_amdgcn_rsq(fmax(q, 1e-300));
+    const double h = q * y;
+    return fma(fma(-h, h, q), 0.5 * y, h);
+}
+
+// sill * (1 - g(|T dh| / range)) for a lean structure (ANISO: T is the
+// structure's rotation/scaling, else the identity).  Summed over the
+// structures this is c0 - gamma(dh) for dh != 0, because c0 = nugget + sum
+// sill (model.rs:636-660); the exponential family gives sill p(q) e^-q'
+// directly, without the cancellation of 1 - g at long range.
 template <int D, bool ANISO, bool SPH>
 __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *dh) {
     double h2s = 0.0;
@@ -217,16 +228,16 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
         for (int d = 0; d < D; d++) h2s = fma(dh[d], dh[d], h2s);
         h2s *= L.ir2;
     }
-    const double d = sqrt_fast(h2s);
-    double g;
+    const double d = sqrt_lean(h2s);
+    double c;
     if constexpr (SPH) {
-        g = d >= 1.0 ? 1.0 : 1.5 * d - 0.5 * d * d * d;
+        c = d >= 1.0 ? 0.0 : 1.0 - (1.5 * d - 0.5 * d * d * d);
     } else {
         const double q = L.alpha * d;
         const double p = fma(fma(L.gam, q, L.beta), q, 1.0);
-        g = 1.0 - p * exp_neg(L.gau ? h2s : q);
+        c = p * exp_neg(L.gau ? h2s : q);
     }
-    return L.sill * g;
+    return L.sill * c;
 }
 
 // Data-data entries e0 <= e < e1 of one pair of row blocks into the lane's
@@ -310,6 +321,14 @@ __device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int e0
     }
 }
 
+// Covariance entry from the staged structure sum: the lean class sums
+// sill (1 - g) (the entry itself), the general class sums sill g.
+template <bool LEAN>
+__device__ __forceinline__ double entry(double c0, double nug, double s) {
+    if constexpr (LEAN) return s;
+    else return c0 - (nug + s);
+}
+
 // Staging slots per lane: the NB-1 entries of a block pair go through the
 // staging column in two passes.
 template <int NB>
@@ -358,7 +377,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
                 constexpr int c = decltype(CC)::value;
                 if constexpr (c >= e0 && c < e1) {
                     const double sv = stage[(c - e0) * kWave + lane];
-                    a[rA][c] = liveA ? c0 - (nug + sv) : 0.0;
+                    a[rA][c] = liveA ? entry<LEAN>(c0, nug, sv) : 0.0;
                 }
             });
             if constexpr (!single) {
@@ -367,7 +386,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
                     constexpr int e = NB - 2 - c;
                     if constexpr (e >= e0 && e < e1) {
                         const double sv = stage[(e - e0) * kWave + lane];
-                        a[rB][c] = liveB ? c0 - (nug + sv) : 0.0;
+                        a[rB][c] = liveB ? entry<LEAN>(c0, nug, sv) : 0.0;
                     }
                 });
             }
@@ -376,12 +395,12 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         bool zA = true;
 #pragma unroll
         for (int d = 0; d < D; d++) zA = zA && rowx[rA][d] == tg[d];
-        y[0][rA] = liveA ? (zA ? c0 : c0 - (nug + tA)) : 0.0;
+        y[0][rA] = liveA ? (zA ? c0 : entry<LEAN>(c0, nug, tA)) : 0.0;
         if constexpr (!single) {
             bool zB = true;
 #pragma unroll
             for (int d = 0; d < D; d++) zB = zB && rowx[rB][d] == tg[d];
-            y[0][rB] = liveB ? (zB ? c0 : c0 - (nug + tB)) : 0.0;
+            y[0][rB] = liveB ? (zB ? c0 : entry<LEAN>(c0, nug, tB)) : 0.0;
         }
     });
 
