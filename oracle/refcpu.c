@@ -1295,6 +1295,173 @@ int32_t rc_vecchia_loglik_plan(int32_t D, const double *coords, const double *va
     return RC_OK;
 }
 
+/* g_and_dg (vecchia.rs:353-385): g(d) and dg/dd for the kinds with a
+ * closed-form range derivative; returns 0 for the others. */
+static int g_and_dg(int32_t kind, double d, double *g, double *dg) {
+    switch (kind) {
+    case RC_SPHERICAL:
+        if (d >= 1.0) { *g = 1.0; *dg = 0.0; }
+        else { *g = 1.5 * d - 0.5 * d * d * d; *dg = 1.5 - 1.5 * d * d; }
+        return 1;
+    case RC_EXPONENTIAL: { double e = exp(-d); *g = 1.0 - e; *dg = e; return 1; }
+    case RC_GAUSSIAN: { double e = exp(-(d * d)); *g = 1.0 - e; *dg = 2.0 * d * e; return 1; }
+    case RC_MATERN15: {
+        double q = sqrt(3.0) * d, e = exp(-q);
+        *g = 1.0 - (1.0 + q) * e;
+        *dg = 3.0 * d * e;
+        return 1;
+    }
+    case RC_MATERN25: {
+        double q = sqrt(5.0) * d, e = exp(-q);
+        *g = 1.0 - (1.0 + q + q * q / 3.0) * e;
+        *dg = 5.0 * d * (1.0 + q) / 3.0 * e;
+        return 1;
+    }
+    default: return 0;
+    }
+}
+
+static double vdist2(const double *a, const double *b, int D) {   /* vecchia.rs:95-102 */
+    double s = 0.0;
+    for (int d = 0; d < D; d++) { double v = a[d] - b[d]; s += v * v; }
+    return s;
+}
+
+/* cholesky_factor_in_place (linalg.rs:123-160) and the two triangular
+ * solves (linalg.rs:162-203), split so one factor serves several
+ * right-hand sides. */
+static int32_t chol_factor(double *a, int32_t n) {
+    double scale = 0.0;
+    for (int i = 0; i < n; i++) scale = fmax(scale, a[i * n + i]);
+    scale = fmax(scale, 2.2250738585072014e-308);
+    for (int k = 0; k < n; k++) {
+        double d = a[k * n + k];
+        for (int j = 0; j < k; j++) d -= a[k * n + j] * a[k * n + j];
+        if (!(d > 1e-12 * scale)) return RC_SINGULAR_SYSTEM;
+        d = sqrt(d);
+        a[k * n + k] = d;
+        for (int i = k + 1; i < n; i++) {
+            double s = a[i * n + k];
+            for (int j = 0; j < k; j++) s -= a[i * n + j] * a[k * n + j];
+            a[i * n + k] = s / d;
+        }
+    }
+    return RC_OK;
+}
+static void chol_solve(const double *a, int32_t n, double *b) {
+    for (int i = 0; i < n; i++) {
+        double s = b[i];
+        for (int j = 0; j < i; j++) s -= a[i * n + j] * b[j];
+        b[i] = s / a[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++) s -= a[j * n + i] * b[j];
+        b[i] = s / a[i * n + i];
+    }
+}
+
+/* loglik_grad_inner (vecchia.rs:428-566): log-likelihood and its gradient
+ * in (nugget, psill, range) for one isotropic structure of a kind with a
+ * closed-form range derivative.  Dense per-point matrices K_SS, dK/dpsill,
+ * dK/drange; one factorization reused for w = K^-1 k_i and the three
+ * derivative solves; the variance term's derivative is zeroed where the
+ * variance is clamped at 1e-12. */
+int32_t rc_vecchia_loglik_grad_plan(int32_t D, const double *coords, const double *values,
+                                    int64_t n, int32_t kind, double nugget, double psill,
+                                    double range, const int64_t *order, const int32_t *nb,
+                                    const int32_t *nb_count, int64_t m, double *ll_out,
+                                    double *grad_out) {
+    double g0, dg0;
+    if (!g_and_dg(kind, 0.5, &g0, &dg0)) return RC_INVALID_PARAMETER;
+    double mean = -0.0;
+    for (int64_t i = 0; i < n; i++) mean += values[i];
+    mean = mean / (double)n;
+    const double sill = nugget + psill;
+    const size_t mm = (size_t)(m * m);
+    double *kss = (double *)malloc(sizeof(double) * mm);
+    double *dkp = (double *)malloc(sizeof(double) * mm);
+    double *dkr = (double *)malloc(sizeof(double) * mm);
+    double *vec = (double *)malloc(sizeof(double) * (size_t)(8 * m));
+    double *ki = vec, *dkip = vec + m, *dkir = vec + 2 * m, *w = vec + 3 * m;
+    double *rhs = vec + 4 * m, *dw = vec + 5 * m, *zn = vec + 6 * m, *zeros = vec + 7 * m;
+    for (int64_t a = 0; a < m; a++) zeros[a] = 0.0;
+    const double ln2pi = log(2.0 * M_PI);
+    double ll = 0.0, grad[3] = {0.0, 0.0, 0.0};
+    int32_t st = RC_OK;
+    for (int64_t k = 0; k < n && st == RC_OK; k++) {
+        const int64_t i = order[k];
+        const int64_t s = nb_count[k];
+        const double zi = values[i] - mean;
+        if (s == 0) {
+            const double var = sill, r = zi;
+            ll += -0.5 * (ln2pi + log(var) + r * r / var);
+            const double common = -0.5 / var + 0.5 * r * r / (var * var);
+            grad[0] += common;
+            grad[1] += common;
+            continue;
+        }
+        const int32_t *row = nb + k * m;
+        for (int64_t a = 0; a < s; a++) {
+            zn[a] = values[row[a]] - mean;
+            double g, dg;
+            const double h_ia = sqrt(vdist2(coords + i * D, coords + (int64_t)row[a] * D, D));
+            g_and_dg(kind, h_ia / range, &g, &dg);
+            ki[a] = sill - (nugget + psill * g);
+            dkip[a] = 1.0 - g;
+            dkir[a] = psill * dg * h_ia / (range * range);
+            kss[a * s + a] = sill;
+            dkp[a * s + a] = 1.0;
+            dkr[a * s + a] = 0.0;
+            for (int64_t b = a + 1; b < s; b++) {
+                const double h = sqrt(vdist2(coords + (int64_t)row[a] * D, coords + (int64_t)row[b] * D, D));
+                g_and_dg(kind, h / range, &g, &dg);
+                const double c = sill - (nugget + psill * g);
+                kss[a * s + b] = kss[b * s + a] = c;
+                dkp[a * s + b] = dkp[b * s + a] = 1.0 - g;
+                const double dr = psill * dg * h / (range * range);
+                dkr[a * s + b] = dkr[b * s + a] = dr;
+            }
+        }
+        st = chol_factor(kss, (int32_t)s);
+        if (st != RC_OK) break;
+        memcpy(w, ki, sizeof(double) * (size_t)s);
+        chol_solve(kss, (int32_t)s, w);
+        double mu = 0.0, red = 0.0;
+        for (int64_t a = 0; a < s; a++) mu += w[a] * zn[a];
+        for (int64_t a = 0; a < s; a++) red += w[a] * ki[a];
+        const double raw = sill - red;
+        const double var = fmax(raw, 1e-12);
+        const int clamped = raw <= 1e-12;
+        const double r = zi - mu;
+        ll += -0.5 * (ln2pi + log(var) + r * r / var);
+        for (int p = 0; p < 3; p++) {
+            const double dsill = p < 2 ? 1.0 : 0.0;
+            const double *dk = p == 0 ? NULL : (p == 1 ? dkp : dkr);
+            const double *dki = p == 0 ? zeros : (p == 1 ? dkip : dkir);
+            for (int64_t a = 0; a < s; a++) {
+                if (!dk) { rhs[a] = -w[a]; continue; }
+                double y = 0.0;                                 /* matvec (vecchia.rs:388-395) */
+                for (int64_t b = 0; b < s; b++) y += dk[a * s + b] * w[b];
+                rhs[a] = dki[a] - y;
+            }
+            memcpy(dw, rhs, sizeof(double) * (size_t)s);
+            chol_solve(kss, (int32_t)s, dw);
+            double dmu = 0.0, dr1 = 0.0, dr2 = 0.0;
+            for (int64_t a = 0; a < s; a++) dmu += dw[a] * zn[a];
+            for (int64_t a = 0; a < s; a++) dr1 += dw[a] * ki[a];
+            for (int64_t a = 0; a < s; a++) dr2 += w[a] * dki[a];
+            const double dvar = clamped ? 0.0 : dsill - (dr1 + dr2);
+            grad[p] += -0.5 * dvar / var + r * dmu / var + 0.5 * r * r / (var * var) * dvar;
+        }
+    }
+    free(kss); free(dkp); free(dkr); free(vec);
+    if (st != RC_OK) return st;
+    *ll_out = ll;
+    for (int p = 0; p < 3; p++) grad_out[p] = grad[p];
+    return RC_OK;
+}
+
 /* exact_loglik (vecchia.rs:1958-1974): dense MVN loglik through LU. */
 int32_t rc_dense_mvn_loglik(int32_t D, const double *coords, const double *values, int64_t n,
                             const rc_model *model, double *out) {

@@ -117,6 +117,11 @@ int32_t rc_vecchia_loglik_plan(int32_t D, const double *coords, const double *va
                                const rc_model *model, const int64_t *order,
                                const int32_t *nb, const int32_t *nb_count, int64_t m,
                                double *out);
+int32_t rc_vecchia_loglik_grad_plan(int32_t D, const double *coords, const double *values,
+                                    int64_t n, int32_t kind, double nugget, double psill,
+                                    double range, const int64_t *order, const int32_t *nb,
+                                    const int32_t *nb_count, int64_t m, double *ll_out,
+                                    double *grad_out);
 int32_t rc_dense_mvn_loglik(int32_t D, const double *coords, const double *values, int64_t n,
                             const rc_model *model, double *out);
 

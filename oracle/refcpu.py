@@ -98,6 +98,10 @@ def lib() -> C.CDLL:
             "rc_vecchia_loglik_plan": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64,
                                                    C.POINTER(rc_model), _I64P, _I32P, _I32P,
                                                    C.c_int64, _DP]),
+            "rc_vecchia_loglik_grad_plan": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64,
+                                                        C.c_int32, C.c_double, C.c_double,
+                                                        C.c_double, _I64P, _I32P, _I32P,
+                                                        C.c_int64, _DP, _DP]),
             "rc_dense_mvn_loglik": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64,
                                                 C.POINTER(rc_model), _DP]),
             "rc_duplicate_pair": (C.c_int32, [C.c_int32, _DP, C.c_int64, _I64P]),
@@ -294,6 +298,22 @@ def vecchia_loglik_plan(coords, values, m_, order, nb, cnt):
                                       _arr(cnt, np.int32).ctypes.data_as(_I32P), nb.shape[1],
                                       C.byref(out))
     return rc, out.value
+
+
+def vecchia_loglik_grad_plan(coords, values, kind, nugget, psill, rng, order, nb, cnt):
+    """loglik_grad_inner (vecchia.rs:428-566): (rc, loglik, [d/dnugget,
+    d/dpsill, d/drange]) for one isotropic structure of `kind`."""
+    coords = _arr(coords)
+    values = _arr(values)
+    nb = _arr(nb, np.int32)
+    ll = C.c_double()
+    grad = np.zeros(3)
+    rc = lib().rc_vecchia_loglik_grad_plan(
+        coords.shape[1], _d(coords), _d(values), coords.shape[0], int(kind), float(nugget),
+        float(psill), float(rng), _arr(order, np.int64).ctypes.data_as(_I64P),
+        nb.ctypes.data_as(_I32P), _arr(cnt, np.int32).ctypes.data_as(_I32P), nb.shape[1],
+        C.byref(ll), _d(grad))
+    return rc, ll.value, grad
 
 
 def dense_mvn_loglik(coords, values, m_):

@@ -469,3 +469,84 @@ def test_incremental_predecessors_match_brute_force(oracle):
         d2 = ((c[prev] - c[order[k]]) ** 2).sum(1)
         want = sorted(prev[np.lexsort((prev, d2))][:m])
         assert sorted(nb[k, :cnt[k]]) == want
+
+
+# ---- Vecchia log-likelihood gradient (vecchia.rs:353-566) ------------------
+GRAD_KINDS = (ModelKind.SPHERICAL, ModelKind.EXPONENTIAL, ModelKind.GAUSSIAN,
+              ModelKind.MATERN15, ModelKind.MATERN25)
+
+
+def _plain_loglik(oracle, c, v, kind, nug, ps, rg, o, nb, cnt):
+    m = VariogramModel(nug, [Structure(kind, ps, rg)])
+    rc, ll = oracle.vecchia_loglik_plan(c, v, m, o, nb, cnt)
+    assert rc == 0
+    return ll
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_vecchia_gradient_matches_finite_differences(oracle, dim):
+    # vecchia.rs:2878-2937: each of (nugget, psill, range) against a central
+    # difference of the gradient-free log-likelihood, here at a much tighter
+    # bound (1e-6 instead of 1e-2) so that a dropped term cannot hide
+    if dim == 2:
+        c, v = cs.vecchia_field(oracle, 60, 11)
+    else:
+        c, v = cs.vecchia_field(oracle, 60, 12)
+        z = np.array([oracle.Rng(13).uniform() * 30.0 for _ in range(60)])
+        c = np.column_stack([c, z + np.arange(60) * 0.37])
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 8, None)
+    assert rc == 0
+    for kind in GRAD_KINDS:
+        nug, ps, rg = 0.05, 0.9, 22.0
+        rc, ll, g = oracle.vecchia_loglik_grad_plan(c, v, kind[0], nug, ps, rg, o, nb, cnt)
+        assert rc == 0
+        assert abs(ll - _plain_loglik(oracle, c, v, kind, nug, ps, rg, o, nb, cnt)) < 1e-9
+        base = (nug, ps, rg)
+        for p in range(3):
+            h = base[p] * 1e-4
+            up, dn = list(base), list(base)
+            up[p] += h
+            dn[p] -= h
+            fd = (_plain_loglik(oracle, c, v, kind, *up, o, nb, cnt)
+                  - _plain_loglik(oracle, c, v, kind, *dn, o, nb, cnt)) / (2 * h)
+            assert abs(g[p] - fd) < 1e-6 * max(abs(fd), 1.0), (kind, p, g[p], fd)
+
+
+def test_vecchia_gradient_small_partial_sill(oracle):
+    c, v = cs.vecchia_field(oracle, 40, 21)
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 6, None)
+    nug, ps, rg = 0.3, 1e-3, 15.0
+    rc, ll, g = oracle.vecchia_loglik_grad_plan(c, v, ModelKind.MATERN25[0], nug, ps, rg, o, nb, cnt)
+    assert rc == 0
+    h = ps * 1e-3
+    fd = (_plain_loglik(oracle, c, v, ModelKind.MATERN25, nug, ps + h, rg, o, nb, cnt)
+          - _plain_loglik(oracle, c, v, ModelKind.MATERN25, nug, ps - h, rg, o, nb, cnt)) / (2 * h)
+    assert abs(g[1] - fd) < 1e-6 * max(abs(fd), 1.0)
+
+
+def test_vecchia_gradient_zeroes_clamped_variance(oracle):
+    # vecchia.rs:2939-2990: a point conditioned on a neighbour 1e-13 away
+    # with zero nugget has its kriging variance floored at 1e-12; its
+    # contribution to d/dpsill is exactly 0, leaving the marginal term of
+    # the first point, -0.5/sill + 0.5 z0^2/sill^2 with z0 = 0.5
+    c = np.array([[0.0, 0.0], [1e-13, 0.0]])
+    v = np.array([0.3, -0.7])
+    o = np.array([0, 1], dtype=np.int64)
+    nb = np.array([[-1], [0]], dtype=np.int32)
+    cnt = np.array([0, 1], dtype=np.int32)
+    nug, ps, rg = 0.0, 1.0, 10.0
+    rc, ll, g = oracle.vecchia_loglik_grad_plan(c, v, ModelKind.SPHERICAL[0], nug, ps, rg, o, nb, cnt)
+    assert rc == 0
+    assert abs(ll - _plain_loglik(oracle, c, v, ModelKind.SPHERICAL, nug, ps, rg, o, nb, cnt)) < 1e-9
+    sill = nug + ps
+    assert abs(g[1] - (-0.5 / sill + 0.5 * 0.25 / sill ** 2)) < 1e-9
+
+
+def test_vecchia_gradient_rejects_kinds_without_derivative(oracle):
+    # vecchia.rs:2992-2999
+    c, v = cs.vecchia_field(oracle, 20, 2)
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 5, None)
+    for kind in (ModelKind.CIRCULAR, ModelKind.Stable(1.2), ModelKind.HOLE, ModelKind.WAVE,
+                 ModelKind.Matern(0.8)):
+        rc, _, _ = oracle.vecchia_loglik_grad_plan(c, v, kind[0], 0.1, 0.9, 20.0, o, nb, cnt)
+        assert rc == 4     # GeostatError::InvalidParameter (the reference returns None)
