@@ -45,6 +45,9 @@ KRIGE = [(d, nb, m, g, lean) for d, ms in ((2, (0, 1, 3, 6)), (3, (0, 1, 4, 10))
          for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
 VECCHIA = [(d, nb, g, lean) for d in (2, 3) for nb in (16, 32, 64) for lean in (1, 0)
            for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
+# Vecchia gradient: lean single-structure models, conditioning sets <= 32,
+# two row blocks per lane
+VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]
 
 
 def _gen_sources() -> list[Path]:
@@ -64,6 +67,14 @@ def _gen_sources() -> list[Path]:
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
                f"namespace gs {{\ntemplate hipError_t launch_vecchia_inst<{d}, {nb}, {g}, "
                f"{'true' if lean else 'false'}>"
+               f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
+    for d, nb, g in VGRAD:
+        p = GEN / f"vgrad_d{d}_nb{nb}_g{g}.hip"
+        txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_vecchia_grad_inst<{d}, {nb}, {g}>"
                f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)

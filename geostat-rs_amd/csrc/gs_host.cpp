@@ -1499,6 +1499,104 @@ int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_mode
     return GS_OK;
 }
 
+int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs_model *models,
+                                     int32_t n_models, int32_t precision, double *loglik,
+                                     double *grad, int32_t *status) {
+    if (!p) return fail(GS_ERR_INVALID_PARAMETER, "plan is NULL");
+    if (n_models < 0) return fail(GS_ERR_INVALID_PARAMETER, "n_models < 0");
+    if (n_models == 0) return GS_OK;
+    if (!values || !models || !loglik || !grad) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    if (precision != GS_PREC_FP64)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined Vecchia");
+    if (p->max_count > 32)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "unsupported on GPU: gradient with conditioning sets above 32 points");
+    const int64_t n = p->n;
+    for (int64_t i = 0; i < n; i++)
+        if (!std::isfinite(values[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite value");
+    // models the reference's loglik_grad_with_plan answers (g_and_dg kinds,
+    // one isotropic structure); the others get status InvalidParameter
+    std::vector<int32_t> st((size_t)n_models, 0), elig;
+    for (int32_t i = 0; i < n_models; i++) {
+        int32_t v = validate_model(&models[i], p->D, false);
+        if (v != GS_OK) return v;
+        const gs_model &m = models[i];
+        const int32_t k = m.n_structures == 1 ? m.structures[0].kind : -1;
+        const bool ok = m.n_structures == 1 && !m.structures[0].has_anis &&
+                        (k == GS_SPHERICAL || k == GS_EXPONENTIAL || k == GS_GAUSSIAN ||
+                         k == GS_MATERN15 || k == GS_MATERN25);
+        if (ok) elig.push_back(i);
+        else st[(size_t)i] = GS_ERR_INVALID_PARAMETER;
+    }
+    std::vector<double> res((size_t)n_models * 4, std::numeric_limits<double>::quiet_NaN());
+    if (!elig.empty()) {
+        std::lock_guard<std::mutex> lk(p->mu);
+        HIP_TRY(hipSetDevice(p->ctx->device));
+        hipStream_t s = p->ctx->stream;
+        double mean = -0.0;        // z = v - mean (vecchia.rs:437-439)
+        for (int64_t i = 0; i < n; i++) mean += values[i];
+        mean = mean / (double)n;
+        std::vector<double> zc((size_t)n);
+        for (int64_t i = 0; i < n; i++) zc[(size_t)i] = values[i] - mean;
+        const int32_t ne = (int32_t)elig.size();
+        std::vector<DevModel> dm((size_t)ne);
+        for (int32_t i = 0; i < ne; i++)
+            dm[(size_t)i] = resolve_model(models[elig[(size_t)i]], p->D, p->ctx->nodes.as<gs::BesselNodes>());
+        const int NB = pad_nb(std::max<int32_t>(p->max_count, 1));
+        const int32_t ppb = 64;
+        const int64_t n_blocks = (n + ppb - 1) / ppb;
+        HIP_TRY(p->zc.ensure(sizeof(double) * n));
+        HIP_TRY(p->models.ensure(sizeof(DevModel) * ne));
+        HIP_TRY(p->partial.ensure(sizeof(double) * n_blocks * gs::kMaxModelsPerLaunch * 4));
+        HIP_TRY(p->out.ensure(sizeof(double) * ne * 4));
+        HIP_TRY(p->status.ensure(sizeof(int32_t) * ne));
+        HIP_TRY(hipMemcpyAsync(p->zc.p, zc.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(p->models.p, dm.data(), sizeof(DevModel) * ne, hipMemcpyHostToDevice, s));
+        HIP_TRY(gs::launch_zero_i32(p->status.as<int32_t>(), ne, s));
+        for (int32_t b0 = 0; b0 < ne; b0 += gs::kMaxModelsPerLaunch) {
+            const int32_t nm = std::min<int32_t>(gs::kMaxModelsPerLaunch, ne - b0);
+            gs::VecchiaParams vp;
+            std::memset(&vp, 0, sizeof(vp));
+            vp.coords = p->coords.as<double>();
+            vp.zc = p->zc.as<double>();
+            vp.order = p->d_order.as<int64_t>();
+            vp.nb = p->d_nb.as<int32_t>();
+            vp.nb_count = p->d_count.as<int32_t>();
+            vp.n = n;
+            vp.m = (int32_t)p->m;
+            vp.n_models = nm;
+            vp.models = p->models.as<DevModel>() + b0;
+            vp.partial = p->partial.as<double>();
+            vp.status = p->status.as<int32_t>() + b0;
+            vp.points_per_block = ppb;
+            hipError_t e = gs::launch_vecchia_grad(p->D, NB, vp, n_blocks, s);
+            if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
+            HIP_TRY(e);
+            HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm * 4,
+                                              p->out.as<double>() + (int64_t)b0 * 4, s));
+        }
+        std::vector<double> o4((size_t)ne * 4);
+        std::vector<int32_t> st_e((size_t)ne);
+        HIP_TRY(hipMemcpyAsync(o4.data(), p->out.p, sizeof(double) * ne * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(st_e.data(), p->status.p, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int32_t i = 0; i < ne; i++) {
+            const int32_t mi = elig[(size_t)i];
+            st[(size_t)mi] = st_e[(size_t)i];
+            if (st_e[(size_t)i] == 0)
+                for (int q = 0; q < 4; q++) res[(size_t)mi * 4 + q] = o4[(size_t)i * 4 + q];
+        }
+    }
+    for (int32_t i = 0; i < n_models; i++) {
+        loglik[i] = res[(size_t)i * 4];
+        for (int q = 0; q < 3; q++) grad[(size_t)i * 3 + q] = res[(size_t)i * 4 + 1 + q];
+        if (status) status[i] = st[(size_t)i];
+    }
+    return GS_OK;
+}
+
 int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
                           int64_t n, const gs_model *model, int64_t m, const int64_t *order,
                           double *out) {

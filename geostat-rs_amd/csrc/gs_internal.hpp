@@ -134,6 +134,10 @@ hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_
                          int64_t list_cap, hipStream_t s);
 hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaParams &p,
                           int64_t n_blocks, hipStream_t s);
+// Vecchia log-likelihood gradient (partial holds 4 columns per model:
+// loglik, d/dnugget, d/dpsill, d/drange); NB 16 or 32.
+hipError_t launch_vecchia_grad(int D, int NB, const VecchiaParams &p, int64_t n_blocks,
+                               hipStream_t s);
 hipError_t launch_vecchia_reduce(const double *partial, int64_t n_blocks, int32_t n_models,
                                  double *out, hipStream_t s);
 
@@ -143,6 +147,8 @@ template <int D, int NB, int M, int G, bool LEAN>
 hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
 template <int D, int NB, int G, bool LEAN>
 hipError_t launch_vecchia_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
+template <int D, int NB, int G>
+hipError_t launch_vecchia_grad_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
 
 // Models evaluated by one Vecchia launch (host splits larger batches).
 constexpr int kMaxModelsPerLaunch = 64;

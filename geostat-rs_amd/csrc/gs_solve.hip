@@ -158,6 +158,16 @@ hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaPa
     return hipErrorInvalidValue;
 }
 
+// Group sizes of the gradient kernel: two row blocks per lane.
+hipError_t launch_vecchia_grad(int D, int NB, const VecchiaParams &p, int64_t n_blocks,
+                               hipStream_t s) {
+    if (D == 2 && NB == 16) return launch_vecchia_grad_inst<2, 16, 8>(p, n_blocks, s);
+    if (D == 2 && NB == 32) return launch_vecchia_grad_inst<2, 32, 16>(p, n_blocks, s);
+    if (D == 3 && NB == 16) return launch_vecchia_grad_inst<3, 16, 8>(p, n_blocks, s);
+    if (D == 3 && NB == 32) return launch_vecchia_grad_inst<3, 32, 16>(p, n_blocks, s);
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_vecchia_reduce(const double *partial, int64_t n_blocks, int32_t n_models,
                                  double *out, hipStream_t s) {
     hipLaunchKernelGGL(vecchia_reduce_kernel, dim3((unsigned)n_models), dim3(kW), 0, s, partial,

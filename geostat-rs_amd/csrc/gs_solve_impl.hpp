@@ -334,6 +334,81 @@ __device__ __forceinline__ double entry(double c0, double nug, double s) {
 template <int NB>
 constexpr int stage_slots() { return NB / 2; }
 
+// Right-looking Cholesky of the row-distributed C held in a (lane g owns
+// row row_of(r, g) of block r, columns before the diagonal), fused with the
+// forward substitution of the Q right-hand sides in y.  On return y holds
+// L^-1 b, invd[r] = 1/L_ii of the owned row of block r, and a[r][c] for c
+// below the row holds L_ic * L_cc (the multipliers before scaling).
+// Returns false when a pivot falls below 1e-12 * c0 (the reference's
+// singular threshold relative to the matrix scale, linalg.rs:36-57,
+// 135-140).
+template <int NB, int G, int Q>
+__device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (&y)[Q][NB / G],
+                                               const double c0, double *col, const int g,
+                                               double (&invd)[NB / G]) {
+    constexpr int R = NB / G;
+    // dg[r]: the diagonal entry of the owned row of block r, updated as
+    // the entry itself would be (d -= l^2 per column).
+    // invd[r]: 1/L_ii of the owned row of block r.  The right-hand sides of
+    // a row are scaled once at the end instead of being overwritten by the
+    // owner at its step.
+    int rid[R];
+    double dg[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        rid[r] = row_of<G>(r, g);
+        dg[r] = c0;
+        invd[r] = 0.0;
+    }
+    const double thresh = 1e-12 * c0;
+    bool ok = true;
+    static_for<0, NB>([&](auto JJ) {
+        constexpr int j = decltype(JJ)::value;
+        constexpr int oj = owner_of<G>(j);
+        constexpr int rj = j / G;
+        const double d = bcast<G, oj>(dg[rj]);
+        ok = ok && (d > thresh);
+        const double inv = rsqrt_nr(d);
+        invd[rj] = (g == oj) ? inv : invd[rj];
+        double l[R];
+#pragma unroll
+        for (int r = rj; r < R; r++) l[r] = a[r][j] * inv;
+        l[rj] = (rid[rj] > j) ? l[rj] : 0.0;
+#pragma unroll
+        for (int r = rj; r < R; r++) dg[r] -= l[r] * l[r];
+        // Publish column j (rows > j) and the scaled right-hand sides of
+        // row j through the group's LDS buffer; every lane then reads the
+        // column back with wide broadcast loads.
+#pragma unroll
+        for (int r = rj; r < R; r++) col[rid[r]] = l[r];
+#pragma unroll
+        for (int qq = 0; qq < Q; qq++) col[NB + qq * G + g] = y[qq][rj] * inv;
+        wave_sync();
+#pragma unroll
+        for (int qq = 0; qq < Q; qq++) {
+            const double yj = col[NB + qq * G + oj];
+#pragma unroll
+            for (int r = rj; r < R; r++) y[qq][r] -= l[r] * yj;
+        }
+        static_for<j + 1, NB - 1>([&](auto KK) {
+            constexpr int k = decltype(KK)::value;
+            const double lk = col[k];
+#pragma unroll
+            for (int r = k / G; r < R; r++) a[r][k] -= l[r] * lk;
+            // at most 8 column values in flight: keeps the scheduler from
+            // loading the whole column ahead (register pressure)
+            if constexpr (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+        });
+        wave_sync();
+    });
+#pragma unroll
+    for (int qq = 0; qq < Q; qq++)
+#pragma unroll
+        for (int r = 0; r < R; r++) y[qq][r] *= invd[r];
+    return ok;
+}
+
+
 // Core: builds C (rows owned by this lane), factors it and forward-solves
 // the Q right-hand sides held in y.  `rowx` are the coordinates of the
 // owned rows, `xy` the group's neighbour coordinates in LDS, `tg` the
@@ -404,65 +479,8 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         }
     });
 
-    // dg[r]: the diagonal entry of the owned row of block r, updated as
-    // the entry itself would be (d -= l^2 per column).
-    // invd[r]: 1/L_ii of the owned row of block r.  The right-hand sides of
-    // a row are scaled once at the end instead of being overwritten by the
-    // owner at its step.
-    int rid[R];
-    double dg[R], invd[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        rid[r] = row_of<G>(r, g);
-        dg[r] = c0;
-        invd[r] = 0.0;
-    }
-    const double thresh = 1e-12 * c0;
-    bool ok = true;
-    static_for<0, NB>([&](auto JJ) {
-        constexpr int j = decltype(JJ)::value;
-        constexpr int oj = owner_of<G>(j);
-        constexpr int rj = j / G;
-        const double d = bcast<G, oj>(dg[rj]);
-        ok = ok && (d > thresh);
-        const double inv = rsqrt_nr(d);
-        invd[rj] = (g == oj) ? inv : invd[rj];
-        double l[R];
-#pragma unroll
-        for (int r = rj; r < R; r++) l[r] = a[r][j] * inv;
-        l[rj] = (rid[rj] > j) ? l[rj] : 0.0;
-#pragma unroll
-        for (int r = rj; r < R; r++) dg[r] -= l[r] * l[r];
-        // Publish column j (rows > j) and the scaled right-hand sides of
-        // row j through the group's LDS buffer; every lane then reads the
-        // column back with wide broadcast loads.
-#pragma unroll
-        for (int r = rj; r < R; r++) col[rid[r]] = l[r];
-#pragma unroll
-        for (int qq = 0; qq < Q; qq++) col[NB + qq * G + g] = y[qq][rj] * inv;
-        wave_sync();
-#pragma unroll
-        for (int qq = 0; qq < Q; qq++) {
-            const double yj = col[NB + qq * G + oj];
-#pragma unroll
-            for (int r = rj; r < R; r++) y[qq][r] -= l[r] * yj;
-        }
-        static_for<j + 1, NB - 1>([&](auto KK) {
-            constexpr int k = decltype(KK)::value;
-            const double lk = col[k];
-#pragma unroll
-            for (int r = k / G; r < R; r++) a[r][k] -= l[r] * lk;
-            // at most 8 column values in flight: keeps the scheduler from
-            // loading the whole column ahead (register pressure)
-            if constexpr (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
-        });
-        wave_sync();
-    });
-#pragma unroll
-    for (int qq = 0; qq < Q; qq++)
-#pragma unroll
-        for (int r = 0; r < R; r++) y[qq][r] *= invd[r];
-    return ok;
+    double invd[R];
+    return factor_forward<NB, G, Q>(a, y, c0, col, g, invd);
 }
 
 // Drift block by modified Gram-Schmidt on the augmented columns
@@ -760,6 +778,313 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
         for (int q = 0; q < GPW; q++) s += s_sum[q * kMaxModelsPerLaunch + p];
         P.partial[(int64_t)blockIdx.x * nm + p] = s;
     }
+}
+
+// ---- Vecchia log-likelihood gradient --------------------------------------
+// loglik_grad_inner (vecchia.rs:428-566) for many single-structure isotropic
+// models of the five closed-form kinds (g_and_dg, vecchia.rs:353-385).
+// Per ordered point, with d = h / range, correlations r_a = 1 - g(d_ia),
+// R_ab = 1 - g(d_ab) (diagonal 1), K = nugget I + psill R, k = psill r, and
+// the range-derivative factors rho_a = d_ia g'(d_ia), P_ab = d_ab g'(d_ab):
+// factor K = L L' once and solve yr = L^-1 r, yz = L^-1 z,
+// wr = L^-T yr (= K^-1 r), v = L^-T yz (= K^-1 z).  Then with w = psill wr
+// (the reference's K^-1 k):
+//   mu = psill yr.yz, reduction = psill^2 |yr|^2, var = max(sill - reduction, 1e-12)
+//   d/dnugget: dmu = -psill wr.v,  dvar = 1 + psill^2 |wr|^2
+//   d/dpsill:  dmu = nugget wr.v,  dvar = 1 - psill (nugget |wr|^2 + |yr|^2)
+//   d/drange:  dmu = psill/range (rho.v - psill wr'Pv),
+//              dvar = -psill^2/range (2 rho.wr - psill wr'P wr)
+// which are the reference's solves with right-hand sides dk/dtheta -
+// dK/dtheta w written through the same factor (dK/dnugget = I, dK/dpsill =
+// R, dK/drange = psill/range P); every dvar is zeroed where the variance is
+// clamped, as the reference does.  The per-point terms
+// -0.5 dvar/var + r dmu/var + 0.5 r^2 dvar/var^2 (r = z_i - mu) and the
+// log-density are summed per model in the same fixed order as the
+// log-likelihood kernel.  Solving with r instead of k keeps psill = 0 exact.
+
+template <int D>
+struct GradRegs {
+    LeanRegs<D> L;
+    double r1, r2, r3;   // d g'(d) = d (r1 + d (r2 + d r3)) e^-q' (exponential family)
+};
+
+template <int D>
+__device__ __forceinline__ GradRegs<D> load_grad(const DevStructure &S) {
+    GradRegs<D> R;
+    R.L = load_lean<D>(S);
+    R.r1 = S.kind == K_EXP ? 1.0 : 0.0;
+    R.r2 = S.kind == K_GAU ? 2.0 : (S.kind == K_MAT15 ? 3.0 : (S.kind == K_MAT25 ? 5.0 / 3.0 : 0.0));
+    R.r3 = S.kind == K_MAT25 ? 5.0 * 2.23606797749979 / 3.0 : 0.0;
+    return R;
+}
+
+// Correlation c = 1 - g(d) and rho = d g'(d) for an isotropic separation.
+template <int D, bool SPH>
+__device__ __forceinline__ void grad_pair(const GradRegs<D> &R, const double *dh, double &c,
+                                          double &rho) {
+    double h2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) h2 = fma(dh[d], dh[d], h2);
+    const double h2s = h2 * R.L.ir2;
+    const double d = sqrt_lean(h2s);
+    if constexpr (SPH) {
+        const bool in = d < 1.0;
+        c = in ? 1.0 - (1.5 * d - 0.5 * d * d * d) : 0.0;
+        rho = in ? d * (1.5 - 1.5 * d * d) : 0.0;
+    } else {
+        const double q = R.L.alpha * d;
+        const double e = exp_neg(R.L.gau ? h2s : q);
+        c = fma(fma(R.L.gam, q, R.L.beta), q, 1.0) * e;
+        rho = d * fma(fma(R.r3, d, R.r2), d, R.r1) * e;
+    }
+}
+
+// Entries e0 <= e < e1 of block pair bp: correlations to the staging column,
+// P to the lane's persistent column (zero for rows past the count).
+template <int D, int NB, bool SPH>
+__device__ __forceinline__ void grad_slots(const GradRegs<D> &R, const int e0, const int e1,
+                                           const int iA, const bool liveA, const bool liveB,
+                                           const double (&xA)[D], const double (&xB)[D],
+                                           const double *xy, double *stage, double *pcol,
+                                           const int lane) {
+    for (int e = e0; e < e1; e++) {
+        const bool inA = e < iA;
+        const int c = inA ? e : NB - 2 - e;
+        double dh[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
+        double cv, rho;
+        grad_pair<D, SPH>(R, dh, cv, rho);
+        stage[(e - e0) * kWave + lane] = cv;
+        pcol[e * kWave + lane] = (inA ? liveA : liveB) ? rho : 0.0;
+    }
+}
+
+template <int D, int NB, int G>
+__global__ __launch_bounds__(kWave) void vecchia_grad_kernel(VecchiaParams P) {
+    constexpr int R = NB / G;
+    static_assert(R >= 2 && R % 2 == 0, "gradient kernel pairs row blocks");
+    constexpr int GPW = kWave / G;
+    constexpr int NE = NB - 1;
+    constexpr int HS = stage_slots<NB>();
+    constexpr int NP = R / 2;
+    constexpr int kQ = 4;                       // loglik, d/dnugget, d/dpsill, d/drange
+    __shared__ double s_xy[GPW * NB * D];
+    __shared__ double s_stage[HS * kWave];
+    __shared__ double s_p[NP * NE * kWave];
+    __shared__ double s_wv[GPW * 2 * NB];
+    __shared__ double s_sum[GPW * kMaxModelsPerLaunch * kQ];
+    __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, 2, G>::value];
+    const int lane = threadIdx.x;
+    const int g = lane % G;
+    const int grp = lane / G;
+    double *xy = s_xy + grp * NB * D;
+    double *colbuf = s_col + grp * ColStride<NB, 2, G>::value;
+    double *wv = s_wv + grp * 2 * NB;
+    double *sums = s_sum + grp * kMaxModelsPerLaunch * kQ;
+    const int nm = P.n_models;
+    for (int q = g; q < nm * kQ; q += G) sums[q] = 0.0;
+    wave_sync();
+    const double ln2pi = 1.8378770664093453;   // ln(2 pi)
+
+    const int64_t k0 = (int64_t)blockIdx.x * P.points_per_block;
+    int64_t k1 = k0 + P.points_per_block;
+    if (k1 > P.n) k1 = P.n;
+    for (int64_t k = k0 + grp; k < k1; k += GPW) {
+        const int64_t self = P.order[k];
+        const int cnt = P.nb_count[k];
+        const int32_t *nbl = P.nb + k * (int64_t)P.m;
+        double tg[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) tg[d] = P.coords[self * D + d];
+        const double zi = P.zc[self];
+        for (int j = g; j < cnt; j += G) {
+            const int64_t id = nbl[j];
+#pragma unroll
+            for (int d = 0; d < D; d++) xy[j * D + d] = P.coords[id * D + d];
+        }
+        double zr[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = row_of<G>(r, g);
+            zr[r] = i < cnt ? P.zc[nbl[i]] : 0.0;
+        }
+        wave_sync();
+        double rowx[R][D];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = row_of<G>(r, g);
+#pragma unroll
+            for (int d = 0; d < D; d++) rowx[r][d] = i < cnt ? xy[i * D + d] : 0.0;
+        }
+        for (int p = 0; p < nm; p++) {
+            const DevModel &Mdl = P.models[p];
+            const double nug = Mdl.nugget;
+            const double sill = Mdl.c0;
+            const double psill = Mdl.s[0].sill;
+            const double ir = Mdl.s[0].inv_range;
+            double ll, gn, gp, gr;
+            bool ok = true;
+            if (cnt == 0) {
+                // first point: N(0, sill); only nugget and psill move sill
+                const double var = sill, r = zi;
+                ll = -0.5 * (ln2pi + log(var) + r * r / var);
+                const double common = -0.5 / var + 0.5 * r * r / (var * var);
+                gn = common;
+                gp = common;
+                gr = 0.0;
+            } else {
+                const int gg = opaque(g);
+                const GradRegs<D> GR = load_grad<D>(Mdl.s[0]);
+                const bool sph = GR.L.sph;
+                double a[R][NB], y[2][R], rho[R];
+                static_for<0, NP>([&](auto BP) {
+                    constexpr int bp = decltype(BP)::value;
+                    constexpr int rA = bp, rB = R - 1 - bp;
+                    const int iA = row_of<G>(rA, gg);
+                    const int iB = NB - 1 - iA;
+                    const bool liveA = iA < cnt, liveB = iB < cnt;
+                    double *pcol = s_p + (size_t)bp * NE * kWave;
+                    static_for<0, 2>([&](auto PS) {
+                        constexpr int e0 = decltype(PS)::value * HS;
+                        constexpr int e1 = e0 + HS < NE ? e0 + HS : NE;
+                        if (sph) grad_slots<D, NB, true>(GR, e0, e1, iA, liveA, liveB, rowx[rA], rowx[rB], xy, s_stage, pcol, lane);
+                        else grad_slots<D, NB, false>(GR, e0, e1, iA, liveA, liveB, rowx[rA], rowx[rB], xy, s_stage, pcol, lane);
+                        static_for<0, G * (rA + 1)>([&](auto CC) {
+                            constexpr int c = decltype(CC)::value;
+                            if constexpr (c >= e0 && c < e1)
+                                a[rA][c] = liveA ? psill * s_stage[(c - e0) * kWave + lane] : 0.0;
+                        });
+                        static_for<0, G * (rB + 1)>([&](auto CC) {
+                            constexpr int c = decltype(CC)::value;
+                            constexpr int e = NB - 2 - c;
+                            if constexpr (e >= e0 && e < e1)
+                                a[rB][c] = liveB ? psill * s_stage[(e - e0) * kWave + lane] : 0.0;
+                        });
+                    });
+                    if constexpr (G * (rB + 1) == NB) a[rB][NB - 1] = 0.0;
+                });
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bool live = row_of<G>(r, gg) < cnt;
+                    double dh[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) dh[d] = rowx[r][d] - tg[d];
+                    double cv, rv;
+                    if (sph) grad_pair<D, true>(GR, dh, cv, rv);
+                    else grad_pair<D, false>(GR, dh, cv, rv);
+                    y[0][r] = live ? cv : 0.0;
+                    y[1][r] = zr[r];
+                    rho[r] = live ? rv : 0.0;
+                }
+                double invd[R];
+                ok = factor_forward<NB, G, 2>(a, y, sill, colbuf, gg, invd);
+                // back substitution L' x = y for both right-hand sides; a lane
+                // only needs x of its own rows
+                double x[2][R];
+#pragma unroll
+                for (int r = 0; r < R; r++) x[0][r] = x[1][r] = 0.0;
+                int rid[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) rid[r] = row_of<G>(r, gg);
+                static_for<0, NB>([&](auto JJ) {
+                    constexpr int j = NB - 1 - decltype(JJ)::value;
+                    constexpr int oj = owner_of<G>(j);
+                    constexpr int rj = j / G;
+                    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                    for (int r = rj; r < R; r++) {
+                        const bool below = rid[r] > j;
+                        s0 += below ? a[r][j] * x[0][r] : 0.0;
+                        s1 += below ? a[r][j] * x[1][r] : 0.0;
+                    }
+                    s0 = gsum<G>(s0);
+                    s1 = gsum<G>(s1);
+                    const bool own = gg == oj;
+                    x[0][rj] = own ? (y[0][rj] - invd[rj] * s0) * invd[rj] : x[0][rj];
+                    x[1][rj] = own ? (y[1][rj] - invd[rj] * s1) * invd[rj] : x[1][rj];
+                });
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    wv[rid[r]] = x[0][r];
+                    wv[NB + rid[r]] = x[1][r];
+                }
+                wave_sync();
+                double yr2 = 0.0, yrz = 0.0, w2 = 0.0, wvv = 0.0, rw = 0.0, rvv = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    yr2 += y[0][r] * y[0][r];
+                    yrz += y[0][r] * y[1][r];
+                    w2 += x[0][r] * x[0][r];
+                    wvv += x[0][r] * x[1][r];
+                    rw += rho[r] * x[0][r];
+                    rvv += rho[r] * x[1][r];
+                }
+                // bilinear forms wr'P wr and wr'P v over the lane's entries
+                double b1 = 0.0, b2 = 0.0;
+                static_for<0, NP>([&](auto BP) {
+                    constexpr int bp = decltype(BP)::value;
+                    constexpr int rA = bp, rB = R - 1 - bp;
+                    const int iA = rid[rA];
+                    const double *pcol = s_p + (size_t)bp * NE * kWave;
+                    for (int e = 0; e < NE; e++) {
+                        const bool inA = e < iA;
+                        const int c = inA ? e : NB - 2 - e;
+                        const double pe = pcol[e * kWave + lane];
+                        const double wa = inA ? x[0][rA] : x[0][rB];
+                        const double va = inA ? x[1][rA] : x[1][rB];
+                        const double wb = wv[c], vb = wv[NB + c];
+                        b1 += pe * (wa * wb);
+                        b2 += pe * fma(wa, vb, wb * va);
+                    }
+                });
+                yr2 = gsum<G>(yr2);
+                yrz = gsum<G>(yrz);
+                w2 = gsum<G>(w2);
+                wvv = gsum<G>(wvv);
+                rw = gsum<G>(rw);
+                rvv = gsum<G>(rvv);
+                b1 = 2.0 * gsum<G>(b1);
+                b2 = gsum<G>(b2);
+                wave_sync();   // wv and the staging columns are rewritten next
+                const double mu = psill * yrz;
+                const double red = psill * psill * yr2;
+                const double raw = sill - red;
+                const double var = fmax(raw, 1e-12);
+                const bool clamped = raw <= 1e-12;
+                const double r = zi - mu;
+                ll = -0.5 * (ln2pi + log(var) + r * r / var);
+                const double inv_var = 1.0 / var;
+                auto term = [&](double dvar, double dmu) {
+                    dvar = clamped ? 0.0 : dvar;
+                    return -0.5 * dvar * inv_var + r * dmu * inv_var + 0.5 * r * r * dvar * inv_var * inv_var;
+                };
+                gn = term(1.0 + psill * psill * w2, -psill * wvv);
+                gp = term(1.0 - psill * (nug * w2 + yr2), nug * wvv);
+                gr = term(-psill * psill * ir * (2.0 * rw - psill * b1), psill * ir * (rvv - psill * b2));
+            }
+            if (g == 0) {
+                sums[p * kQ + 0] += ll;
+                sums[p * kQ + 1] += gn;
+                sums[p * kQ + 2] += gp;
+                sums[p * kQ + 3] += gr;
+                if (!ok) atomicMax(&P.status[p], 3);
+            }
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    for (int c = lane; c < nm * kQ; c += kWave) {
+        double acc = 0.0;
+        for (int q = 0; q < GPW; q++) acc += s_sum[q * kMaxModelsPerLaunch * kQ + c];
+        P.partial[(int64_t)blockIdx.x * nm * kQ + c] = acc;
+    }
+}
+
+template <int D, int NB, int G>
+hipError_t launch_vecchia_grad_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s) {
+    hipLaunchKernelGGL((vecchia_grad_kernel<D, NB, G>), dim3((unsigned)n_blocks), dim3(kWave), 0, s, p);
+    return hipGetLastError();
 }
 
 // ---- launchers (instantiated per template argument set) -------------------

@@ -509,6 +509,24 @@ class VecchiaPlan:
                                                   _abi.i32ptr(st)))
         return out, st
 
+    def loglik_grad_batch(self, values, models: Sequence[VariogramModel], precision="fp64"):
+        """Batched loglik_grad_with_plan (vecchia.rs:416-566): (loglik,
+        grad (n_models, 3) as d/dnugget, d/dpsill, d/drange, status); status
+        GS_ERR_INVALID_PARAMETER where the reference returns None (kinds
+        without a closed-form range derivative, several structures,
+        anisotropy), 3 where a conditional is singular."""
+        v = _f64(values).reshape(-1)
+        if v.shape[0] != self.n:
+            raise DimensionMismatch(f"{v.shape[0]} values for {self.n} points")
+        arr = (_abi.gs_model * len(models))(*[mm._c() for mm in models])
+        ll = np.empty(len(models))
+        grad = np.empty((len(models), 3))
+        st = np.empty(len(models), dtype=np.int32)
+        _check(_abi.lib().gs_vecchia_loglik_grad_batch(self._h, _abi.dptr(v), arr, len(models),
+                                                       _PREC[precision], _abi.dptr(ll),
+                                                       _abi.dptr(grad), _abi.i32ptr(st)))
+        return ll, grad, st
+
     def loglik(self, values, model: VariogramModel, precision="fp64") -> float:
         out, st = self.loglik_batch(values, [model], precision)
         if st[0] != 0:

@@ -74,6 +74,8 @@ SIGNATURES = {
     "gs_vecchia_plan_get": (C.c_int32, [_P, _I64P, _I32P, _I32P]),
     "gs_vecchia_loglik_batch": (C.c_int32, [_P, _DP, C.POINTER(gs_model), C.c_int32, C.c_int32,
                                             _DP, _I32P]),
+    "gs_vecchia_loglik_grad_batch": (C.c_int32, [_P, _DP, C.POINTER(gs_model), C.c_int32,
+                                                 C.c_int32, _DP, _DP, _I32P]),
     "gs_vecchia_loglik": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
                                       C.c_int64, _I64P, _DP]),
 }

@@ -177,6 +177,20 @@ int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_mode
                                 int32_t n_models, int32_t precision, double *out,
                                 int32_t *status);
 
+/* Batched loglik_grad_with_plan (vecchia.rs:416-566): log-likelihood and its
+ * gradient in (nugget, psill, range) for n_models single-structure
+ * isotropic models of the kinds with a closed-form range derivative
+ * (spherical, exponential, Gaussian, Matern 3/2, 5/2; g_and_dg,
+ * vecchia.rs:353-385).  loglik[i], grad[3*i + {0,1,2}] = d/dnugget,
+ * d/dpsill, d/drange.  status[i]: GS_OK; GS_ERR_INVALID_PARAMETER for a
+ * model the reference answers with None (other kinds, several structures,
+ * anisotropy); GS_ERR_SINGULAR_SYSTEM as in gs_vecchia_loglik_batch.
+ * Outputs of a model with a non-zero status are NaN.  Plans whose
+ * conditioning sets exceed 32 points are rejected. */
+int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs_model *models,
+                                     int32_t n_models, int32_t precision, double *loglik,
+                                     double *grad, int32_t *status);
+
 /* vecchia_loglik (vecchia.rs:874-884): plan + one likelihood. */
 int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
                           int64_t n, const gs_model *model, int64_t m, const int64_t *order,

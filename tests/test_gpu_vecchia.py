@@ -108,3 +108,90 @@ def test_singular_and_rejections(oracle):
     with pytest.raises(gh.DuplicatePoints) as ei:
         gh.vecchia_plan(np.vstack([c, c[:1]]), 3)
     assert (ei.value.i, ei.value.j) == (0, 5)
+
+
+# ---- gradient (loglik_grad_with_plan, vecchia.rs:416-566) -----------------
+GRAD_KINDS = (gh.ModelKind.SPHERICAL, gh.ModelKind.EXPONENTIAL, gh.ModelKind.GAUSSIAN,
+              gh.ModelKind.MATERN15, gh.ModelKind.MATERN25)
+GRAD_TOL = 1e-8      # normwise relative, floor 1
+
+
+def _grad_models(rng_scale):
+    out = []
+    params = [(0.01, 1.0, 0.05), (0.2, 0.7, 0.12), (0.0, 1.3, 0.03), (0.5, 1e-3, 0.08),
+              (0.05, 2.0, 0.2)]
+    for kind in GRAD_KINDS:
+        for nug, ps, rg in params:
+            if kind == gh.ModelKind.GAUSSIAN and nug == 0.0:
+                nug = 0.02      # a nugget-free Gaussian model is numerically singular
+            out.append(gh.VariogramModel(nug, [gh.Structure(kind, ps, rg * rng_scale)]))
+    return out
+
+
+def _check_grad(oracle, c, vals, o, nb, cnt, p, models):
+    ll, g, st = p.loglik_grad_batch(vals, models)
+    ll2, st2 = p.loglik_batch(vals, models)
+    assert (st == 0).all() and (st2 == 0).all(), (st, st2)
+    for i, m in enumerate(models):
+        s0 = m.structures[0]
+        rc, oll, og = oracle.vecchia_loglik_grad_plan(c, vals, s0.kind[0], m.nugget, s0.sill,
+                                                      s0.range, o, nb, cnt)
+        assert rc == 0
+        assert abs(ll[i] - oll) <= TOL64 * abs(oll), (i, ll[i], oll)
+        assert abs(ll[i] - ll2[i]) <= TOL64 * abs(oll)
+        assert np.max(np.abs(g[i] - og)) <= GRAD_TOL * max(np.max(np.abs(og)), 1.0), (i, g[i], og)
+
+
+@pytest.mark.gpu
+def test_gradient_matches_oracle_2d(oracle):
+    c, _ = workloads.points(71, 2500, [1.0, 1.0], recipe=2)
+    p, o, nb, cnt = plan_equal(oracle, c, 30)
+    vals = workloads.vecchia_field(c, o, nb, cnt, 30, seed=72)
+    _check_grad(oracle, c, vals, o, nb, cnt, p, _grad_models(1.0))
+
+
+@pytest.mark.gpu
+def test_gradient_matches_oracle_3d_small_sets(oracle):
+    c, v = workloads.points(73, 1500, [100.0, 100.0, 30.0], recipe=1)
+    p, o, nb, cnt = plan_equal(oracle, c, 12)
+    _check_grad(oracle, c, v, o, nb, cnt, p, _grad_models(100.0))
+
+
+@pytest.mark.gpu
+def test_gradient_clamped_variance(oracle):
+    # vecchia.rs:2939-2990: the clamped point contributes exactly 0 to d/dpsill
+    c = np.array([[0.0, 0.0], [1e-13, 0.0]])
+    v = np.array([0.3, -0.7])
+    p = gh.vecchia_plan(c, 1, np.array([0, 1], dtype=np.int64))
+    m = gh.VariogramModel(0.0, [gh.Structure(gh.ModelKind.SPHERICAL, 1.0, 10.0)])
+    ll, g, st = p.loglik_grad_batch(v, [m])
+    assert st[0] == 0
+    assert abs(g[0, 1] - (-0.5 + 0.5 * 0.25)) < 1e-9
+    o, nb, cnt = p.arrays()
+    rc, oll, og = oracle.vecchia_loglik_grad_plan(c, v, 0, 0.0, 1.0, 10.0, o, nb, cnt)
+    # d/dnugget is ~5e11 here (r dmu / var with var floored at 1e-12)
+    assert abs(ll[0] - oll) <= 1e-12 * abs(oll)
+    assert np.max(np.abs(g[0] - og)) <= 1e-12 * max(np.max(np.abs(og)), 1.0)
+
+
+@pytest.mark.gpu
+def test_gradient_rejections_and_determinism(oracle):
+    c, _ = workloads.points(74, 800, [1.0, 1.0], recipe=2)
+    p, o, nb, cnt = plan_equal(oracle, c, 20)
+    vals = workloads.vecchia_field(c, o, nb, cnt, 20, seed=75)
+    good = gh.VariogramModel(0.05, [gh.Structure(gh.ModelKind.MATERN25, 1.0, 0.05)])
+    models = [good,
+              gh.VariogramModel(0.05, [gh.Structure(gh.ModelKind.CIRCULAR, 1.0, 0.05)]),
+              gh.VariogramModel(0.05, [gh.Structure(gh.ModelKind.EXPONENTIAL, 0.5, 0.05),
+                                       gh.Structure(gh.ModelKind.SPHERICAL, 0.5, 0.2)]),
+              gh.VariogramModel(0.05, [gh.Structure(gh.ModelKind.EXPONENTIAL, 1.0, 0.05,
+                                                    gh.Anisotropy(30.0, 0.5))]),
+              good]
+    ll, g, st = p.loglik_grad_batch(vals, models)
+    assert list(st) == [0, 4, 4, 4, 0]
+    assert np.isnan(ll[1:4]).all() and np.isnan(g[1:4]).all()
+    ll_b, g_b, _ = p.loglik_grad_batch(vals, models)
+    assert ll[0] == ll[4] == ll_b[0] and (g[0] == g[4]).all() and (g[0] == g_b[0]).all()
+    big = gh.vecchia_plan(c, 40)
+    with pytest.raises(gh.InvalidParameter):
+        big.loglik_grad_batch(vals, [good])
