@@ -109,6 +109,14 @@ def bench_vecchia(args) -> dict:
     for _ in range(args.steps):
         ll, st = plan.loglik_batch(vals, models)
     dt = (time.perf_counter() - t0) / args.steps
+    grad_ms = None
+    if args.grad:
+        gmodels = workloads.vecchia_models(64)
+        plan.loglik_grad_batch(vals, gmodels)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, _, gst = plan.loglik_grad_batch(vals, gmodels)
+        grad_ms = (time.perf_counter() - t0) / args.steps * 1e3
     hist = np.bincount(cnt, minlength=m + 1)
     flops = 64 * float(sum(hist[s] * vecchia_flops_per_point(s, 19) for s in range(m + 1) if hist[s]))
     return {
@@ -118,7 +126,8 @@ def bench_vecchia(args) -> dict:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded, sequential Vecchia draw over the plan)",
         "config": {"workload": "vecchia_c3", "points": int(n), "m": m, "models": 64,
-                   "plan_build_s": plan_s, "ok_models": int((st == 0).sum())},
+                   "plan_build_s": plan_s, "ok_models": int((st == 0).sum()),
+                   "grad_ms_per_batch": grad_ms},
         "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / dt / 1e12 / FP64_PEAK_TFLOPS,
                      "traffic": None, "kernel": "vecchia_kernel", "flops_per_launch": flops},
@@ -187,6 +196,8 @@ def main() -> int:
                     choices=["ok2d_large", "ok2d_small", "uk3d_octant", "vecchia_c3", "serving"])
     ap.add_argument("--points", type=int, default=0, help="vecchia_c3: number of points")
     ap.add_argument("--requests", type=int, default=40, help="serving: number of requests")
+    ap.add_argument("--grad", action="store_true",
+                    help="vecchia_c3: also time the 64-model log-likelihood + gradient batch")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32r"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--host-io", action="store_true",
