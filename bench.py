@@ -117,6 +117,13 @@ def bench_vecchia(args) -> dict:
         for _ in range(args.steps):
             _, _, gst = plan.loglik_grad_batch(vals, gmodels)
         grad_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    mle = None
+    if args.mle:
+        t0 = time.perf_counter()
+        fit = plan.mle(vals, gh.ModelKind.MATERN25)
+        s = fit.model.structures[0]
+        mle = {"seconds": time.perf_counter() - t0, "kind": "matern25", "loglik": fit.loglik,
+               "nugget": fit.model.nugget, "psill": s.sill, "range": s.range}
     hist = np.bincount(cnt, minlength=m + 1)
     flops = 64 * float(sum(hist[s] * vecchia_flops_per_point(s, 19) for s in range(m + 1) if hist[s]))
     return {
@@ -127,7 +134,7 @@ def bench_vecchia(args) -> dict:
         "data": "synthetic (seeded, sequential Vecchia draw over the plan)",
         "config": {"workload": "vecchia_c3", "points": int(n), "m": m, "models": 64,
                    "plan_build_s": plan_s, "ok_models": int((st == 0).sum()),
-                   "grad_ms_per_batch": grad_ms},
+                   "grad_ms_per_batch": grad_ms, "mle_fit": mle},
         "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / dt / 1e12 / FP64_PEAK_TFLOPS,
                      "traffic": None, "kernel": "vecchia_kernel", "flops_per_launch": flops},
@@ -198,6 +205,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=40, help="serving: number of requests")
     ap.add_argument("--grad", action="store_true",
                     help="vecchia_c3: also time the 64-model log-likelihood + gradient batch")
+    ap.add_argument("--mle", action="store_true",
+                    help="vecchia_c3: also time one maximum-likelihood fit (Matern 5/2)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32r"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--host-io", action="store_true",

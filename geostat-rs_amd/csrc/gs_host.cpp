@@ -445,9 +445,15 @@ struct gs_vplan {
     std::vector<int32_t> nb;
     std::vector<int32_t> nb_count;
     int32_t max_count = 0;
+    double extent = 0.0;     // length of the bounding-box diagonal (MLE start scale)
     DBuf coords, d_order, d_nb, d_count, zc, models, partial, out, status;
     std::mutex mu;
 };
+
+int gs_plan_dim(const gs_vplan *p) { return p->D; }
+int64_t gs_plan_points(const gs_vplan *p) { return p->n; }
+double gs_plan_extent(const gs_vplan *p) { return p->extent; }
+int32_t gs_fail(int32_t code, const char *msg) { return fail(code, msg); }
 
 namespace {
 
@@ -1389,6 +1395,11 @@ int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int6
     worker();
     for (auto &t : th) t.join();
     for (int64_t k = 0; k < n; k++) P->max_count = std::max(P->max_count, P->nb_count[(size_t)k]);
+    {
+        double e2 = 0.0;      // mle_fit_with_plan's extent (vecchia.rs:965-974)
+        for (int d = 0; d < dim; d++) e2 += (mx[d] - mn[d]) * (mx[d] - mn[d]);
+        P->extent = std::sqrt(e2);
+    }
 
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(P->coords.ensure(sizeof(double) * n * dim));

@@ -7,6 +7,14 @@
 
 #include "gs_device.hpp"
 
+struct gs_vplan;
+// Vecchia plan properties for the MLE driver (defined with the plan).
+int gs_plan_dim(const gs_vplan *p);
+int64_t gs_plan_points(const gs_vplan *p);
+double gs_plan_extent(const gs_vplan *p);
+// Records msg as the calling thread's last error and returns code.
+int32_t gs_fail(int32_t code, const char *msg);
+
 namespace gs {
 
 // Search frame: search_point (kriging.rs:16-21) applies

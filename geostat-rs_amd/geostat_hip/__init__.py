@@ -34,7 +34,8 @@ __all__ = [
     "InvalidParameter", "NoNeighbors", "DuplicatePoints", "DeviceError",
     "ModelKind", "Anisotropy", "Structure", "VariogramModel", "KrigingMethod",
     "KrigingConfig", "KrigingEstimate", "Context", "Kriging", "Grid2D", "Grid3D",
-    "VecchiaPlan", "vecchia_plan", "vecchia_loglik", "PREC_FP64", "PREC_FP32_REFINED",
+    "VecchiaPlan", "VecchiaFit", "vecchia_plan", "vecchia_loglik", "vecchia_mle", "PREC_FP64",
+    "PREC_FP32_REFINED",
 ]
 
 
@@ -532,6 +533,42 @@ class VecchiaPlan:
         if st[0] != 0:
             raise SingularSystem("singular conditioning system")
         return float(out[0])
+
+    def mle(self, values, kind) -> "VecchiaFit":
+        """mle_fit_with_plan (vecchia.rs:952-1081): maximum-likelihood fit
+        of nugget + one structure of `kind` over this plan."""
+        v = _f64(values).reshape(-1)
+        if v.shape[0] != self.n:
+            raise DimensionMismatch(f"{v.shape[0]} values for {self.n} points")
+        code, param = kind
+        fit = _abi.gs_model()
+        ll = C.c_double()
+        _check(_abi.lib().gs_vecchia_mle(self._h, _abi.dptr(v), int(code), float(param),
+                                         C.byref(fit), C.byref(ll)))
+        s = fit.structures[0]
+        model = VariogramModel(fit.nugget, [Structure((int(s.kind), float(s.param)), s.sill,
+                                                      s.range)])
+        return VecchiaFit(model, float(ll.value))
+
+
+@dataclass
+class VecchiaFit:
+    """VecchiaFit (vecchia.rs:904-911): fitted model and its maximized
+    Vecchia log-likelihood."""
+    model: VariogramModel
+    loglik: float
+
+
+def vecchia_mle(coords, values, kind, m: int, order=None,
+                ctx: Optional[Context] = None) -> VecchiaFit:
+    """vecchia_mle (vecchia.rs:921-932): plan + maximum-likelihood fit."""
+    code = kind[0]
+    if code == ModelKind.Power(1.0)[0]:
+        raise InvalidParameter("Power models cannot be fitted by Vecchia MLE")
+    coords = _f64(coords)
+    if code == ModelKind.CIRCULAR[0] and coords.shape[1] > 2:
+        raise InvalidParameter("Circular is not a valid covariance in 3 dimensions")
+    return VecchiaPlan(coords, m, order, ctx).mle(values, kind)
 
 
 def vecchia_plan(coords, m: int, order=None, ctx: Optional[Context] = None) -> VecchiaPlan:

@@ -78,6 +78,7 @@ SIGNATURES = {
                                                  C.c_int32, _DP, _DP, _I32P]),
     "gs_vecchia_loglik": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
                                       C.c_int64, _I64P, _DP]),
+    "gs_vecchia_mle": (C.c_int32, [_P, _DP, C.c_int32, C.c_double, C.POINTER(gs_model), _DP]),
 }
 
 _lib = None

@@ -196,6 +196,21 @@ int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const 
                           int64_t n, const gs_model *model, int64_t m, const int64_t *order,
                           double *out);
 
+/* Vecchia maximum-likelihood fit of one structure of `kind` (param: the
+ * Matern nu or Stable alpha, ignored otherwise) plus a nugget over a built
+ * plan: mle_fit_with_plan (vecchia.rs:952-1081) as called by vecchia_mle
+ * (vecchia.rs:921-932).  Same parametrization (sqrt nugget, ln psill,
+ * ln range), starts, range regularization and optimizers (BFGS for the
+ * g_and_dg kinds, Nelder-Mead otherwise; optim.rs:6-288); the starts run in
+ * lockstep and each round of their evaluations is one batched launch.
+ * fit receives the fitted model (nugget, one isotropic structure), loglik
+ * its Vecchia log-likelihood.  GS_ERR_INVALID_PARAMETER for Power, for
+ * Circular in 3-D, and (BFGS kinds) for plans whose conditioning sets
+ * exceed 32 points; GS_ERR_SINGULAR_SYSTEM when every start stays on a
+ * singular covariance. */
+int32_t gs_vecchia_mle(gs_vplan *p, const double *values, int32_t kind, double param,
+                       gs_model *fit, double *loglik);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1490,3 +1490,249 @@ int32_t rc_dense_mvn_loglik(int32_t D, const double *coords, const double *value
     free(z); free(x); free(C);
     return st;
 }
+
+/* ---------------------------------------------------------------- optim.rs */
+/* Objective callback: f(x) and, when g is non-NULL, the gradient. */
+typedef struct {
+    int32_t D;
+    const double *coords, *values;
+    int64_t n, m;
+    const int64_t *order;
+    const int32_t *nb, *nb_count;
+    int32_t kind;
+    double param, range_max;
+    int64_t evals;
+} mle_ctx;
+
+static double mle_reg(const mle_ctx *c, double range) {   /* vecchia.rs:988-996 */
+    if (range > c->range_max) {
+        double t = (range - c->range_max) / c->range_max;
+        return t * t;
+    }
+    return 0.0;
+}
+
+static double mle_f(mle_ctx *c, const double *x) {        /* vecchia.rs:998-1016 */
+    rc_model md;
+    memset(&md, 0, sizeof(md));
+    md.nugget = x[0] * x[0];
+    md.n_structures = 1;
+    md.s[0].kind = c->kind;
+    md.s[0].param = c->param;
+    md.s[0].sill = exp(x[1]);
+    md.s[0].range = exp(x[2]);
+    double ll;
+    c->evals++;
+    int32_t st = rc_vecchia_loglik_plan(c->D, c->coords, c->values, c->n, &md, c->order, c->nb,
+                                        c->nb_count, c->m, &ll);
+    if (st == RC_OK && isfinite(ll)) return -ll + 1e6 * mle_reg(c, md.s[0].range);
+    return 1e12;
+}
+
+static void mle_g(mle_ctx *c, const double *x, double *gx) { /* vecchia.rs:1030-1050 */
+    const double nugget = x[0] * x[0], psill = exp(x[1]), range = exp(x[2]);
+    double ll, g[3];
+    c->evals++;
+    int32_t st = rc_vecchia_loglik_grad_plan(c->D, c->coords, c->values, c->n, c->kind, nugget,
+                                             psill, range, c->order, c->nb, c->nb_count, c->m,
+                                             &ll, g);
+    if (st != RC_OK) { gx[0] = gx[1] = gx[2] = 0.0; return; }
+    gx[0] = -g[0] * 2.0 * x[0];
+    gx[1] = -g[1] * psill;
+    gx[2] = -g[2] * range;
+    if (range > c->range_max) {
+        double dreg = 2.0 * (range - c->range_max) / (c->range_max * c->range_max);
+        gx[2] += 1e6 * dreg * range;
+    }
+}
+
+static int f64_total_lt(double a, double b) { return total_key(a) < total_key(b); }
+
+/* nelder_mead (optim.rs:6-99) for n = 3 parameters. */
+static double nelder_mead3(mle_ctx *c, const double *x0, double step, int max_iter, double *xout) {
+    enum { N = 3 };
+    double sx[N + 1][N], fv[N + 1];
+    for (int j = 0; j < N; j++) sx[0][j] = x0[j];
+    for (int i = 0; i < N; i++) {
+        for (int j = 0; j < N; j++) sx[i + 1][j] = x0[j];
+        sx[i + 1][i] += step;
+    }
+    for (int i = 0; i <= N; i++) fv[i] = mle_f(c, sx[i]);
+    for (int it = 0; it < max_iter; it++) {
+        int idx[N + 1];
+        for (int i = 0; i <= N; i++) idx[i] = i;
+        for (int i = 1; i <= N; i++)   /* stable insertion sort by total_cmp */
+            for (int j = i; j > 0 && f64_total_lt(fv[idx[j]], fv[idx[j - 1]]); j--) {
+                int t = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = t;
+            }
+        double s2[N + 1][N], f2[N + 1];
+        for (int i = 0; i <= N; i++) {
+            memcpy(s2[i], sx[idx[i]], sizeof(s2[i]));
+            f2[i] = fv[idx[i]];
+        }
+        memcpy(sx, s2, sizeof(sx));
+        memcpy(fv, f2, sizeof(fv));
+        if (fabs(fv[N] - fv[0]) <= 1e-12 * fmax(fabs(fv[0]), 1e-12)) break;
+        double cen[N] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) cen[j] += sx[i][j] / (double)N;
+        double xr[N];
+        for (int j = 0; j < N; j++) xr[j] = cen[j] + 1.0 * (cen[j] - sx[N][j]);
+        double fr = mle_f(c, xr);
+        if (fr < fv[0]) {
+            double xe[N];
+            for (int j = 0; j < N; j++) xe[j] = cen[j] + 2.0 * (cen[j] - sx[N][j]);
+            double fe = mle_f(c, xe);
+            if (fe < fr) { memcpy(sx[N], xe, sizeof(xe)); fv[N] = fe; }
+            else { memcpy(sx[N], xr, sizeof(xr)); fv[N] = fr; }
+        } else if (fr < fv[N - 1]) {
+            memcpy(sx[N], xr, sizeof(xr));
+            fv[N] = fr;
+        } else {
+            double xc[N];
+            if (fr < fv[N])
+                for (int j = 0; j < N; j++) xc[j] = cen[j] + 0.5 * (xr[j] - cen[j]);
+            else
+                for (int j = 0; j < N; j++) xc[j] = cen[j] - 0.5 * (cen[j] - sx[N][j]);
+            double fc = mle_f(c, xc);
+            if (fc < fmin(fr, fv[N])) {
+                memcpy(sx[N], xc, sizeof(xc));
+                fv[N] = fc;
+            } else {
+                for (int i = 1; i <= N; i++) {
+                    for (int j = 0; j < N; j++) sx[i][j] = sx[0][j] + 0.5 * (sx[i][j] - sx[0][j]);
+                    fv[i] = mle_f(c, sx[i]);
+                }
+            }
+        }
+    }
+    int best = 0;
+    for (int i = 1; i <= N; i++)
+        if (fv[i] < fv[best]) best = i;
+    memcpy(xout, sx[best], sizeof(double) * N);
+    return fv[best];
+}
+
+static double dot3(const double *a, const double *b) {   /* optim.rs:125-127 */
+    double s = 0.0;
+    for (int i = 0; i < 3; i++) s += a[i] * b[i];
+    return s;
+}
+
+static void mat3_mul(const double *a, const double *b, double *out) { /* optim.rs:153-165 */
+    double c[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 3; k++) s += a[i * 3 + k] * b[k * 3 + j];
+            c[i * 3 + j] = s;
+        }
+    memcpy(out, c, sizeof(c));
+}
+
+static void bfgs_update3(double *h, const double *s, const double *y, double sy) { /* optim.rs:167-187 */
+    const double rho = 1.0 / sy;
+    double left[9], right[9], t[9];
+    for (int i = 0; i < 9; i++) left[i] = right[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            left[i * 3 + j] -= rho * (s[i] * y[j]);
+            right[i * 3 + j] -= rho * (y[i] * s[j]);
+        }
+    mat3_mul(left, h, t);
+    mat3_mul(t, right, h);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) h[i * 3 + j] += rho * (s[i] * s[j]);
+}
+
+/* bfgs (optim.rs:206-269) for n = 3 parameters. */
+static double bfgs3(mle_ctx *c, const double *x0, int max_iter, double *xout) {
+    double x[3], h[9], g[3], bx[3];
+    memcpy(x, x0, sizeof(x));
+    for (int i = 0; i < 9; i++) h[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    mle_g(c, x, g);
+    double fx = mle_f(c, x);
+    memcpy(bx, x, sizeof(x));
+    double bf = fx;
+    for (int it = 0; it < max_iter; it++) {
+        if (sqrt(dot3(g, g)) < 1e-10) break;
+        double p[3];
+        for (int i = 0; i < 3; i++) {
+            double s = 0.0;
+            for (int j = 0; j < 3; j++) s += h[i * 3 + j] * g[j];
+            p[i] = -s;
+        }
+        double slope = dot3(g, p);
+        if (slope >= 0.0) {
+            for (int i = 0; i < 9; i++) h[i] = (i % 4 == 0) ? 1.0 : 0.0;
+            for (int i = 0; i < 3; i++) p[i] = -g[i];
+            slope = dot3(g, p);
+        }
+        double alpha = 1.0, xn[3], fn;
+        for (;;) {
+            for (int i = 0; i < 3; i++) xn[i] = x[i] + alpha * p[i];
+            fn = mle_f(c, xn);
+            if (fn <= fx + 1e-4 * alpha * slope || alpha < 1e-12) break;
+            alpha *= 0.5;
+        }
+        double gn[3], s[3], y[3];
+        mle_g(c, xn, gn);
+        for (int i = 0; i < 3; i++) { s[i] = xn[i] - x[i]; y[i] = gn[i] - g[i]; }
+        double sy = dot3(s, y);
+        if (sy > 1e-10) bfgs_update3(h, s, y, sy);
+        int stalled = fabs(fx - fn) <= 1e-13 * (1.0 + fabs(fx)) &&
+                      sqrt(dot3(s, s)) <= 1e-10 * (1.0 + sqrt(dot3(x, x)));
+        memcpy(x, xn, sizeof(x));
+        memcpy(g, gn, sizeof(g));
+        fx = fn;
+        if (fx < bf) { bf = fx; memcpy(bx, x, sizeof(x)); }
+        if (stalled) break;
+    }
+    memcpy(xout, bx, sizeof(bx));
+    return bf;
+}
+
+/* mle_fit_with_plan (vecchia.rs:952-1081) with group_size 1, after the kind
+ * checks of vecchia_mle (vecchia.rs:921-932).  fit = (nugget, psill, range). */
+int32_t rc_vecchia_mle_plan(int32_t D, const double *coords, const double *values, int64_t n,
+                            int32_t kind, double param, const int64_t *order, const int32_t *nb,
+                            const int32_t *nb_count, int64_t m, double *fit, double *loglik,
+                            int64_t *evals) {
+    if (kind == RC_POWER) return RC_INVALID_PARAMETER;
+    if (kind == RC_CIRCULAR && D > 2) return RC_INVALID_PARAMETER;
+    double mean = -0.0;                                    /* PointSet::mean: sum / n */
+    for (int64_t i = 0; i < n; i++) mean += values[i];
+    mean = mean / (double)n;
+    double ss = -0.0;
+    for (int64_t i = 0; i < n; i++) ss += (values[i] - mean) * (values[i] - mean);
+    const double var0 = fmax(ss / (double)n, 1e-12);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < D; d++) {
+            lo[d] = fmin(lo[d], coords[i * D + d]);
+            hi[d] = fmax(hi[d], coords[i * D + d]);
+        }
+    double e2 = -0.0;
+    for (int d = 0; d < D; d++) e2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    const double extent = sqrt(e2);
+    const double range0 = fmax(extent / 3.0, 1e-9);
+    mle_ctx c = {D, coords, values, n, m, order, nb, nb_count, kind, param, 10.0 * extent, 0};
+    const double facs[3] = {0.3, 1.0, 3.0};
+    double g0, dg0;
+    const int use_bfgs = g_and_dg(kind, 0.5, &g0, &dg0);
+    double xb[3] = {0, 0, 0}, fb = 0.0;
+    int have = 0;
+    for (int s = 0; s < 3; s++) {   /* bfgs_multistart / nelder_mead_multistart (optim.rs:106-123, 275-288) */
+        double x0[3] = {sqrt(0.1 * var0), log(0.9 * var0), log(range0) + log(facs[s])};
+        double xs[3], fs;
+        fs = use_bfgs ? bfgs3(&c, x0, 200, xs) : nelder_mead3(&c, x0, 0.3, 2000, xs);
+        if (!have || fs < fb) { memcpy(xb, xs, sizeof(xb)); fb = fs; have = 1; }
+    }
+    if (evals) *evals = c.evals;
+    if (fb >= 1e12) return RC_SINGULAR_SYSTEM;
+    fit[0] = xb[0] * xb[0];
+    fit[1] = exp(xb[1]);
+    fit[2] = exp(xb[2]);
+    *loglik = -fb;
+    return RC_OK;
+}

@@ -122,6 +122,12 @@ int32_t rc_vecchia_loglik_grad_plan(int32_t D, const double *coords, const doubl
                                     double range, const int64_t *order, const int32_t *nb,
                                     const int32_t *nb_count, int64_t m, double *ll_out,
                                     double *grad_out);
+/* mle_fit_with_plan (vecchia.rs:952-1081), group_size 1: fit = (nugget,
+ * psill, range); evals (may be NULL) counts objective + gradient calls. */
+int32_t rc_vecchia_mle_plan(int32_t D, const double *coords, const double *values, int64_t n,
+                            int32_t kind, double param, const int64_t *order, const int32_t *nb,
+                            const int32_t *nb_count, int64_t m, double *fit, double *loglik,
+                            int64_t *evals);
 int32_t rc_dense_mvn_loglik(int32_t D, const double *coords, const double *values, int64_t n,
                             const rc_model *model, double *out);
 

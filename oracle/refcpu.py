@@ -102,6 +102,9 @@ def lib() -> C.CDLL:
                                                         C.c_int32, C.c_double, C.c_double,
                                                         C.c_double, _I64P, _I32P, _I32P,
                                                         C.c_int64, _DP, _DP]),
+            "rc_vecchia_mle_plan": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64, C.c_int32,
+                                                C.c_double, _I64P, _I32P, _I32P, C.c_int64,
+                                                _DP, _DP, _I64P]),
             "rc_dense_mvn_loglik": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64,
                                                 C.POINTER(rc_model), _DP]),
             "rc_duplicate_pair": (C.c_int32, [C.c_int32, _DP, C.c_int64, _I64P]),
@@ -314,6 +317,23 @@ def vecchia_loglik_grad_plan(coords, values, kind, nugget, psill, rng, order, nb
         nb.ctypes.data_as(_I32P), _arr(cnt, np.int32).ctypes.data_as(_I32P), nb.shape[1],
         C.byref(ll), _d(grad))
     return rc, ll.value, grad
+
+
+def vecchia_mle_plan(coords, values, kind, param, order, nb, cnt):
+    """mle_fit_with_plan (vecchia.rs:952-1081): (rc, (nugget, psill, range),
+    loglik, evaluations)."""
+    coords = _arr(coords)
+    values = _arr(values)
+    nb = _arr(nb, np.int32)
+    fit = np.zeros(3)
+    ll = C.c_double()
+    ev = C.c_int64()
+    rc = lib().rc_vecchia_mle_plan(
+        coords.shape[1], _d(coords), _d(values), coords.shape[0], int(kind), float(param),
+        _arr(order, np.int64).ctypes.data_as(_I64P), nb.ctypes.data_as(_I32P),
+        _arr(cnt, np.int32).ctypes.data_as(_I32P), nb.shape[1], _d(fit), C.byref(ll),
+        C.byref(ev))
+    return rc, fit, ll.value, ev.value
 
 
 def dense_mvn_loglik(coords, values, m_):

@@ -195,3 +195,77 @@ def test_gradient_rejections_and_determinism(oracle):
     big = gh.vecchia_plan(c, 40)
     with pytest.raises(gh.InvalidParameter):
         big.loglik_grad_batch(vals, [good])
+
+
+# ---- maximum-likelihood fit (mle_fit_with_plan, vecchia.rs:952-1081) -------
+# The GPU driver runs the reference's optimizer steps with the batched GPU
+# likelihood; its likelihood differs from the oracle's by rounding only, so
+# the two optimizer paths may part at a near-tie in a line-search or simplex
+# test.  Bar: the GPU fit's likelihood, evaluated by the oracle, is within
+# 1e-7 (relative, floor 1) of the oracle's optimum; the reported loglik is
+# that likelihood less the range penalty (1e-10); parameters agree to 1e-4
+# relative (nugget: 1e-6 of the sample variance).
+def _check_fit(oracle, c, v, o, nb, cnt, kind, fit):
+    rc, ofit, oll, _ = oracle.vecchia_mle_plan(c, v, kind[0], kind[1], o, nb, cnt)
+    assert rc == 0
+    s = fit.model.structures[0]
+    assert s.kind == kind and s.anis is None
+    rc, at = oracle.vecchia_loglik_plan(c, v, fit.model, o, nb, cnt)
+    assert rc == 0
+    # the reported loglik is minus the objective, range penalty included
+    # (vecchia.rs:988-1016, 1074-1077)
+    rmax = 10.0 * float(np.sqrt(((c.max(0) - c.min(0)) ** 2).sum()))
+    if s.range > rmax:
+        at -= 1e6 * ((s.range - rmax) / rmax) ** 2
+    assert abs(fit.loglik - at) <= 1e-10 * max(1.0, abs(at)), (fit.loglik, at)
+    assert abs(at - oll) <= 1e-7 * max(1.0, abs(oll)), (at, oll, fit, ofit)
+    var0 = float(np.mean((v - v.mean()) ** 2))
+    assert abs(fit.model.nugget - ofit[0]) <= 1e-6 * var0, (fit.model.nugget, ofit)
+    assert abs(s.sill - ofit[1]) <= 1e-4 * ofit[1], (s.sill, ofit)
+    assert abs(s.range - ofit[2]) <= 1e-4 * ofit[2], (s.range, ofit)
+
+
+@pytest.mark.parametrize("kind", [gh.ModelKind.EXPONENTIAL, gh.ModelKind.SPHERICAL,
+                                  gh.ModelKind.GAUSSIAN, gh.ModelKind.MATERN15,
+                                  gh.ModelKind.MATERN25, gh.ModelKind.CIRCULAR,
+                                  gh.ModelKind.Stable(1.3), gh.ModelKind.HOLE],
+                         ids=["exp", "sph", "gau", "m15", "m25", "cir", "stable", "hole"])
+def test_mle_matches_oracle(oracle, kind):
+    c, v = cs.vecchia_field(oracle, 300, 3)
+    p, o, nb, cnt = plan_equal(oracle, c, 10)
+    _check_fit(oracle, c, v, o, nb, cnt, kind, p.mle(v, kind))
+
+
+def test_mle_full_conditioning_and_3d(oracle):
+    # vecchia.rs:2321-2351 workload (field(28, 5), full conditioning)
+    c, v = cs.vecchia_field(oracle, 28, 5)
+    p, o, nb, cnt = plan_equal(oracle, c, 27)
+    _check_fit(oracle, c, v, o, nb, cnt, gh.ModelKind.EXPONENTIAL,
+               p.mle(v, gh.ModelKind.EXPONENTIAL))
+    r = oracle.Rng(64)
+    c3 = np.array([[r.uniform() * 100.0, r.uniform() * 100.0, r.uniform() * 30.0]
+                   for _ in range(600)])
+    v3 = (np.sin(c3[:, 0] / 25.0) + np.cos(c3[:, 1] / 30.0) + c3[:, 2] / 20.0
+          + 0.1 * np.array([r.normal() for _ in range(600)]))
+    p3, o, nb, cnt = plan_equal(oracle, c3, 16)
+    for kind in (gh.ModelKind.MATERN25, gh.ModelKind.Stable(0.9)):
+        _check_fit(oracle, c3, v3, o, nb, cnt, kind, p3.mle(v3, kind))
+
+
+def test_mle_beats_a_wrong_model_and_rejections(oracle):
+    # vecchia.rs:2354-2363, 1986
+    c, v = cs.vecchia_field(oracle, 60, 9)
+    fit = gh.vecchia_mle(c, v, gh.ModelKind.EXPONENTIAL, 12)
+    wrong = gh.VariogramModel(0.5, [gh.Structure(gh.ModelKind.EXPONENTIAL, 0.1, 5.0)])
+    assert fit.loglik > gh.vecchia_loglik(c, v, wrong, 15)
+    with pytest.raises(gh.InvalidParameter):
+        gh.vecchia_mle(c, v, gh.ModelKind.Power(1.0), 5)
+    p = gh.vecchia_plan(np.column_stack([c, np.arange(60.0)]), 5)
+    with pytest.raises(gh.InvalidParameter):
+        p.mle(v, gh.ModelKind.CIRCULAR)
+    # the gradient kernels hold conditioning sets of at most 32 points
+    with pytest.raises(gh.InvalidParameter):
+        gh.vecchia_mle(c, v, gh.ModelKind.EXPONENTIAL, 40)
+    # the Nelder-Mead kinds have no such limit
+    p40, o, nb, cnt = plan_equal(oracle, c, 40)
+    _check_fit(oracle, c, v, o, nb, cnt, gh.ModelKind.CIRCULAR, p40.mle(v, gh.ModelKind.CIRCULAR))

@@ -550,3 +550,92 @@ def test_vecchia_gradient_rejects_kinds_without_derivative(oracle):
                  ModelKind.Matern(0.8)):
         rc, _, _ = oracle.vecchia_loglik_grad_plan(c, v, kind[0], 0.1, 0.9, 20.0, o, nb, cnt)
         assert rc == 4     # GeostatError::InvalidParameter (the reference returns None)
+
+
+# ---- Vecchia MLE (vecchia.rs:921-1081, optim.rs:6-288) ---------------------
+def test_mle_full_conditioning_is_exact_optimum(oracle):
+    # vecchia.rs:2321-2351: with full conditioning the fit is a local optimum
+    # of the exact (dense MVN, scipy-pinned) log-likelihood, and its loglik
+    # is that exact log-likelihood
+    c, v = cs.vecchia_field(oracle, 28, 5)
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 27)
+    rc, fit, ll, _ = oracle.vecchia_mle_plan(c, v, ModelKind.EXPONENTIAL[0], 0.0, o, nb, cnt)
+    assert rc == 0 and fit[0] >= 0 and fit[1] > 0 and fit[2] > 0
+    nug, ps, rg = fit
+    at = VariogramModel(nug, [Structure(ModelKind.EXPONENTIAL, ps, rg)])
+    assert abs(oracle.dense_mvn_loglik(c, v, at)[1] - ll) < 1e-9 * abs(ll)
+    for fn_, fp, fr in [(1.0, 0.7, 1.0), (1.0, 1.4, 1.0), (1.0, 1.0, 0.7), (1.0, 1.0, 1.4),
+                        (3.0, 1.0, 1.0), (1.0, 0.999, 1.0), (1.0, 1.0, 1.001)]:
+        pert = VariogramModel(max(nug * fn_, 1e-9), [Structure(ModelKind.EXPONENTIAL, ps * fp,
+                                                               rg * fr)])
+        assert ll >= oracle.dense_mvn_loglik(c, v, pert)[1] - 1e-6, (fn_, fp, fr)
+
+
+def test_mle_beats_a_wrong_model(oracle):
+    # vecchia.rs:2354-2363
+    c, v = cs.vecchia_field(oracle, 60, 9)
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 12)
+    rc, fit, ll, _ = oracle.vecchia_mle_plan(c, v, ModelKind.EXPONENTIAL[0], 0.0, o, nb, cnt)
+    assert rc == 0
+    wrong = VariogramModel(0.5, [Structure(ModelKind.EXPONENTIAL, 0.1, 5.0)])
+    _, o15, nb15, cnt15 = oracle.vecchia_plan(c, 15)
+    assert ll > oracle.vecchia_loglik_plan(c, v, wrong, o15, nb15, cnt15)[1]
+
+
+def test_mle_errors_when_every_start_is_singular(oracle):
+    # vecchia.rs:2366-2389: two conditioning points at one location make
+    # every K_SS singular; the fit is SingularSystem, not a start point
+    c = np.array([[0.0, 0.0], [5.0, 0.0], [5.0, 0.0], [10.0, 0.0]])
+    v = np.array([0.1, 0.2, -0.3, 0.4])
+    o = np.arange(4, dtype=np.int64)
+    nb = np.array([[-1, -1], [0, -1], [0, 1], [1, 2]], dtype=np.int32)
+    cnt = np.array([0, 1, 2, 2], dtype=np.int32)
+    for kind in (ModelKind.SPHERICAL, ModelKind.CIRCULAR):   # BFGS and Nelder-Mead
+        rc, _, _, _ = oracle.vecchia_mle_plan(c, v, kind[0], 0.0, o, nb, cnt)
+        assert rc == 3
+
+
+def test_mle_rejects_power_and_circular_3d(oracle):
+    # vecchia.rs:1986 and reject_invalid_dim_kind (vecchia.rs:82-91)
+    c, v = cs.vecchia_field(oracle, 20, 2)
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 5)
+    assert oracle.vecchia_mle_plan(c, v, ModelKind.Power(1.0)[0], 1.0, o, nb, cnt)[0] == 4
+    c3 = np.column_stack([c, np.arange(20.0)])
+    rc, o, nb, cnt = oracle.vecchia_plan(c3, 5)
+    assert oracle.vecchia_mle_plan(c3, v, ModelKind.CIRCULAR[0], 0.0, o, nb, cnt)[0] == 4
+
+
+def _objective(oracle, c, v, kind, o, nb, cnt, x):
+    """-ll of the MLE parametrization (sqrt nugget, ln psill, ln range)."""
+    m = VariogramModel(x[0] ** 2, [Structure(kind, math.exp(x[1]), math.exp(x[2]))])
+    rc, ll = oracle.vecchia_loglik_plan(c, v, m, o, nb, cnt)
+    return -ll if rc == 0 else 1e12
+
+
+@pytest.mark.parametrize("kind", [ModelKind.MATERN25, ModelKind.SPHERICAL,
+                                  ModelKind.Stable(1.3)])
+def test_mle_reaches_the_optimum_an_independent_optimizer_finds(oracle, kind):
+    # BFGS (g_and_dg kinds) and Nelder-Mead (others) against scipy's
+    # Powell search started from the oracle's own fit and from the middle
+    # start: no lower objective exists near the fit, and the stationarity
+    # of the gradient-kind fits is checked with the (finite-difference
+    # pinned) analytic gradient
+    from scipy.optimize import minimize
+    c, v = cs.vecchia_field(oracle, 200, 3)
+    rc, o, nb, cnt = oracle.vecchia_plan(c, 10)
+    rc, fit, ll, ev = oracle.vecchia_mle_plan(c, v, kind[0], kind[1], o, nb, cnt)
+    assert rc == 0 and ev > 0
+    x = np.array([math.sqrt(fit[0]), math.log(fit[1]), math.log(fit[2])])
+    f = lambda y: _objective(oracle, c, v, kind, o, nb, cnt, y)
+    assert abs(f(x) + ll) < 1e-9
+    res = minimize(f, x, method="Powell", options={"xtol": 1e-10, "ftol": 1e-14})
+    assert res.fun >= -ll - 1e-6
+    var0 = np.mean((v - v.mean()) ** 2)
+    ext = math.hypot(*(c.max(0) - c.min(0)))
+    x1 = [math.sqrt(0.1 * var0), math.log(0.9 * var0), math.log(ext / 3.0)]
+    res = minimize(f, x1, method="Powell", options={"xtol": 1e-10, "ftol": 1e-14})
+    assert res.fun >= -ll - 1e-6
+    if kind[0] in (ModelKind.MATERN25[0], ModelKind.SPHERICAL[0]):
+        rc, _, g = oracle.vecchia_loglik_grad_plan(c, v, kind[0], *fit, o, nb, cnt)
+        gx = [-g[0] * 2 * x[0], -g[1] * fit[1], -g[2] * fit[2]]
+        assert max(abs(t) for t in gx) < 1e-4, gx
