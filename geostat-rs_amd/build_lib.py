@@ -40,9 +40,14 @@ EXACT = ["-ffp-contract=off"]     # search + host: no a*b+c fusion (bit-exact di
 # The lean model class (spherical / exponential / Gaussian / closed-form
 # Matern) gets both group sizes; the general class only the first.
 GROUPS = {16: (2, 4), 32: (8, 16), 64: (32, 64)}
-KRIGE = [(d, nb, m, g, lean) for d, ms in ((2, (0, 1, 3, 6)), (3, (0, 1, 4, 10)))
+KRIGE = [(d, nb, m, g, lean, 0) for d, ms in ((2, (0, 1, 3, 6)), (3, (0, 1, 4, 10)))
          for nb in (16, 32, 64) for m in ms for lean in (1, 0)
          for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
+# Extended systems (measurement error, external drift with 1-3 covariates,
+# block kriging): first group size only.
+KRIGE += [(d, nb, m, GROUPS[nb][0], lean, 1)
+          for d, ms in ((2, (0, 1, 2, 3, 4, 6)), (3, (0, 1, 2, 3, 4, 10)))
+          for nb in (16, 32, 64) for m in ms for lean in (1, 0)]
 VECCHIA = [(d, nb, g, lean) for d in (2, 3) for nb in (16, 32, 64) for lean in (1, 0)
            for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
@@ -53,11 +58,12 @@ VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]
 def _gen_sources() -> list[Path]:
     GEN.mkdir(parents=True, exist_ok=True)
     out = []
-    for d, nb, m, g, lean in KRIGE:
-        p = GEN / f"krige_d{d}_nb{nb}_m{m}_g{g}_{'lean' if lean else 'gen'}.hip"
+    for d, nb, m, g, lean, x in KRIGE:
+        p = GEN / (f"krige_d{d}_nb{nb}_m{m}_g{g}_{'lean' if lean else 'gen'}"
+                   f"{'_x' if x else ''}.hip")
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
                f"namespace gs {{\ntemplate hipError_t launch_krige_inst<{d}, {nb}, {m}, {g}, "
-               f"{'true' if lean else 'false'}>"
+               f"{'true' if lean else 'false'}, {'true' if x else 'false'}>"
                f"(const SolveParams &, int64_t, hipStream_t);\n}}\n")
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)

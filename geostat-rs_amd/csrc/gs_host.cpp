@@ -412,6 +412,12 @@ struct gs_kriging {
     double r2 = std::numeric_limits<double>::infinity();
     gs::SearchXf xf{};
     DBuf coords, values, tcoords, tidx, dmodel;
+    // extended systems: measurement-error variances, normalized external
+    // covariates (nv of them) and block-kriging workspace
+    bool has_err = false;
+    int nv = 0;
+    double xmean[3] = {0, 0, 0}, xscale[3] = {1, 1, 1};
+    DBuf err, xdrift, h_tdrift, blk;
     // cell grid of the search-frame coordinates (tile search)
     DBuf cg_start, cg_pts, cg_idx;
     gs::CellGridView cg{};
@@ -581,27 +587,39 @@ int32_t gs_ctx_synchronize(gs_ctx *ctx) {
     return GS_OK;
 }
 
-int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
-                       int64_t n, const gs_model *model, const gs_kconfig *config, gs_kriging **out,
-                       int64_t dup_pair[2]) {
+}  // extern "C"
+
+namespace {
+
+// PointSet::new checks (data.rs:57-77) and the model's own validation, which
+// the reference performs before any predictor constructor runs.
+int32_t check_pointset(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                       int64_t n, const gs_model *model, const gs_kconfig *config,
+                       gs_kriging **out) {
     if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
     *out = nullptr;
     if (!ctx || !config) return fail(GS_ERR_INVALID_PARAMETER, "ctx/config is NULL");
     if (dim != 2 && dim != 3) return fail(GS_ERR_DIMENSION_MISMATCH, "dim must be 2 or 3");
-    // PointSet::new (data.rs:57-77)
     if (n <= 0) return fail(GS_ERR_INSUFFICIENT_DATA, "no points provided");
     if (!coords || !values) return fail(GS_ERR_INVALID_PARAMETER, "coords/values is NULL");
     for (int64_t i = 0; i < n * dim; i++)
         if (!std::isfinite(coords[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite coordinate");
     for (int64_t i = 0; i < n; i++)
         if (!std::isfinite(values[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite value");
-    int32_t st = validate_model(model, dim, true);
-    if (st != GS_OK) return st;
+    return validate_model(model, dim, true);
+}
+
+// Kriging::build (kriging.rs:262-408).  errors: measurement-error variances
+// (n) or NULL; drift: n x nv external covariates or NULL.
+int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                      int64_t n, const gs_model *model, const gs_kconfig *config,
+                      const double *errors, int32_t nv, const double *drift, gs_kriging **out,
+                      int64_t dup_pair[2]) {
     const gs_kconfig &cfg = *config;
     // Kriging::build (kriging.rs:262-339), in the reference's check order.
     if (cfg.method == GS_UNIVERSAL && !(cfg.degree == 1 || cfg.degree == 2))
         return fail(GS_ERR_INVALID_PARAMETER, "universal kriging drift degree must be 1 or 2");
-    if (cfg.method < GS_SIMPLE || cfg.method > GS_UNIVERSAL)
+    if (cfg.method < GS_SIMPLE || cfg.method > GS_EXTERNAL_DRIFT)
         return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: kriging method");
     bool power = false;
     for (int i = 0; i < model->n_structures; i++) power |= model->structures[i].kind == GS_POWER;
@@ -637,6 +655,7 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
     K->model = *model;
     K->M = cfg.method == GS_SIMPLE ? 0
          : cfg.method == GS_ORDINARY ? 1
+         : cfg.method == GS_EXTERNAL_DRIFT ? 1 + nv
          : (cfg.degree == 1 ? 1 + dim : 1 + dim + dim * (dim + 1) / 2);   // kriging.rs:50-59
     K->min_nb = cfg.min_neighbors == GS_NONE ? 1 : (int32_t)cfg.min_neighbors;
     // centre / scale for the drift basis (kriging.rs:340-347)
@@ -653,6 +672,29 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
         spread = std::fmax(spread, mx[d] - mn[d]);
     }
     K->scale = std::fmax(0.5 * spread, std::numeric_limits<double>::min());
+    // per-covariate normalization of the external drift (kriging.rs:349-364)
+    std::vector<double> xd;
+    if (nv > 0) {
+        K->nv = nv;
+        const double nn = (double)n;
+        for (int k = 0; k < nv; k++) {
+            double sum = 0.0;
+            for (int64_t i = 0; i < n; i++) sum += drift[i * nv + k];
+            const double mean = sum / nn;
+            double ss = 0.0;
+            for (int64_t i = 0; i < n; i++) {
+                const double d = drift[i * nv + k] - mean;
+                ss += d * d;
+            }
+            K->xmean[k] = mean;
+            K->xscale[k] = std::fmax(std::sqrt(ss / nn), std::numeric_limits<double>::min());
+        }
+        // basis values at the data, as fill_basis computes them (kriging.rs:431-437)
+        xd.resize((size_t)(n * nv));
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < nv; k++)
+                xd[(size_t)(i * nv + k)] = (drift[i * nv + k] - K->xmean[k]) / K->xscale[k];
+    }
     K->has_tree = cfg.max_neighbors != GS_NONE || cfg.has_search_radius || cfg.max_per_octant != GS_NONE;
     // Neighbour-list capacity (stride) and padded system bound (nb_bound).
     // Systems hold at most 64 neighbours on the GPU.  When the configuration
@@ -686,6 +728,15 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
     HIP_TRY(K->values.ensure(sizeof(double) * n));
     HIP_TRY(hipMemcpy(K->coords.p, coords, sizeof(double) * n * dim, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(K->values.p, values, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (errors) {
+        K->has_err = true;
+        HIP_TRY(K->err.ensure(sizeof(double) * n));
+        HIP_TRY(hipMemcpy(K->err.p, errors, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    if (nv > 0) {
+        HIP_TRY(K->xdrift.ensure(sizeof(double) * n * nv));
+        HIP_TRY(hipMemcpy(K->xdrift.p, xd.data(), sizeof(double) * n * nv, hipMemcpyHostToDevice));
+    }
     DevModel dm = resolve_model(*model, dim, ctx->nodes.as<gs::BesselNodes>());
     K->lean = gs::model_is_lean(dm);
     HIP_TRY(K->dmodel.ensure(sizeof(DevModel)));
@@ -710,6 +761,71 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
     }
     *out = K.release();
     return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                       int64_t n, const gs_model *model, const gs_kconfig *config, gs_kriging **out,
+                       int64_t dup_pair[2]) {
+    int32_t st = check_pointset(ctx, dim, coords, values, n, model, config, out);
+    if (st != GS_OK) return st;
+    if (config->method == GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "unsupported on GPU: external drift needs gs_kriging_new_with_external_drift");
+    return kriging_build(ctx, dim, coords, values, n, model, config, nullptr, 0, nullptr, out,
+                         dup_pair);
+}
+
+int32_t gs_kriging_new_with_measurement_error(gs_ctx *ctx, int32_t dim, const double *coords,
+                                              const double *values, int64_t n,
+                                              const gs_model *model, const gs_kconfig *config,
+                                              const double *errors, gs_kriging **out,
+                                              int64_t dup_pair[2]) {
+    int32_t st = check_pointset(ctx, dim, coords, values, n, model, config, out);
+    if (st != GS_OK) return st;
+    // with_measurement_error (kriging.rs:181-218), in the reference's order
+    if (config->method == GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "measurement error with external drift is not supported yet");
+    for (int i = 0; i < model->n_structures; i++)
+        if (model->structures[i].kind == GS_POWER)
+            return fail(GS_ERR_INVALID_PARAMETER,
+                        "measurement error with Power (IRF-0) models is not supported yet");
+    if (!errors) return fail(GS_ERR_DIMENSION_MISMATCH, "no error variances for the data points");
+    for (int64_t i = 0; i < n; i++)
+        if (!(std::isfinite(errors[i]) && errors[i] >= 0.0))
+            return fail(GS_ERR_INVALID_PARAMETER,
+                        "measurement-error variances must be finite and non-negative");
+    return kriging_build(ctx, dim, coords, values, n, model, config, errors, 0, nullptr, out,
+                         dup_pair);
+}
+
+int32_t gs_kriging_new_with_external_drift(gs_ctx *ctx, int32_t dim, const double *coords,
+                                           const double *values, int64_t n,
+                                           const gs_model *model, const gs_kconfig *config,
+                                           int32_t n_vars, const double *drift_data,
+                                           gs_kriging **out, int64_t dup_pair[2]) {
+    int32_t st = check_pointset(ctx, dim, coords, values, n, model, config, out);
+    if (st != GS_OK) return st;
+    // with_external_drift (kriging.rs:223-260), in the reference's order
+    if (config->method != GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "with_external_drift requires KrigingMethod::ExternalDrift");
+    if (n_vars <= 0)
+        return fail(GS_ERR_INVALID_PARAMETER, "external drift requires at least one covariate");
+    if (!drift_data) return fail(GS_ERR_DIMENSION_MISMATCH, "no drift rows for the data points");
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < n_vars; k++)
+            if (!std::isfinite(drift_data[i * n_vars + k]))
+                return fail(GS_ERR_INVALID_PARAMETER,
+                            "non-finite drift value at data point " + std::to_string(i));
+    if (n_vars > 3)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: more than 3 external covariates");
+    return kriging_build(ctx, dim, coords, values, n, model, config, nullptr, n_vars, drift_data,
+                         out, dup_pair);
 }
 
 void gs_kriging_free(gs_kriging *k) {
@@ -821,13 +937,23 @@ int32_t run_neighbors(gs_kriging *K, const double *d_targets, int64_t T, hipStre
     return GS_OK;
 }
 
+// Per-call inputs of the extended systems: target covariates (external
+// drift) and the block discretization with its device-side within-block
+// covariance.
+struct Extra {
+    const double *tdrift = nullptr;
+    const double *offsets = nullptr;
+    int32_t n_off = 0;
+    const double *cbb = nullptr;
+};
+
 int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
                    double *d_value, double *d_var, double *d_lag, int32_t *d_status,
-                   hipStream_t s);
+                   hipStream_t s, const Extra &x);
 
 int32_t run_predict(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
                     double *d_value, double *d_var, double *d_lag, int32_t *d_status,
-                    hipStream_t s) {
+                    hipStream_t s, const Extra &x = Extra()) {
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (T == 0) return GS_OK;
@@ -839,7 +965,7 @@ int32_t run_predict(gs_kriging *K, const double *d_targets, int64_t T, int32_t p
     int32_t st = run_neighbors(K, d_targets, T, s);
     if (st != GS_OK) return st;
     if (K->timing) HIP_TRY(hipEventRecord(ev[1], s));
-    st = run_solves(K, d_targets, T, precision, d_value, d_var, d_lag, d_status, s);
+    st = run_solves(K, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x);
     if (st != GS_OK) return st;
     if (K->timing) {
         HIP_TRY(hipEventRecord(ev[2], s));
@@ -850,7 +976,7 @@ int32_t run_predict(gs_kriging *K, const double *d_targets, int64_t T, int32_t p
 
 int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
                    double *d_value, double *d_var, double *d_lag, int32_t *d_status,
-                   hipStream_t s) {
+                   hipStream_t s, const Extra &x) {
     gs::SolveParams p;
     std::memset(&p, 0, sizeof(p));
     p.coords = K->coords.as<double>();
@@ -871,13 +997,24 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
     p.variance = d_var;
     p.lagrange = d_lag;
     p.status = d_status;
+    p.err = K->has_err ? K->err.as<double>() : nullptr;
+    p.xdrift = K->nv > 0 ? K->xdrift.as<double>() : nullptr;
+    p.tdrift = x.tdrift;
+    for (int k = 0; k < K->nv; k++) {
+        p.xmean[k] = K->xmean[k];
+        p.xscale[k] = K->xscale[k];
+    }
+    p.offsets = x.offsets;
+    p.n_off = x.n_off;
+    p.cbb = x.cbb;
+    const bool ext = p.err || p.xdrift || p.offsets;
     if (precision != GS_PREC_FP64)
         return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
     int device = K->ctx->device;
     auto solve = [&](int NB, const gs::SolveParams &pp, int64_t groups) -> int32_t {
         std::string key = "krige:" + std::to_string(device) + ":" + std::to_string(K->D) + ":" +
                           std::to_string(NB) + ":" + std::to_string(K->M) + ":" +
-                          model_signature(K->model) + (pp.list ? "list" : "");
+                          model_signature(K->model) + (pp.list ? "list" : "") + (ext ? "x" : "");
         // the tuner times a bounded prefix of the systems (their outputs
         // are rewritten by the real launch below)
         constexpr int32_t kTuneSystems = 131072;
@@ -892,11 +1029,11 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
             } else {
                 tp.T = std::min<int64_t>(pp.T, kTuneSystems);
             }
-            return gs::launch_krige_fp64(K->D, NB, K->M, v, K->lean, tp,
+            return gs::launch_krige_fp64(K->D, NB, K->M, v, K->lean, ext, tp,
                                          std::min<int64_t>(groups, kTuneSystems), s);
         }, &variant);
         if (st != GS_OK) return st;
-        hipError_t e = gs::launch_krige_fp64(K->D, NB, K->M, variant, K->lean, pp, groups, s);
+        hipError_t e = gs::launch_krige_fp64(K->D, NB, K->M, variant, K->lean, ext, pp, groups, s);
         if (e == hipErrorInvalidValue)
             return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size");
         HIP_TRY(e);
@@ -947,17 +1084,42 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
     std::lock_guard<std::mutex> lk(k->mu);
     HIP_TRY(hipSetDevice(k->ctx->device));
     hipStream_t s = stream ? (hipStream_t)stream : k->ctx->stream;
+    if (k->nv > 0) {
+        // predict on an external-drift predictor fails per target
+        // (kriging.rs:494-501) and predict_many maps that to NaN
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        HIP_TRY(gs::launch_fill_f64(d_value, T, nan, s));
+        HIP_TRY(gs::launch_fill_f64(d_variance, T, nan, s));
+        if (d_lagrange) HIP_TRY(gs::launch_fill_f64(d_lagrange, T, nan, s));
+        if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INVALID_PARAMETER, s));
+        return GS_OK;
+    }
     return run_predict(k, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status, s);
 }
 
-int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T, int32_t precision,
-                                double *value, double *variance, double *lagrange, int32_t *status) {
-    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
-    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
-    if (T == 0) return GS_OK;
-    if (!targets || !value || !variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
-    std::lock_guard<std::mutex> lk(k->mu);
-    HIP_TRY(hipSetDevice(k->ctx->device));
+}  // extern "C"
+
+namespace {
+
+// Every target failed with `code` (a per-target error the reference's
+// predict_many / predict_block_grid map to NaN).
+int32_t fail_all_host(int64_t T, double *value, double *variance, double *lagrange,
+                      int32_t *status, int32_t code) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (int64_t t = 0; t < T; t++) {
+        value[t] = nan;
+        variance[t] = nan;
+        if (lagrange) lagrange[t] = nan;
+        if (status) status[t] = code;
+    }
+    return GS_OK;
+}
+
+// Host-buffer predict: uploads the targets (and the per-target covariates
+// when tdrift is given), runs the pass, downloads the estimates.
+int32_t predict_host(gs_kriging *k, const double *targets, int64_t T, int32_t precision,
+                     double *value, double *variance, double *lagrange, int32_t *status,
+                     const double *tdrift, Extra x) {
     hipStream_t s = k->ctx->stream;
     HIP_TRY(k->h_targets.ensure(sizeof(double) * T * k->D));
     HIP_TRY(k->h_value.ensure(sizeof(double) * T));
@@ -965,8 +1127,15 @@ int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T,
     HIP_TRY(k->h_lag.ensure(sizeof(double) * T));
     HIP_TRY(k->h_status.ensure(sizeof(int32_t) * T));
     HIP_TRY(hipMemcpyAsync(k->h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
+    if (tdrift) {
+        HIP_TRY(k->h_tdrift.ensure(sizeof(double) * T * k->nv));
+        HIP_TRY(hipMemcpyAsync(k->h_tdrift.p, tdrift, sizeof(double) * T * k->nv,
+                               hipMemcpyHostToDevice, s));
+        x.tdrift = k->h_tdrift.as<double>();
+    }
     int32_t st = run_predict(k, k->h_targets.as<double>(), T, precision, k->h_value.as<double>(),
-                             k->h_var.as<double>(), k->h_lag.as<double>(), k->h_status.as<int32_t>(), s);
+                             k->h_var.as<double>(), k->h_lag.as<double>(), k->h_status.as<int32_t>(),
+                             s, x);
     if (st != GS_OK) return st;
     HIP_TRY(hipMemcpyAsync(value, k->h_value.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(variance, k->h_var.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
@@ -974,6 +1143,78 @@ int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T,
     if (status) HIP_TRY(hipMemcpyAsync(status, k->h_status.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T, int32_t precision,
+                                double *value, double *variance, double *lagrange, int32_t *status) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T == 0) return GS_OK;
+    if (!targets || !value || !variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    if (k->nv > 0)   // predict needs predict_with_drift here (kriging.rs:494-501)
+        return fail_all_host(T, value, variance, lagrange, status, GS_ERR_INVALID_PARAMETER);
+    return predict_host(k, targets, T, precision, value, variance, lagrange, status, nullptr,
+                        Extra());
+}
+
+int32_t gs_kriging_predict_many_with_drift(gs_kriging *k, const double *targets, int64_t T,
+                                           const double *drifts, int32_t precision, double *value,
+                                           double *variance, double *lagrange, int32_t *status) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T == 0) return GS_OK;
+    if (!targets || !value || !variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    // predict_with_drift on another method fails per target (kriging.rs:507-512)
+    if (k->nv == 0) return fail_all_host(T, value, variance, lagrange, status, GS_ERR_INVALID_PARAMETER);
+    if (!drifts) return fail(GS_ERR_DIMENSION_MISMATCH, "no drift rows for the targets");
+    return predict_host(k, targets, T, precision, value, variance, lagrange, status, drifts,
+                        Extra());
+}
+
+int32_t gs_kriging_predict_block_many(gs_kriging *k, const double *centers, int64_t T,
+                                      const double *offsets, int32_t n_off, int32_t precision,
+                                      double *value, double *variance, double *lagrange,
+                                      int32_t *status) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T == 0) return GS_OK;
+    if (!centers || !value || !variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    if (n_off < 0) return fail(GS_ERR_INVALID_PARAMETER, "n_off < 0");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    // predict_block's own errors (kriging.rs:679-697) fail every target
+    if (k->cfg.method != GS_SIMPLE && k->cfg.method != GS_ORDINARY)
+        return fail_all_host(T, value, variance, lagrange, status, GS_ERR_INVALID_PARAMETER);
+    if (n_off == 0 || !offsets)
+        return fail_all_host(T, value, variance, lagrange, status, GS_ERR_INVALID_PARAMETER);
+    for (int64_t i = 0; i < (int64_t)n_off * k->D; i++)
+        if (!std::isfinite(offsets[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite block offset");
+    hipStream_t s = k->ctx->stream;
+    const size_t ob = sizeof(double) * (size_t)n_off * k->D;
+    HIP_TRY(k->blk.ensure(ob + 64));
+    double *d_offs = k->blk.as<double>();
+    double *d_cbb = reinterpret_cast<double *>(static_cast<char *>(k->blk.p) + ob);
+    HIP_TRY(hipMemcpyAsync(d_offs, offsets, ob, hipMemcpyHostToDevice, s));
+    HIP_TRY(gs::launch_block_cbb(k->D, k->lean, k->dmodel.as<DevModel>(), d_offs, n_off, d_cbb, s));
+    Extra x;
+    x.offsets = d_offs;
+    x.n_off = n_off;
+    x.cbb = d_cbb;
+    return predict_host(k, centers, T, precision, value, variance, lagrange, status, nullptr, x);
 }
 
 int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, int64_t max_out,

@@ -99,6 +99,14 @@ struct SolveParams {
     double *variance;
     double *lagrange;
     int32_t *status;
+    // Extended systems (kernels instantiated with X = true only):
+    const double *err;       // [n] measurement-error variances on the data diagonal, or null
+    const double *xdrift;    // [n][M-1] normalized external covariates at the data, or null
+    const double *tdrift;    // [T][M-1] covariates at the targets (raw)
+    double xmean[3], xscale[3];
+    const double *offsets;   // [n_off][D] block discretization, or null (point kriging)
+    int32_t n_off;
+    const double *cbb;       // device scalar: within-block covariance
 };
 
 struct VecchiaParams {
@@ -129,13 +137,19 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t s);
 // exponential, Gaussian or closed-form Matern.  Returns
 // hipErrorInvalidDeviceFunction for a (variant, class) pair that is not
 // compiled (general-class kernels exist for variant 0 only).
-hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, const SolveParams &p,
-                             int64_t grid_groups, hipStream_t s);
+// ext: the extended-system kernels (measurement error, external drift,
+// block kriging; variant 0 only).
+hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, bool ext,
+                             const SolveParams &p, int64_t grid_groups, hipStream_t s);
+// *out = within-block covariance of the n_off discretization offsets.
+hipError_t launch_block_cbb(int D, bool lean, const DevModel *model, const double *offs,
+                            int32_t n_off, double *out, hipStream_t s);
 hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
                               hipStream_t s);
 hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
                            hipStream_t s);
 hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
+hipError_t launch_fill_f64(double *p, int64_t n, double value, hipStream_t s);
 // *dst = min(*src, cap) on the device (stream-ordered)
 hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStream_t s);
 hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
@@ -151,7 +165,7 @@ hipError_t launch_vecchia_reduce(const double *partial, int64_t n_blocks, int32_
 
 // Per-instantiation launchers (explicit instantiations live in generated
 // translation units, one per template argument set, built in parallel).
-template <int D, int NB, int M, int G, bool LEAN>
+template <int D, int NB, int M, int G, bool LEAN, bool X>
 hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
 template <int D, int NB, int G, bool LEAN>
 hipError_t launch_vecchia_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);

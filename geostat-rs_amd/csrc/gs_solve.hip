@@ -70,6 +70,18 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t s) 
 
 hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) { return launch_fill_i32(p, n, 0, s); }
 
+__global__ void fill_f64_kernel(double *p, int64_t n, double value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+hipError_t launch_fill_f64(double *p, int64_t n, double value, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n,
+                       value);
+    return hipGetLastError();
+}
+
 __global__ void min_i32_kernel(const int32_t *src, int32_t cap, int32_t *dst) {
     if (threadIdx.x == 0) dst[0] = min(src[0], cap);
 }
@@ -79,44 +91,111 @@ hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStre
     return hipGetLastError();
 }
 
-template <int D, int M, int V, bool LEAN>
+template <int D, int M, int V, bool LEAN, bool X>
 static hipError_t krige_nb(int NB, const SolveParams &p, int64_t gg, hipStream_t s) {
     switch (NB) {
-    case 16: return launch_krige_inst<D, 16, M, group_size(16, V), LEAN>(p, gg, s);
-    case 32: return launch_krige_inst<D, 32, M, group_size(32, V), LEAN>(p, gg, s);
-    case 64: return launch_krige_inst<D, 64, M, group_size(64, V), LEAN>(p, gg, s);
+    case 16: return launch_krige_inst<D, 16, M, group_size(16, V), LEAN, X>(p, gg, s);
+    case 32: return launch_krige_inst<D, 32, M, group_size(32, V), LEAN, X>(p, gg, s);
+    case 64: return launch_krige_inst<D, 64, M, group_size(64, V), LEAN, X>(p, gg, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int D, int M>
+// Extended systems (X) are compiled for group-size variant 0 only.
+template <int D, int M, bool X>
 static hipError_t krige_v(int NB, int variant, bool lean, const SolveParams &p, int64_t gg,
                           hipStream_t s) {
-    if (!lean) return variant ? hipErrorInvalidDeviceFunction : krige_nb<D, M, 0, false>(NB, p, gg, s);
-    return variant ? krige_nb<D, M, 1, true>(NB, p, gg, s) : krige_nb<D, M, 0, true>(NB, p, gg, s);
+    if (!lean || X)
+        return variant ? hipErrorInvalidDeviceFunction
+                       : (lean ? krige_nb<D, M, 0, true, X>(NB, p, gg, s)
+                               : krige_nb<D, M, 0, false, X>(NB, p, gg, s));
+    return variant ? krige_nb<D, M, 1, true, X>(NB, p, gg, s) : krige_nb<D, M, 0, true, X>(NB, p, gg, s);
 }
 
-hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, const SolveParams &p,
-                             int64_t grid_groups, hipStream_t s) {
+hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, bool ext,
+                             const SolveParams &p, int64_t grid_groups, hipStream_t s) {
     if (variant < 0 || variant >= kVariants) return hipErrorInvalidValue;
+    if (ext) {
+        if (D == 2) {
+            switch (M) {
+            case 0: return krige_v<2, 0, true>(NB, variant, lean, p, grid_groups, s);
+            case 1: return krige_v<2, 1, true>(NB, variant, lean, p, grid_groups, s);
+            case 2: return krige_v<2, 2, true>(NB, variant, lean, p, grid_groups, s);
+            case 3: return krige_v<2, 3, true>(NB, variant, lean, p, grid_groups, s);
+            case 4: return krige_v<2, 4, true>(NB, variant, lean, p, grid_groups, s);
+            case 6: return krige_v<2, 6, true>(NB, variant, lean, p, grid_groups, s);
+            default: return hipErrorInvalidValue;
+            }
+        } else if (D == 3) {
+            switch (M) {
+            case 0: return krige_v<3, 0, true>(NB, variant, lean, p, grid_groups, s);
+            case 1: return krige_v<3, 1, true>(NB, variant, lean, p, grid_groups, s);
+            case 2: return krige_v<3, 2, true>(NB, variant, lean, p, grid_groups, s);
+            case 3: return krige_v<3, 3, true>(NB, variant, lean, p, grid_groups, s);
+            case 4: return krige_v<3, 4, true>(NB, variant, lean, p, grid_groups, s);
+            case 10: return krige_v<3, 10, true>(NB, variant, lean, p, grid_groups, s);
+            default: return hipErrorInvalidValue;
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     if (D == 2) {
         switch (M) {
-        case 0: return krige_v<2, 0>(NB, variant, lean, p, grid_groups, s);
-        case 1: return krige_v<2, 1>(NB, variant, lean, p, grid_groups, s);
-        case 3: return krige_v<2, 3>(NB, variant, lean, p, grid_groups, s);
-        case 6: return krige_v<2, 6>(NB, variant, lean, p, grid_groups, s);
+        case 0: return krige_v<2, 0, false>(NB, variant, lean, p, grid_groups, s);
+        case 1: return krige_v<2, 1, false>(NB, variant, lean, p, grid_groups, s);
+        case 3: return krige_v<2, 3, false>(NB, variant, lean, p, grid_groups, s);
+        case 6: return krige_v<2, 6, false>(NB, variant, lean, p, grid_groups, s);
         default: return hipErrorInvalidValue;
         }
     } else if (D == 3) {
         switch (M) {
-        case 0: return krige_v<3, 0>(NB, variant, lean, p, grid_groups, s);
-        case 1: return krige_v<3, 1>(NB, variant, lean, p, grid_groups, s);
-        case 4: return krige_v<3, 4>(NB, variant, lean, p, grid_groups, s);
-        case 10: return krige_v<3, 10>(NB, variant, lean, p, grid_groups, s);
+        case 0: return krige_v<3, 0, false>(NB, variant, lean, p, grid_groups, s);
+        case 1: return krige_v<3, 1, false>(NB, variant, lean, p, grid_groups, s);
+        case 4: return krige_v<3, 4, false>(NB, variant, lean, p, grid_groups, s);
+        case 10: return krige_v<3, 10, false>(NB, variant, lean, p, grid_groups, s);
         default: return hipErrorInvalidValue;
         }
     }
     return hipErrorInvalidValue;
+}
+
+// Within-block covariance (kriging.rs:729-743): the mean of C(u - v) over
+// all pairs of discretization points, with c0 - nugget on the diagonal
+// pairs (the nugget is a measure-zero discontinuity of the block
+// integral).  One thread, pairs in the reference's order.
+template <int D, bool LEAN>
+__global__ void block_cbb_kernel(const DevModel *model, const double *offs, int32_t n_off,
+                                 double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const DevModel &Mdl = *model;
+    const double c0c = Mdl.c0 - Mdl.nugget;
+    double cbb = 0.0;
+    for (int u = 0; u < n_off; u++)
+        for (int v = 0; v < n_off; v++) {
+            if (u == v) {
+                cbb += c0c;
+            } else {
+                double dh[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) dh[d] = offs[u * D + d] - offs[v * D + d];
+                cbb += cov_offdiag<D, LEAN>(Mdl, dh);
+            }
+        }
+    out[0] = cbb / ((double)n_off * (double)n_off);
+}
+
+hipError_t launch_block_cbb(int D, bool lean, const DevModel *model, const double *offs,
+                            int32_t n_off, double *out, hipStream_t s) {
+    if (D == 2) {
+        if (lean) hipLaunchKernelGGL((block_cbb_kernel<2, true>), dim3(1), dim3(64), 0, s, model, offs, n_off, out);
+        else hipLaunchKernelGGL((block_cbb_kernel<2, false>), dim3(1), dim3(64), 0, s, model, offs, n_off, out);
+    } else if (D == 3) {
+        if (lean) hipLaunchKernelGGL((block_cbb_kernel<3, true>), dim3(1), dim3(64), 0, s, model, offs, n_off, out);
+        else hipLaunchKernelGGL((block_cbb_kernel<3, false>), dim3(1), dim3(64), 0, s, model, offs, n_off, out);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,

@@ -342,10 +342,11 @@ constexpr int stage_slots() { return NB / 2; }
 // Returns false when a pivot falls below 1e-12 * c0 (the reference's
 // singular threshold relative to the matrix scale, linalg.rs:36-57,
 // 135-140).
-template <int NB, int G, int Q>
+template <int NB, int G, int Q, bool DADD = false>
 __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (&y)[Q][NB / G],
                                                const double c0, double *col, const int g,
-                                               double (&invd)[NB / G]) {
+                                               double (&invd)[NB / G],
+                                               const double *dadd = nullptr) {
     constexpr int R = NB / G;
     // dg[r]: the diagonal entry of the owned row of block r, updated as
     // the entry itself would be (d -= l^2 per column).
@@ -357,7 +358,8 @@ __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (
 #pragma unroll
     for (int r = 0; r < R; r++) {
         rid[r] = row_of<G>(r, g);
-        dg[r] = c0;
+        if constexpr (DADD) dg[r] = c0 + dadd[r];   // data-data diagonal c0 + err_i
+        else dg[r] = c0;
         invd[r] = 0.0;
     }
     const double thresh = 1e-12 * c0;
@@ -417,12 +419,19 @@ __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (
 // gamma(0) = 0, so a target on a data point gets c0.  Returns false when a
 // pivot falls below 1e-12 * c0 (the reference's singular threshold relative
 // to the matrix scale, linalg.rs:36-57, 135-140).
-template <int D, int NB, int G, int Q, bool LEAN>
+//
+// With X: ea[r] is added to the diagonal of the owned row of block r
+// (measurement error), and when n_off > 0 the target column is the block
+// average (1/U) sum_u C(x_i - (tg + off_u)) instead (kriging.rs:709-725).
+template <int D, int NB, int G, int Q, bool LEAN, bool X = false>
 __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const int cnt,
                                                    const double (&rowx)[NB / G][D],
                                                    const double *xy, const double *tg,
                                                    double *stage, double *col, const int lane,
-                                                   const int g_in, double (&y)[Q][NB / G]) {
+                                                   const int g_in, double (&y)[Q][NB / G],
+                                                   const double *ea = nullptr,
+                                                   const double *offs = nullptr,
+                                                   const int n_off = 0) {
     constexpr int R = NB / G;
     const int g = opaque(g_in);
     const double c0 = Mdl.c0;
@@ -479,8 +488,25 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         }
     });
 
+    if constexpr (X) {
+        if (n_off > 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                double acc = 0.0;
+                for (int u = 0; u < n_off; u++) {
+                    double dh[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) dh[d] = rowx[r][d] - (tg[d] + offs[u * D + d]);
+                    acc += cov_offdiag<D, LEAN>(Mdl, dh);
+                }
+                y[0][r] = row_of<G>(r, g) < cnt ? acc / (double)n_off : 0.0;
+            }
+        }
+    }
+
     double invd[R];
-    return factor_forward<NB, G, Q>(a, y, c0, col, g, invd);
+    if constexpr (X) return factor_forward<NB, G, Q, true>(a, y, c0, col, g, invd, ea);
+    else return factor_forward<NB, G, Q>(a, y, c0, col, g, invd);
 }
 
 // Drift block by modified Gram-Schmidt on the augmented columns
@@ -585,7 +611,9 @@ __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&
 template <int NB, int G, int M>
 constexpr int krige_waves() { return (NB == 32 && G == 16 && M <= 1) ? 3 : 1; }
 
-template <int D, int NB, int G, int M, bool LEAN>
+// X = true adds the extended systems: measurement error (kriging.rs:181-218,
+// 544), external drift (223-260, 431-437) and block kriging (678-777).
+template <int D, int NB, int G, int M, bool LEAN, bool X>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_waves<NB, G, M>()))) void krige_fp64_kernel(SolveParams P) {
     constexpr int R = NB / G;
     constexpr int Q = 2 + M;
@@ -626,10 +654,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
             for (int d = 0; d < D; d++) xy[j * D + d] = P.coords[id * D + d];
         }
         double zr[R];
+        double ea[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = row_of<G>(r, g);
             zr[r] = i < cnt ? P.values[nbl[i]] : 0.0;
+            if constexpr (X) ea[r] = (P.err && i < cnt) ? P.err[nbl[i]] : 0.0;
         }
         wave_sync();
         double rowx[R][D];
@@ -647,13 +677,32 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
             y[1][r] = live ? (M == 0 ? zr[r] - P.sk_mean : zr[r]) : 0.0;
             if constexpr (M > 0) {
                 double f[M];
-                basis<D, M>(P, rowx[r], f);
+                if constexpr (X && M > 1) {
+                    if (P.xdrift) {
+                        f[0] = 1.0;
+#pragma unroll
+                        for (int q = 1; q < M; q++)
+                            f[q] = live ? P.xdrift[(int64_t)nbl[i] * (M - 1) + (q - 1)] : 0.0;
+                    } else {
+                        basis<D, M>(P, rowx[r], f);
+                    }
+                } else {
+                    basis<D, M>(P, rowx[r], f);
+                }
 #pragma unroll
                 for (int q = 0; q < M; q++) y[2 + q][r] = live ? f[q] : 0.0;
             }
         }
-        bool ok = build_factor_solve<D, NB, G, Q, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, y);
+        bool ok;
+        if constexpr (X)
+            ok = build_factor_solve<D, NB, G, Q, LEAN, true>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf,
+                                                             lane, g, y, ea, P.offsets, P.n_off);
+        else
+            ok = build_factor_solve<D, NB, G, Q, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, y);
         wave_sync();  // the group's LDS rows are rewritten by its next system
+        // variance base: c0, or the within-block covariance for a block
+        double vbase = c0;
+        if constexpr (X) vbase = P.offsets ? *P.cbb : c0;
 
         double value, var, lag;
         if constexpr (M == 0) {
@@ -664,12 +713,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
                 bb += y[0][r] * y[0][r];
             }
             value = P.sk_mean + gsum<G>(bz);
-            var = c0 - gsum<G>(bb);
+            var = vbase - gsum<G>(bb);
             lag = __builtin_nan("");
         } else {
             double fT[M];
-            basis<D, M>(P, tg, fT);
-            value = drift_solve<R, G, M>(y, fT, c0, ok, var, lag);
+            if constexpr (X && M > 1) {
+                if (P.xdrift) {
+                    fT[0] = 1.0;
+#pragma unroll
+                    for (int q = 1; q < M; q++)
+                        fT[q] = (P.tdrift[t * (M - 1) + (q - 1)] - P.xmean[q - 1]) / P.xscale[q - 1];
+                } else {
+                    basis<D, M>(P, tg, fT);
+                }
+            } else {
+                basis<D, M>(P, tg, fT);
+            }
+            value = drift_solve<R, G, M>(y, fT, vbase, ok, var, lag);
         }
         if (g == 0) {
             if (ok) {
@@ -1088,14 +1148,14 @@ hipError_t launch_vecchia_grad_inst(const VecchiaParams &p, int64_t n_blocks, hi
 }
 
 // ---- launchers (instantiated per template argument set) -------------------
-template <int D, int NB, int M, int G, bool LEAN>
+template <int D, int NB, int M, int G, bool LEAN, bool X>
 hipError_t launch_krige_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s) {
     constexpr int GPW = kWave / G;
     int64_t blocks = (grid_groups + GPW - 1) / GPW;
     if (blocks < 1) blocks = 1;
     if (blocks > (1 << 30)) blocks = 1 << 30;
-    hipLaunchKernelGGL((krige_fp64_kernel<D, NB, G, M, LEAN>), dim3((unsigned)blocks), dim3(kWave), 0,
-                       s, p);
+    hipLaunchKernelGGL((krige_fp64_kernel<D, NB, G, M, LEAN, X>), dim3((unsigned)blocks),
+                       dim3(kWave), 0, s, p);
     return hipGetLastError();
 }
 

@@ -34,7 +34,7 @@ __all__ = [
     "InvalidParameter", "NoNeighbors", "DuplicatePoints", "DeviceError",
     "ModelKind", "Anisotropy", "Structure", "VariogramModel", "KrigingMethod",
     "KrigingConfig", "KrigingEstimate", "Context", "Kriging", "Grid2D", "Grid3D",
-    "VecchiaPlan", "VecchiaFit", "vecchia_plan", "vecchia_loglik", "vecchia_mle", "PREC_FP64",
+    "block_offsets", "VecchiaPlan", "VecchiaFit", "vecchia_plan", "vecchia_loglik", "vecchia_mle", "PREC_FP64",
     "PREC_FP32_REFINED",
 ]
 
@@ -184,7 +184,8 @@ class VariogramModel:
 # ----------------------------------------------------------------- config --
 @dataclass
 class KrigingMethod:
-    """KrigingMethod (kriging.rs:25-44): kind 0 simple, 1 ordinary, 2 universal."""
+    """KrigingMethod (kriging.rs:25-44): kind 0 simple, 1 ordinary, 2
+    universal, 3 external drift (degree holds the covariate count)."""
     kind: int = 1
     mean: float = 0.0
     degree: int = 1
@@ -200,6 +201,10 @@ class KrigingMethod:
     @staticmethod
     def Universal(degree: int):
         return KrigingMethod(2, degree=degree)
+
+    @staticmethod
+    def ExternalDrift(n_vars: int):
+        return KrigingMethod(3, degree=n_vars)
 
 
 @dataclass
@@ -281,7 +286,8 @@ class Kriging:
     """
 
     def __init__(self, coords, values, model: VariogramModel,
-                 config: Optional[KrigingConfig] = None, ctx: Optional[Context] = None):
+                 config: Optional[KrigingConfig] = None, ctx: Optional[Context] = None,
+                 _errors=None, _drift=None):
         coords = _f64(coords)
         values = _f64(values).reshape(-1)
         if coords.ndim != 2 or coords.shape[1] not in (2, 3):
@@ -293,15 +299,54 @@ class Kriging:
         self.n = coords.shape[0]
         self.model = model
         self.config = config or KrigingConfig()
+        self.n_vars = 0
         cm = model._c()
         cc = self.config._c()
         h = C.c_void_p()
         dup = np.zeros(2, dtype=np.int64)
-        st = _abi.lib().gs_kriging_new(self.ctx._h, self.dim, _abi.dptr(coords),
-                                       _abi.dptr(values), self.n, C.byref(cm), C.byref(cc),
-                                       C.byref(h), _abi.i64ptr(dup))
+        lib = _abi.lib()
+        if _errors is not None:
+            e = _f64(_errors).reshape(-1)
+            power = any(st_.kind[0] == 10 for st_ in model.structures)
+            if e.shape[0] != self.n and not (self.config.method.kind == 3 or power):
+                # (external drift and Power are rejected first, before the
+                # variances are read: kriging.rs:187-204)
+                raise DimensionMismatch(f"{e.shape[0]} error variances vs {self.n} data points")
+            st = lib.gs_kriging_new_with_measurement_error(
+                self.ctx._h, self.dim, _abi.dptr(coords), _abi.dptr(values), self.n,
+                C.byref(cm), C.byref(cc), _abi.dptr(e), C.byref(h), _abi.i64ptr(dup))
+        elif _drift is not None:
+            d = _f64(_drift)
+            if d.ndim != 2 or d.shape[0] != self.n:
+                raise DimensionMismatch(f"{d.shape[0] if d.ndim else 0} drift rows vs {self.n} "
+                                        "data points")
+            if self.config.method.kind == 3 and d.shape[1] != self.config.method.degree:
+                raise DimensionMismatch(f"drift rows have {d.shape[1]} values, expected "
+                                        f"{self.config.method.degree}")
+            self.n_vars = d.shape[1]
+            st = lib.gs_kriging_new_with_external_drift(
+                self.ctx._h, self.dim, _abi.dptr(coords), _abi.dptr(values), self.n,
+                C.byref(cm), C.byref(cc), self.n_vars, _abi.dptr(d), C.byref(h),
+                _abi.i64ptr(dup))
+        else:
+            st = lib.gs_kriging_new(self.ctx._h, self.dim, _abi.dptr(coords), _abi.dptr(values),
+                                    self.n, C.byref(cm), C.byref(cc), C.byref(h),
+                                    _abi.i64ptr(dup))
         _check(st, dup)
         self._h = h
+
+    @classmethod
+    def with_measurement_error(cls, coords, values, model: VariogramModel,
+                               config: KrigingConfig, errors, ctx: Optional[Context] = None):
+        """with_measurement_error (kriging.rs:181-218): errors[i] is the
+        error variance of datum i (data-data diagonal only)."""
+        return cls(coords, values, model, config, ctx, _errors=errors)
+
+    @classmethod
+    def with_external_drift(cls, coords, values, model: VariogramModel,
+                            config: KrigingConfig, drift_data, ctx: Optional[Context] = None):
+        """with_external_drift (kriging.rs:223-260): drift_data (n, n_vars)."""
+        return cls(coords, values, model, config, ctx, _drift=drift_data)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -326,6 +371,50 @@ class Kriging:
                                                   _abi.dptr(v), _abi.dptr(var), _abi.dptr(lag),
                                                   _abi.i32ptr(st)))
         return v, var, lag, st
+
+    def predict_many_with_drift(self, targets, drifts, precision="fp64"):
+        """predict_many_with_drift (kriging.rs:656-672): drifts (T, n_vars)."""
+        t = _f64(targets).reshape(-1, self.dim)
+        T = t.shape[0]
+        d = _f64(drifts).reshape(T, -1) if T else _f64(drifts)
+        if d.shape[0] != T:
+            raise DimensionMismatch(f"{T} targets vs {d.shape[0]} drift rows")
+        if self.n_vars and T and d.shape[1] != self.n_vars:
+            raise DimensionMismatch(f"target drift has {d.shape[1]} values, expected {self.n_vars}")
+        v, var, lag = np.empty(T), np.empty(T), np.empty(T)
+        st = np.empty(T, dtype=np.int32)
+        _check(_abi.lib().gs_kriging_predict_many_with_drift(
+            self._h, _abi.dptr(t), T, _abi.dptr(d), _PREC[precision], _abi.dptr(v),
+            _abi.dptr(var), _abi.dptr(lag), _abi.i32ptr(st)))
+        return v, var, lag, st
+
+    def predict_block_many(self, centers, offsets, precision="fp64"):
+        """predict_block (kriging.rs:678-777) at every centre: arrays (value,
+        variance, lagrange, status)."""
+        t = _f64(centers).reshape(-1, self.dim)
+        T = t.shape[0]
+        off = _f64(offsets).reshape(-1, self.dim)
+        v, var, lag = np.empty(T), np.empty(T), np.empty(T)
+        st = np.empty(T, dtype=np.int32)
+        _check(_abi.lib().gs_kriging_predict_block_many(
+            self._h, _abi.dptr(t), T, _abi.dptr(off) if off.shape[0] else None, off.shape[0],
+            _PREC[precision], _abi.dptr(v), _abi.dptr(var), _abi.dptr(lag), _abi.i32ptr(st)))
+        return v, var, lag, st
+
+    def predict_block(self, center, offsets, precision="fp64") -> KrigingEstimate:
+        """predict_block (kriging.rs:678-777): raises the per-target error."""
+        v, var, lag, st = self.predict_block_many(np.asarray(center, dtype=np.float64)
+                                                  .reshape(1, -1), offsets, precision)
+        if st[0] != 0:
+            raise _ERRORS.get(int(st[0]), GeostatError)(f"block failed with status {st[0]}")
+        lg = None if self.config.method.kind == 0 else float(lag[0])
+        return KrigingEstimate(float(v[0]), float(var[0]), lg)
+
+    def predict_block_grid(self, grid: "Grid2D", block_size, discr, precision="fp64"):
+        """predict_block_grid (kriging.rs:790-803): (values, variances)."""
+        offsets = block_offsets(block_size, discr)
+        v, var, _, _ = self.predict_block_many(grid.centers(), offsets, precision)
+        return v, var
 
     def predict(self, target, precision="fp64") -> KrigingEstimate:
         """predict (kriging.rs:494-501): raises the per-target error."""
@@ -382,6 +471,23 @@ class Kriging:
         _check(_abi.lib().gs_kriging_neighbors(self._h, _abi.dptr(t), T, k, _abi.i32ptr(idx),
                                                _abi.i32ptr(cnt)))
         return idx, cnt
+
+
+def block_offsets(size, n) -> np.ndarray:
+    """block_offsets (kriging.rs:817-838): regular n[0] x n[1] discretization
+    of a size[0] x size[1] block, x fastest."""
+    if n[0] == 0 or n[1] == 0:
+        raise InvalidParameter("block discretization needs at least 1 point per axis")
+    if not (size[0] > 0.0) or not (size[1] > 0.0):
+        raise InvalidParameter(f"block size must be positive, got {list(size)}")
+    out = np.empty((n[0] * n[1], 2))
+    k = 0
+    for iy in range(n[1]):
+        for ix in range(n[0]):
+            out[k, 0] = ((ix + 0.5) / n[0] - 0.5) * size[0]
+            out[k, 1] = ((iy + 0.5) / n[1] - 0.5) * size[1]
+            k += 1
+    return out
 
 
 # ------------------------------------------------------------------ grids --

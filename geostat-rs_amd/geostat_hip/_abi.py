@@ -78,6 +78,18 @@ SIGNATURES = {
                                                  C.c_int32, _DP, _DP, _I32P]),
     "gs_vecchia_loglik": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
                                       C.c_int64, _I64P, _DP]),
+    "gs_kriging_new_with_measurement_error": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64,
+                                                           C.POINTER(gs_model),
+                                                           C.POINTER(gs_kconfig), _DP,
+                                                           C.POINTER(_P), _I64P]),
+    "gs_kriging_new_with_external_drift": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64,
+                                                        C.POINTER(gs_model),
+                                                        C.POINTER(gs_kconfig), C.c_int32, _DP,
+                                                        C.POINTER(_P), _I64P]),
+    "gs_kriging_predict_many_with_drift": (C.c_int32, [_P, _DP, C.c_int64, _DP, C.c_int32, _DP,
+                                                        _DP, _DP, _I32P]),
+    "gs_kriging_predict_block_many": (C.c_int32, [_P, _DP, C.c_int64, _DP, C.c_int32, C.c_int32,
+                                                   _DP, _DP, _DP, _I32P]),
     "gs_vecchia_mle": (C.c_int32, [_P, _DP, C.c_int32, C.c_double, C.POINTER(gs_model), _DP]),
 }
 

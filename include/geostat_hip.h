@@ -50,7 +50,9 @@ typedef enum {
 } gs_kind;
 
 /* KrigingMethod (kriging.rs:25-44); external drift is not exposed. */
-typedef enum { GS_SIMPLE = 0, GS_ORDINARY = 1, GS_UNIVERSAL = 2 } gs_method;
+/* KrigingMethod (kriging.rs:25-44).  GS_EXTERNAL_DRIFT handles are built by
+ * gs_kriging_new_with_external_drift, which takes the covariate count. */
+typedef enum { GS_SIMPLE = 0, GS_ORDINARY = 1, GS_UNIVERSAL = 2, GS_EXTERNAL_DRIFT = 3 } gs_method;
 
 /* Arithmetic of the covariance build and solve. */
 typedef enum {
@@ -121,6 +123,26 @@ int32_t gs_kriging_new(gs_ctx *ctx, int32_t dim, const double *coords, const dou
                        gs_kriging **out, int64_t dup_pair[2]);
 void gs_kriging_free(gs_kriging *k);
 
+/* Kriging::with_measurement_error (kriging.rs:181-218): errors[i] (n
+ * doubles, finite, >= 0) is the error variance of datum i, added to the
+ * data-data diagonal only.  External drift and Power models are rejected
+ * as in the reference. */
+int32_t gs_kriging_new_with_measurement_error(gs_ctx *ctx, int32_t dim, const double *coords,
+                                              const double *values, int64_t n,
+                                              const gs_model *model, const gs_kconfig *config,
+                                              const double *errors, gs_kriging **out,
+                                              int64_t dup_pair[2]);
+
+/* Kriging::with_external_drift (kriging.rs:223-260): config->method must be
+ * GS_EXTERNAL_DRIFT; drift_data is n*n_vars doubles (row i = covariates of
+ * datum i), normalized per covariate as the reference does (349-364).  On
+ * the GPU n_vars is 1..3. */
+int32_t gs_kriging_new_with_external_drift(gs_ctx *ctx, int32_t dim, const double *coords,
+                                           const double *values, int64_t n,
+                                           const gs_model *model, const gs_kconfig *config,
+                                           int32_t n_vars, const double *drift_data,
+                                           gs_kriging **out, int64_t dup_pair[2]);
+
 /* Kriging::predict_many (kriging.rs:648-652) over host buffers: targets is
  * T*dim AoS doubles; outputs have T entries each.  lagrange is NaN where the
  * reference returns None; status holds the per-target gs_status (0 = ok;
@@ -128,6 +150,27 @@ void gs_kriging_free(gs_kriging *k);
 int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T,
                                 int32_t precision, double *value, double *variance,
                                 double *lagrange, int32_t *status);
+
+/* Kriging::predict_many_with_drift (kriging.rs:656-672): drifts is T*n_vars
+ * doubles (the covariates at each target).  On a handle without external
+ * drift every target fails with GS_ERR_INVALID_PARAMETER, as the
+ * reference's predict_with_drift does per target (507-512); predict_many
+ * on an external-drift handle likewise fails every target (494-501). */
+int32_t gs_kriging_predict_many_with_drift(gs_kriging *k, const double *targets, int64_t T,
+                                           const double *drifts, int32_t precision,
+                                           double *value, double *variance, double *lagrange,
+                                           int32_t *status);
+
+/* Kriging::predict_block (kriging.rs:678-777) at T block centres (T*dim
+ * AoS), each block discretized at centre + offsets[u] (n_off*dim AoS; the
+ * reference's block_offsets, 817-838, gives the regular grid used by
+ * predict_block_grid, 790-803).  Simple and ordinary kriging only: other
+ * methods, and n_off = 0, fail every target with GS_ERR_INVALID_PARAMETER
+ * (the per-target Err that predict_block_grid maps to NaN). */
+int32_t gs_kriging_predict_block_many(gs_kriging *k, const double *centers, int64_t T,
+                                      const double *offsets, int32_t n_off, int32_t precision,
+                                      double *value, double *variance, double *lagrange,
+                                      int32_t *status);
 
 /* Same as gs_kriging_predict_many with every buffer already in device
  * memory, enqueued on `stream` (a hipStream_t; NULL = the context stream).

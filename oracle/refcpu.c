@@ -768,6 +768,10 @@ typedef struct {
     int32_t m;
     int power;
     double c0;
+    const double *err;          /* measurement-error variances (n) or NULL */
+    int32_t nv;                 /* external covariates */
+    const double *drift;        /* n x nv raw covariates or NULL */
+    double dmean[8], dscale[8]; /* per-covariate normalization (kriging.rs:349-364) */
 } krig;
 
 static int32_t n_drift(const rc_kconfig *c, int32_t D) { /* kriging.rs:50-59 */
@@ -778,6 +782,7 @@ static int32_t n_drift(const rc_kconfig *c, int32_t D) { /* kriging.rs:50-59 */
         if (c->degree == 1) return 1 + D;
         if (c->degree == 2) return 1 + D + D * (D + 1) / 2;
         return 0;
+    case RC_EXTERNAL_DRIFT: return 1 + c->degree;   /* degree holds n_vars */
     }
     return 0;
 }
@@ -824,6 +829,7 @@ static int32_t krig_build(krig *K, int32_t D, const double *coords, const double
     K->scale = fmax(0.5 * spread, 2.2250738585072014e-308);
     K->m = n_drift(cfg, D);
     K->c0 = rc_total_sill(model);
+    K->nv = 0;
     K->has_tree = cfg->max_neighbors >= 0 || cfg->has_radius || cfg->max_per_octant >= 0;
     if (K->has_tree) {
         K->tcoords = (double *)malloc(sizeof(double) * (size_t)(n * D));
@@ -836,9 +842,13 @@ static void krig_free(krig *K) {
     if (K->has_tree) { free(K->tcoords); free(K->tree.idx); }
 }
 
-static void fill_basis(const krig *K, const double *p, double *out) { /* kriging.rs:410-439 */
+static void fill_basis(const krig *K, const double *p, const double *drift, double *out) { /* kriging.rs:410-439 */
     int D = K->D;
     switch (K->cfg.method) {
+    case RC_EXTERNAL_DRIFT:
+        out[0] = 1.0;
+        for (int k = 0; k < K->nv; k++) out[1 + k] = (drift[k] - K->dmean[k]) / K->dscale[k];
+        break;
     case RC_SIMPLE: break;
     case RC_ORDINARY: out[0] = 1.0; break;
     case RC_UNIVERSAL: {
@@ -913,9 +923,10 @@ static inline double cov_entry(const krig *K, const double *dh) {
     return K->power ? g : K->c0 - g;
 }
 
-/* build_lhs (kriging.rs:527-567) + predict_inner (569-644). */
-static int32_t krig_predict(const krig *K, const double *target, kws *w, double *val, double *var,
-                            double *lag, int *has_lag) {
+/* build_lhs (kriging.rs:527-567) + predict_inner (569-644); tdrift: the
+ * target's covariates (external drift). */
+static int32_t krig_predict(const krig *K, const double *target, const double *tdrift, kws *w,
+                            double *val, double *var, double *lag, int *has_lag) {
     int D = K->D;
     int64_t nn = krig_neighbors(K, target, w);
     int64_t min_nb = K->cfg.min_neighbors >= 0 ? K->cfg.min_neighbors : 1;
@@ -935,13 +946,13 @@ static int32_t krig_predict(const krig *K, const double *target, kws *w, double 
         for (int d = 0; d < D; d++) dh[d] = pi[d] - target[d];
         b[ii] = cov_entry(K, dh);
     }
-    fill_basis(K, target, f);
+    fill_basis(K, target, tdrift, f);
     for (int64_t q = 0; q < m; q++) b[nn + q] = f[q];
     memcpy(w->b0, b, sizeof(double) * (size_t)dim);
     memset(A, 0, sizeof(double) * (size_t)(dim * dim));
     for (int64_t ii = 0; ii < nn; ii++) {
         const double *pi = K->coords + w->nb[ii] * D;
-        A[ii * dim + ii] = cov_entry(K, zero) + 0.0;
+        A[ii * dim + ii] = cov_entry(K, zero) + (K->err ? K->err[w->nb[ii]] : 0.0);
         for (int64_t jj = ii + 1; jj < nn; jj++) {
             const double *pj = K->coords + w->nb[jj] * D;
             double dh[3];
@@ -950,7 +961,7 @@ static int32_t krig_predict(const krig *K, const double *target, kws *w, double 
             A[ii * dim + jj] = c;
             A[jj * dim + ii] = c;
         }
-        fill_basis(K, pi, f);
+        fill_basis(K, pi, K->drift ? K->drift + w->nb[ii] * K->nv : NULL, f);
         for (int64_t q = 0; q < m; q++) {
             A[ii * dim + nn + q] = f[q];
             A[(nn + q) * dim + ii] = f[q];
@@ -983,10 +994,96 @@ static int32_t krig_predict(const krig *K, const double *target, kws *w, double 
     return RC_OK;
 }
 
+/* predict_block (kriging.rs:678-777): block average of centre + offsets[u]. */
+static int32_t krig_predict_block(const krig *K, const double *center, const double *offs,
+                                  int64_t n_off, kws *w, double *val, double *var, double *lag,
+                                  int *has_lag) {
+    int D = K->D;
+    if (!(K->cfg.method == RC_SIMPLE || K->cfg.method == RC_ORDINARY)) return RC_INVALID_PARAMETER;
+    if (K->power) return RC_INVALID_PARAMETER;
+    if (n_off == 0) return RC_INVALID_PARAMETER;
+    int64_t nn = krig_neighbors(K, center, w);
+    int64_t min_nb = K->cfg.min_neighbors >= 0 ? K->cfg.min_neighbors : 1;
+    if (nn < min_nb) return RC_NO_NEIGHBORS;
+    int64_t m = K->m;
+    int64_t dim = nn + m;
+    double c0 = K->c0;
+    double nu = (double)n_off;
+    w->A = (double *)realloc(w->A, sizeof(double) * (size_t)(dim * dim));
+    w->b = (double *)realloc(w->b, sizeof(double) * (size_t)dim);
+    w->b0 = (double *)realloc(w->b0, sizeof(double) * (size_t)dim);
+    double *A = w->A, *b = w->b;
+    memset(A, 0, sizeof(double) * (size_t)(dim * dim));
+    for (int64_t ii = 0; ii < nn; ii++) {
+        const double *pi = K->coords + w->nb[ii] * D;
+        A[ii * dim + ii] = c0 + (K->err ? K->err[w->nb[ii]] : 0.0);
+        for (int64_t jj = ii + 1; jj < nn; jj++) {
+            const double *pj = K->coords + w->nb[jj] * D;
+            double dh[3];
+            for (int d = 0; d < D; d++) dh[d] = pi[d] - pj[d];
+            double c = c0 - rc_gamma_dh(K->model, D, dh);
+            A[ii * dim + jj] = c;
+            A[jj * dim + ii] = c;
+        }
+        if (m == 1) {
+            A[ii * dim + nn] = 1.0;
+            A[nn * dim + ii] = 1.0;
+        }
+        double acc = -0.0;
+        for (int64_t u = 0; u < n_off; u++) {
+            double dh[3];
+            for (int d = 0; d < D; d++) dh[d] = pi[d] - (center[d] + offs[u * D + d]);
+            acc += c0 - rc_gamma_dh(K->model, D, dh);
+        }
+        b[ii] = acc / nu;
+    }
+    if (m == 1) b[nn] = 1.0;
+    double c0c = c0 - K->model->nugget;
+    double cbb = 0.0;
+    for (int64_t u = 0; u < n_off; u++)
+        for (int64_t v = 0; v < n_off; v++) {
+            if (u == v) cbb += c0c;
+            else {
+                double dh[3];
+                for (int d = 0; d < D; d++) dh[d] = offs[u * D + d] - offs[v * D + d];
+                cbb += c0 - rc_gamma_dh(K->model, D, dh);
+            }
+        }
+    cbb /= nu * nu;
+    memcpy(w->b0, b, sizeof(double) * (size_t)dim);
+    int32_t st = rc_lu_solve(A, (int32_t)dim, b);
+    if (st != RC_OK) return st;
+    const double *x = b, *b0 = w->b0;
+    if (K->cfg.method == RC_SIMPLE) {
+        double mean = K->cfg.simple_mean, v = mean, red = 0.0;
+        for (int64_t ii = 0; ii < nn; ii++) {
+            v += x[ii] * (K->values[w->nb[ii]] - mean);
+            red += x[ii] * b0[ii];
+        }
+        *val = v;
+        *var = fmax(cbb - red, 0.0);
+        *has_lag = 0;
+        *lag = NAN;
+    } else {
+        double v = -0.0, red = -0.0;
+        for (int64_t ii = 0; ii < nn; ii++) v += x[ii] * K->values[w->nb[ii]];
+        for (int64_t i = 0; i < dim; i++) red += x[i] * b0[i];
+        *val = v;
+        *var = fmax(cbb - red, 0.0);
+        *has_lag = 1;
+        *lag = x[nn];
+    }
+    return RC_OK;
+}
+
 /* parallel.rs:9-17 analogue: dynamic chunks over targets, output in order. */
 typedef struct {
     const krig *K;
     const double *targets;
+    const double *tdrift;     /* T x nv target covariates (predict_with_drift) */
+    const double *offsets;    /* block discretization (predict_block) */
+    int64_t n_off;
+    int block;
     int64_t T;
     atomic_llong next;
     double *value, *variance, *lagrange;
@@ -1016,7 +1113,17 @@ static void *kworker(void *arg) {
             }
             double v, var, lag;
             int hl;
-            int32_t st = krig_predict(J->K, tg, &w, &v, &var, &lag, &hl);
+            int32_t st;
+            if (J->block)
+                st = krig_predict_block(J->K, tg, J->offsets, J->n_off, &w, &v, &var, &lag, &hl);
+            else if (J->tdrift)   /* predict_with_drift (kriging.rs:505-522) */
+                st = J->K->cfg.method != RC_EXTERNAL_DRIFT
+                         ? RC_INVALID_PARAMETER
+                         : krig_predict(J->K, tg, J->tdrift + t * J->K->nv, &w, &v, &var, &lag, &hl);
+            else                  /* predict (kriging.rs:494-501) */
+                st = J->K->cfg.method == RC_EXTERNAL_DRIFT
+                         ? RC_INVALID_PARAMETER
+                         : krig_predict(J->K, tg, NULL, &w, &v, &var, &lag, &hl);
             if (st != RC_OK) { v = NAN; var = NAN; lag = NAN; } /* NAN_ESTIMATE, kriging.rs:130-134 */
             J->value[t] = v;
             J->variance[t] = var;
@@ -1047,6 +1154,61 @@ int32_t rc_kriging_predict_many(int32_t D, const double *coords, const double *v
     kjob J;
     memset(&J, 0, sizeof(J));
     J.K = &K; J.targets = targets; J.T = T;
+    atomic_init(&J.next, 0);
+    J.value = value; J.variance = variance; J.lagrange = lagrange; J.status = status;
+    run_kjob(&J, nthreads);
+    krig_free(&K);
+    return RC_OK;
+}
+
+/* with_measurement_error (kriging.rs:181-218) / with_external_drift
+ * (223-260) + predict_many (648-652), predict_many_with_drift (656-672) or
+ * predict_block per target (678-777, as predict_block_grid maps it). */
+int32_t rc_kriging_predict_many_ex(int32_t D, const double *coords, const double *values,
+                                   int64_t n, const rc_model *model, const rc_kconfig *cfg,
+                                   const double *errors, int32_t n_vars, const double *drift_data,
+                                   const double *targets, int64_t T, const double *tdrift,
+                                   const double *offsets, int64_t n_off, int32_t block,
+                                   int32_t nthreads, double *value, double *variance,
+                                   double *lagrange, int32_t *status, int64_t *dup_pair) {
+    if (errors) {
+        if (cfg->method == RC_EXTERNAL_DRIFT) return RC_INVALID_PARAMETER;
+        if (has_power(model)) return RC_INVALID_PARAMETER;
+        for (int64_t i = 0; i < n; i++)
+            if (!(isfinite(errors[i]) && errors[i] >= 0.0)) return RC_INVALID_PARAMETER;
+    }
+    if (drift_data) {
+        if (cfg->method != RC_EXTERNAL_DRIFT) return RC_INVALID_PARAMETER;
+        if (n_vars == 0) return RC_INVALID_PARAMETER;
+        if (n_vars != cfg->degree || n_vars > 8) return RC_DIMENSION_MISMATCH;
+        for (int64_t i = 0; i < n * n_vars; i++)
+            if (!isfinite(drift_data[i])) return RC_INVALID_PARAMETER;
+    }
+    krig K;
+    int32_t st = krig_build(&K, D, coords, values, n, model, cfg, dup_pair);
+    if (st != RC_OK) return st;
+    K.err = errors;
+    if (drift_data) {
+        K.nv = n_vars;
+        K.drift = drift_data;
+        double nn = (double)n;
+        for (int k = 0; k < n_vars; k++) {   /* kriging.rs:349-364 */
+            double sum = -0.0;
+            for (int64_t i = 0; i < n; i++) sum += drift_data[i * n_vars + k];
+            double mean = sum / nn;
+            double ss = -0.0;
+            for (int64_t i = 0; i < n; i++) {
+                double d = drift_data[i * n_vars + k] - mean;
+                ss += d * d;
+            }
+            K.dmean[k] = mean;
+            K.dscale[k] = fmax(sqrt(ss / nn), 2.2250738585072014e-308);
+        }
+    }
+    kjob J;
+    memset(&J, 0, sizeof(J));
+    J.K = &K; J.targets = targets; J.T = T;
+    J.tdrift = tdrift; J.offsets = offsets; J.n_off = n_off; J.block = block;
     atomic_init(&J.next, 0);
     J.value = value; J.variance = variance; J.lagrange = lagrange; J.status = status;
     run_kjob(&J, nthreads);

@@ -33,7 +33,7 @@ enum { RC_OK = 0, RC_INSUFFICIENT_DATA = 1, RC_DIMENSION_MISMATCH = 2,
        RC_SINGULAR_SYSTEM = 3, RC_INVALID_PARAMETER = 4, RC_NO_NEIGHBORS = 5,
        RC_DUPLICATE_POINTS = 6 };
 
-enum { RC_SIMPLE = 0, RC_ORDINARY = 1, RC_UNIVERSAL = 2 };
+enum { RC_SIMPLE = 0, RC_ORDINARY = 1, RC_UNIVERSAL = 2, RC_EXTERNAL_DRIFT = 3 };
 
 typedef struct {
     double azimuth_deg, ratio, ratio_z, dip_deg, rake_deg;
@@ -58,7 +58,7 @@ typedef struct {
 typedef struct {
     int32_t method;          /* RC_SIMPLE / RC_ORDINARY / RC_UNIVERSAL */
     double simple_mean;
-    int32_t degree;          /* universal drift degree */
+    int32_t degree;          /* universal drift degree; n_vars for external drift */
     int64_t max_neighbors;   /* -1 = None */
     int64_t min_neighbors;   /* -1 = None */
     int64_t max_per_octant;  /* -1 = None */
@@ -102,6 +102,17 @@ int32_t rc_kriging_predict_many(int32_t D, const double *coords, const double *v
                                 const double *targets, int64_t T, int32_t nthreads,
                                 double *value, double *variance, double *lagrange,
                                 int32_t *status, int64_t *dup_pair);
+/* measurement error / external drift / block kriging (kriging.rs:181-260,
+ * 494-522, 656-672, 678-777): errors (n) or NULL; drift_data n x n_vars or
+ * NULL; tdrift T x n_vars (predict_with_drift) or NULL; block != 0 runs
+ * predict_block with the n_off offsets at every target. */
+int32_t rc_kriging_predict_many_ex(int32_t D, const double *coords, const double *values,
+                                   int64_t n, const rc_model *model, const rc_kconfig *cfg,
+                                   const double *errors, int32_t n_vars, const double *drift_data,
+                                   const double *targets, int64_t T, const double *tdrift,
+                                   const double *offsets, int64_t n_off, int32_t block,
+                                   int32_t nthreads, double *value, double *variance,
+                                   double *lagrange, int32_t *status, int64_t *dup_pair);
 int32_t rc_kriging_neighbors(int32_t D, const double *coords, const double *values, int64_t n,
                              const rc_model *model, const rc_kconfig *cfg,
                              const double *targets, int64_t T, int64_t max_out,

@@ -88,6 +88,12 @@ def lib() -> C.CDLL:
                                                     C.POINTER(rc_model), C.POINTER(rc_kconfig),
                                                     _DP, C.c_int64, C.c_int32, _DP, _DP, _DP,
                                                     _I32P, _I64P]),
+            "rc_kriging_predict_many_ex": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64,
+                                                       C.POINTER(rc_model),
+                                                       C.POINTER(rc_kconfig), _DP, C.c_int32,
+                                                       _DP, _DP, C.c_int64, _DP, _DP, C.c_int64,
+                                                       C.c_int32, C.c_int32, _DP, _DP, _DP,
+                                                       _I32P, _I64P]),
             "rc_kriging_neighbors": (C.c_int32, [C.c_int32, _DP, _DP, C.c_int64,
                                                  C.POINTER(rc_model), C.POINTER(rc_kconfig), _DP,
                                                  C.c_int64, C.c_int64, _I32P, _I32P, C.c_int32]),
@@ -248,6 +254,36 @@ def predict_many(coords, values, m, cfg, targets, threads=None):
                                        C.byref(cc), _d(targets), T, threads, _d(v), _d(var),
                                        _d(lag), st.ctypes.data_as(_I32P),
                                        dup.ctypes.data_as(_I64P))
+    return rc, v, var, lag, st, dup
+
+
+def predict_many_ex(coords, values, m, cfg, targets, errors=None, drift=None, tdrift=None,
+                    offsets=None, block=False, threads=None):
+    """with_measurement_error / with_external_drift + predict_many,
+    predict_many_with_drift (tdrift given) or predict_block per target
+    (block=True with offsets).  Same returns as predict_many."""
+    coords = _arr(coords)
+    values = _arr(values)
+    D = coords.shape[1]
+    targets = _arr(targets).reshape(-1, D)
+    T = targets.shape[0]
+    v, var, lag = np.empty(T), np.empty(T), np.empty(T)
+    st = np.empty(T, dtype=np.int32)
+    dup = np.zeros(2, dtype=np.int64)
+    cm, cc = model(m), kconfig(cfg)
+    threads = threads or (os.cpu_count() or 1)
+    e = None if errors is None else _arr(errors)
+    dd = None if drift is None else _arr(drift).reshape(coords.shape[0], -1)
+    nv = 0 if dd is None else dd.shape[1]
+    td = None if tdrift is None else _arr(tdrift).reshape(T, -1)
+    off = None if offsets is None else _arr(offsets).reshape(-1, D)
+    n_off = 0 if off is None else off.shape[0]
+    rc = lib().rc_kriging_predict_many_ex(
+        D, _d(coords), _d(values), coords.shape[0], C.byref(cm), C.byref(cc),
+        None if e is None else _d(e), nv, None if dd is None else _d(dd), _d(targets), T,
+        None if td is None else _d(td), None if off is None else _d(off), n_off,
+        1 if block else 0, threads, _d(v), _d(var), _d(lag), st.ctypes.data_as(_I32P),
+        dup.ctypes.data_as(_I64P))
     return rc, v, var, lag, st, dup
 
 

@@ -639,3 +639,155 @@ def test_mle_reaches_the_optimum_an_independent_optimizer_finds(oracle, kind):
         rc, _, g = oracle.vecchia_loglik_grad_plan(c, v, kind[0], *fit, o, nb, cnt)
         gx = [-g[0] * 2 * x[0], -g[1] * fit[1], -g[2] * fit[2]]
         assert max(abs(t) for t in gx) < 1e-4, gx
+
+
+# ---- measurement error, external drift, block kriging (kriging.rs:181-260,
+# 505-522, 656-803) ----------------------------------------------------------
+def _dense_general(c, v, m, oracle, t, nb, err=None, basis=None, tbasis=None, rhs=None,
+                   vbase=None):
+    """Bordered system from the oracle's gamma, solved by numpy: diagonal
+    c0 + err_i, drift columns basis (n, M) with target row tbasis, optional
+    replacement right-hand side rhs (block averages) and variance base."""
+    c0 = m.total_sill()
+    n = len(nb)
+    M = 0 if basis is None else basis.shape[1]
+    A = np.zeros((n + M, n + M))
+    b = np.zeros(n + M)
+    for a in range(n):
+        for q in range(n):
+            A[a, q] = c0 if a == q else c0 - oracle.gamma_dh(m, c[nb[a]] - c[nb[q]])
+        if err is not None:
+            A[a, a] += err[nb[a]]
+        b[a] = c0 - oracle.gamma_dh(m, c[nb[a]] - t) if rhs is None else rhs[a]
+        for k in range(M):
+            A[a, n + k] = A[n + k, a] = basis[a, k]
+    for k in range(M):
+        b[n + k] = tbasis[k]
+    w = np.linalg.solve(A, b)
+    base = c0 if vbase is None else vbase
+    return float(w[:n] @ v[nb]), float(max(base - w @ b, 0.0))
+
+
+def test_measurement_error_smooths_instead_of_honoring(oracle):
+    # kriging.rs:858-905
+    c, v = cs.sample_data()
+    m = cs.sample_model()
+    cfg = KrigingConfig()
+    _, ex, _, _, _, _ = oracle.predict_many(c, v, m, cfg, c)
+    rc, nv_, nvar, _, st, _ = oracle.predict_many_ex(c, v, m, cfg, c, errors=np.full(5, 0.5))
+    assert rc == 0 and (st == 0).all()
+    assert np.all(np.abs(ex - v) < 1e-8)
+    assert np.all(np.abs(nv_ - v) > 1e-3) and np.all(nvar > 1e-3)
+    t = np.array([[0.3, 0.7]])
+    a = oracle.predict_many(c, v, m, cfg, t)
+    b = oracle.predict_many_ex(c, v, m, cfg, t, errors=np.zeros(5))
+    assert abs(a[1][0] - b[1][0]) < 1e-14 and abs(a[2][0] - b[2][0]) < 1e-14
+    assert oracle.predict_many_ex(c, v, m, cfg, t, errors=np.full(5, -0.1))[0] == 4
+    # the error variance sits on the data-data diagonal only
+    for i, tt in enumerate(np.vstack([c, t])):
+        e = np.array([0.5, 0.1, 0.0, 0.3, 0.2])
+        r = oracle.predict_many_ex(c, v, m, cfg, tt, errors=e)
+        ev, evar = _dense_general(c, v, m, oracle, tt, np.arange(5), err=e,
+                                  basis=np.ones((5, 1)), tbasis=[1.0])
+        assert abs(r[1][0] - ev) < 1e-12 and abs(r[2][0] - evar) < 1e-12, i
+
+
+def test_external_drift_reproduces_linear_relation(oracle):
+    # kriging.rs:1415-1450
+    coords, values, drift = [], [], []
+    for i in range(5):
+        for j in range(5):
+            x, y = i * 10.0, j * 10.0
+            d = math.sin(x * 0.13 + y * 0.07)
+            coords.append([x, y])
+            drift.append([d])
+            values.append(2.0 + 3.0 * d)
+    c, v, dd = np.array(coords), np.array(values), np.array(drift)
+    m = cs.sample_model()
+    cfg = KrigingConfig(method=KrigingMethod.ExternalDrift(1))
+    t = np.array([[12.0, 33.0], [45.0, 5.0]])
+    rc, val, _, _, st, _ = oracle.predict_many_ex(c, v, m, cfg, t, drift=dd,
+                                                  tdrift=np.array([[0.4], [-0.7]]))
+    assert rc == 0 and (st == 0).all()
+    assert abs(val[0] - (2 + 3 * 0.4)) < 1e-7 and abs(val[1] - (2 - 3 * 0.7)) < 1e-7
+    # predict (no drift) on an external-drift predictor fails per target
+    r = oracle.predict_many_ex(c, v, m, cfg, t, drift=dd)
+    assert (r[4] == 4).all() and np.isnan(r[1]).all()
+    # and predict_with_drift on an ordinary one
+    r = oracle.predict_many_ex(c, v, m, KrigingConfig(), t, tdrift=np.zeros((2, 1)))
+    assert (r[4] == 4).all()
+
+
+@pytest.mark.parametrize("nvars", [1, 2, 3])
+def test_external_drift_against_dense_numpy(oracle, nvars):
+    # raw covariates as drift columns: the reference's per-covariate
+    # normalization spans the same space, so the estimate is the same
+    from geostat_hip import workloads
+    c, v = workloads.points(17, 300, [100.0, 100.0])
+    rng = np.random.default_rng(5)
+    dd = np.column_stack([np.sin(c[:, 0] / (9.0 + k)) + 0.1 * rng.standard_normal(300)
+                          for k in range(nvars)])
+    t = workloads.points(18, 12, [100.0, 100.0])[0]
+    td = np.column_stack([np.sin(t[:, 0] / (9.0 + k)) for k in range(nvars)])
+    m = VariogramModel(0.1, [Structure(ModelKind.EXPONENTIAL, 1.0, 25.0)])
+    cfg = KrigingConfig(method=KrigingMethod.ExternalDrift(nvars), max_neighbors=16)
+    rc, val, var, _, st, _ = oracle.predict_many_ex(c, v, m, cfg, t, drift=dd, tdrift=td)
+    assert rc == 0 and (st == 0).all()
+    _, idx, cnt = oracle.neighbors(c, v, m, KrigingConfig(max_neighbors=16), t, 16)
+    for i in range(len(t)):
+        nb = idx[i, :cnt[i]]
+        basis = np.column_stack([np.ones(len(nb)), dd[nb]])
+        ev, evar = _dense_general(c, v, m, oracle, t[i], nb, basis=basis,
+                                  tbasis=np.concatenate([[1.0], td[i]]))
+        assert abs(val[i] - ev) < 1e-9 * max(1, abs(ev)) and abs(var[i] - evar) < 1e-9
+
+
+def test_block_kriging_averages_point_predictions(oracle):
+    # kriging.rs:1524-1566
+    from geostat_hip import block_offsets
+    c, v = cs.sample_data()
+    m = cs.sample_model()
+    center = np.array([0.6, 0.4])
+    offs = block_offsets([0.4, 0.4], [4, 4])
+    rc, bv, bvar, _, st, _ = oracle.predict_many_ex(c, v, m, KrigingConfig(), center, offsets=offs,
+                                                    block=True)
+    assert rc == 0 and st[0] == 0
+    _, pv, _, _, _, _ = oracle.predict_many(c, v, m, KrigingConfig(), center + offs)
+    assert abs(bv[0] - pv.mean()) < 1e-10
+    pvar = oracle.predict_many(c, v, m, KrigingConfig(), center)[2][0]
+    assert bvar[0] < pvar
+    r = oracle.predict_many_ex(c, v, m, KrigingConfig(method=KrigingMethod.Universal(1)), center,
+                               offsets=offs, block=True)
+    assert r[4][0] == 4
+    with pytest.raises(Exception):
+        block_offsets([0.0, 1.0], [4, 4])
+    with pytest.raises(Exception):
+        block_offsets([1.0, 1.0], [0, 4])
+    # independent assembly: block-averaged right-hand side and C(B,B)
+    c0 = m.total_sill()
+    rhs = [np.mean([c0 - oracle.gamma_dh(m, c[a] - (center + o)) for o in offs]) for a in range(5)]
+    cbb = np.mean([[c0 - m.nugget if u == w else c0 - oracle.gamma_dh(m, offs[u] - offs[w])
+                    for w in range(len(offs))] for u in range(len(offs))])
+    ev, evar = _dense_general(c, v, m, oracle, center, np.arange(5), basis=np.ones((5, 1)),
+                              tbasis=[1.0], rhs=rhs, vbase=cbb)
+    assert abs(bv[0] - ev) < 1e-12 and abs(bvar[0] - evar) < 1e-12
+
+
+def test_block_kriging_3d_uses_z_separation(oracle):
+    # three_d.rs:219-250 (data: 100 seeded points in [0,100]^2 x [0,50])
+    r = oracle.Rng(13)
+    c = np.array([[r.uniform() * 100.0, r.uniform() * 100.0, r.uniform() * 50.0]
+                  for _ in range(100)])
+    v = np.sin(c[:, 0] / 20.0) + 0.01 * c[:, 2]
+    m = VariogramModel(0.05, [Structure(ModelKind.SPHERICAL, 1.0, 30.0)])
+    center = np.array([42.0, 37.0, 18.0])
+    one = oracle.predict_many_ex(c, v, m, KrigingConfig(), center, offsets=np.zeros((1, 3)),
+                                 block=True)
+    pt = oracle.predict_many(c, v, m, KrigingConfig(), center)
+    assert abs(one[1][0] - pt[1][0]) < 1e-10
+    assert abs(one[2][0] - (pt[2][0] - m.nugget)) < 1e-10
+    deg = oracle.predict_many_ex(c, v, m, KrigingConfig(), center, offsets=np.zeros((2, 3)),
+                                 block=True)
+    ver = oracle.predict_many_ex(c, v, m, KrigingConfig(), center,
+                                 offsets=np.array([[0, 0, -5.0], [0, 0, 5.0]]), block=True)
+    assert abs(ver[1][0] - deg[1][0]) > 1e-6
