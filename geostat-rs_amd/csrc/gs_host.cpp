@@ -1245,6 +1245,236 @@ int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, in
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// Cross-validation (validation.rs:136-400): each fold builds a predictor on
+// the points outside it and predicts its members, as the reference does
+// (PointSet::excluding / the training subset, then Kriging::new and predict).
+// ---------------------------------------------------------------------------
+namespace {
+
+// rng.rs Rng: splitmix64 seeding, xoshiro256++ and Fisher-Yates shuffle
+// (rng.rs:8-52, 76-82); the fold assignment depends on the exact stream.
+struct FoldRng {
+    uint64_t s[4];
+    static uint64_t splitmix(uint64_t &st) {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    explicit FoldRng(uint64_t seed) {
+        uint64_t sm = seed;
+        for (int i = 0; i < 4; i++) s[i] = splitmix(sm);
+    }
+    uint64_t next() {
+        const uint64_t r = rotl(s[0] + s[3], 23) + s[0];
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    template <class T>
+    void shuffle(std::vector<T> &xs) {
+        for (size_t i = xs.size(); i-- > 1;) {
+            const size_t j = (size_t)(next() % (uint64_t)(i + 1));
+            std::swap(xs[i], xs[j]);
+        }
+    }
+};
+
+// cv_with_folds (validation.rs:214-256): folds in index order, empty folds
+// skipped; the first failing fold (and member) decides the error.
+int32_t cv_with_folds(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                      int64_t n, const gs_model *model, const gs_kconfig *cfg, int32_t nv,
+                      const double *drift, const std::vector<int64_t> &fold_of,
+                      const std::vector<std::vector<int64_t>> &members, double *predicted,
+                      double *variance) {
+    std::vector<double> tc, tv, td, mc, md, ov, ovar, olag;
+    std::vector<int32_t> ost;
+    for (size_t f = 0; f < members.size(); f++) {
+        const auto &mem = members[f];
+        if (mem.empty()) continue;
+        tc.clear();
+        tv.clear();
+        td.clear();
+        for (int64_t i = 0; i < n; i++) {
+            if (fold_of[(size_t)i] == (int64_t)f) continue;
+            for (int d = 0; d < dim; d++) tc.push_back(coords[i * dim + d]);
+            tv.push_back(values[i]);
+            for (int k = 0; k < nv; k++) td.push_back(drift[i * nv + k]);
+        }
+        const int64_t nt = (int64_t)tv.size();
+        if (nt == 0) return fail(GS_ERR_INSUFFICIENT_DATA, "cross-validation fold has no training points");
+        gs_kriging *K = nullptr;
+        int64_t dup[2];
+        int32_t st = kriging_build(ctx, dim, tc.data(), tv.data(), nt, model, cfg, nullptr, nv,
+                                   nv > 0 ? td.data() : nullptr, &K, dup);
+        if (st != GS_OK) return st;
+        std::unique_ptr<gs_kriging> own(K);
+        const int64_t T = (int64_t)mem.size();
+        mc.resize((size_t)(T * dim));
+        md.resize((size_t)(T * std::max(nv, 1)));
+        for (int64_t j = 0; j < T; j++) {
+            for (int d = 0; d < dim; d++) mc[(size_t)(j * dim + d)] = coords[mem[(size_t)j] * dim + d];
+            for (int k = 0; k < nv; k++) md[(size_t)(j * nv + k)] = drift[mem[(size_t)j] * nv + k];
+        }
+        ov.resize((size_t)T);
+        ovar.resize((size_t)T);
+        olag.resize((size_t)T);
+        ost.resize((size_t)T);
+        {
+            std::lock_guard<std::mutex> lk(K->mu);
+            HIP_TRY(hipSetDevice(ctx->device));
+            st = predict_host(K, mc.data(), T, GS_PREC_FP64, ov.data(), ovar.data(), olag.data(),
+                              ost.data(), nv > 0 ? md.data() : nullptr, Extra());
+        }
+        if (st != GS_OK) return st;
+        for (int64_t j = 0; j < T; j++) {
+            if (ost[(size_t)j] != GS_OK)
+                return fail(ost[(size_t)j], "cross-validation: prediction at point " +
+                                                std::to_string(mem[(size_t)j]) + " failed");
+            predicted[mem[(size_t)j]] = ov[(size_t)j];
+            variance[mem[(size_t)j]] = ovar[(size_t)j];
+        }
+    }
+    return GS_OK;
+}
+
+int32_t cv_args(gs_ctx *ctx, int32_t dim, const double *coords, const double *values, int64_t n,
+                const gs_model *model, const gs_kconfig *cfg, double *predicted,
+                double *variance) {
+    gs_kriging *dummy = nullptr;
+    int32_t st = check_pointset(ctx, dim, coords, values, n, model, cfg, &dummy);
+    if (st != GS_OK) return st;
+    if (!predicted || !variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL output buffer");
+    if (n < 3) return fail(GS_ERR_INSUFFICIENT_DATA, "cross-validation requires at least 3 points");
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gs_cv_leave_one_out(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                            int64_t n, const gs_model *model, const gs_kconfig *config,
+                            double *predicted, double *variance) {
+    int32_t st = cv_args(ctx, dim, coords, values, n, model, config, predicted, variance);
+    if (st != GS_OK) return st;
+    if (config->method == GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER, "external drift needs gs_cv_leave_one_out_with_drift");
+    std::vector<int64_t> fold_of((size_t)n);
+    std::vector<std::vector<int64_t>> members((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        fold_of[(size_t)i] = i;
+        members[(size_t)i].push_back(i);
+    }
+    return cv_with_folds(ctx, dim, coords, values, n, model, config, 0, nullptr, fold_of, members,
+                         predicted, variance);
+}
+
+int32_t gs_cv_leave_one_out_with_drift(gs_ctx *ctx, int32_t dim, const double *coords,
+                                       const double *values, int64_t n, const gs_model *model,
+                                       const gs_kconfig *config, int32_t n_vars,
+                                       const double *drift_data, double *predicted,
+                                       double *variance) {
+    int32_t st = cv_args(ctx, dim, coords, values, n, model, config, predicted, variance);
+    if (st != GS_OK) return st;
+    if (!drift_data) return fail(GS_ERR_DIMENSION_MISMATCH, "no drift rows for the data points");
+    // with_external_drift's checks run per fold in the reference; they do
+    // not depend on the fold
+    if (config->method != GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER, "with_external_drift requires KrigingMethod::ExternalDrift");
+    if (n_vars <= 0) return fail(GS_ERR_INVALID_PARAMETER, "external drift requires at least one covariate");
+    for (int64_t i = 0; i < n * n_vars; i++)
+        if (!std::isfinite(drift_data[i]))
+            return fail(GS_ERR_INVALID_PARAMETER, "non-finite drift value");
+    if (n_vars > 3)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: more than 3 external covariates");
+    std::vector<int64_t> fold_of((size_t)n);
+    std::vector<std::vector<int64_t>> members((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        fold_of[(size_t)i] = i;
+        members[(size_t)i].push_back(i);
+    }
+    return cv_with_folds(ctx, dim, coords, values, n, model, config, n_vars, drift_data, fold_of,
+                         members, predicted, variance);
+}
+
+int32_t gs_cv_k_fold(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                     int64_t n, const gs_model *model, const gs_kconfig *config, int64_t k,
+                     uint64_t seed, double *predicted, double *variance) {
+    int32_t st = cv_args(ctx, dim, coords, values, n, model, config, predicted, variance);
+    if (st != GS_OK) return st;
+    if (k < 2 || k > n) return fail(GS_ERR_INVALID_PARAMETER, "k-fold requires 2 <= k <= n");
+    if (config->method == GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: k-fold with external drift");
+    // fold_assignment (validation.rs:199-212)
+    std::vector<int64_t> order((size_t)n);
+    for (int64_t i = 0; i < n; i++) order[(size_t)i] = i;
+    FoldRng(seed).shuffle(order);
+    std::vector<int64_t> fold_of((size_t)n);
+    std::vector<std::vector<int64_t>> members((size_t)k);
+    for (int64_t j = 0; j < n; j++) {
+        const int64_t i = order[(size_t)j];
+        fold_of[(size_t)i] = j % k;
+        members[(size_t)(j % k)].push_back(i);
+    }
+    return cv_with_folds(ctx, dim, coords, values, n, model, config, 0, nullptr, fold_of, members,
+                         predicted, variance);
+}
+
+int32_t gs_cv_block(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                    int64_t n, const gs_model *model, const gs_kconfig *config,
+                    const int64_t *blocks_per_dim, double *predicted, double *variance) {
+    int32_t st = cv_args(ctx, dim, coords, values, n, model, config, predicted, variance);
+    if (st != GS_OK) return st;
+    if (!blocks_per_dim) return fail(GS_ERR_INVALID_PARAMETER, "blocks_per_dim is NULL");
+    int64_t nb = 1;
+    for (int d = 0; d < dim; d++) {
+        if (blocks_per_dim[d] <= 0)
+            return fail(GS_ERR_INVALID_PARAMETER, "blocks_per_dim entries must be at least 1");
+        nb *= blocks_per_dim[d];
+    }
+    if (nb < 2) return fail(GS_ERR_INVALID_PARAMETER, "block CV requires at least 2 blocks total");
+    if (config->method == GS_EXTERNAL_DRIFT)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: block CV with external drift");
+    // spatial_block_assignment (validation.rs:294-323)
+    double mn[3], mx[3];
+    for (int d = 0; d < dim; d++) mn[d] = mx[d] = coords[d];
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) {
+            mn[d] = std::fmin(mn[d], coords[i * dim + d]);
+            mx[d] = std::fmax(mx[d], coords[i * dim + d]);
+        }
+    std::vector<int64_t> fold_of((size_t)n);
+    std::vector<std::vector<int64_t>> members((size_t)nb);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t idx = 0, stride = 1;
+        for (int d = 0; d < dim; d++) {
+            const double span = std::fmax(mx[d] - mn[d], std::numeric_limits<double>::min());
+            const double frac = std::clamp((coords[i * dim + d] - mn[d]) / span, 0.0, 1.0 - 1e-12);
+            // float -> usize cast saturates at 0 and truncates toward zero
+            const double scaled = frac * (double)blocks_per_dim[d];
+            int64_t bd = scaled > 0.0 ? (int64_t)scaled : 0;
+            bd = std::min<int64_t>(bd, blocks_per_dim[d] - 1);
+            idx += bd * stride;
+            stride *= blocks_per_dim[d];
+        }
+        fold_of[(size_t)i] = idx;
+        members[(size_t)idx].push_back(i);
+    }
+    return cv_with_folds(ctx, dim, coords, values, n, model, config, 0, nullptr, fold_of, members,
+                         predicted, variance);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
 // Vecchia plan: maxmin ordering + nearest predecessors (vecchia.rs:144-282)
 // ---------------------------------------------------------------------------
 namespace {

@@ -90,6 +90,15 @@ SIGNATURES = {
                                                         _DP, _DP, _I32P]),
     "gs_kriging_predict_block_many": (C.c_int32, [_P, _DP, C.c_int64, _DP, C.c_int32, C.c_int32,
                                                    _DP, _DP, _DP, _I32P]),
+    "gs_cv_leave_one_out": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
+                                        C.POINTER(gs_kconfig), _DP, _DP]),
+    "gs_cv_leave_one_out_with_drift": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64,
+                                                   C.POINTER(gs_model), C.POINTER(gs_kconfig),
+                                                   C.c_int32, _DP, _DP, _DP]),
+    "gs_cv_k_fold": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
+                                 C.POINTER(gs_kconfig), C.c_int64, C.c_uint64, _DP, _DP]),
+    "gs_cv_block": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
+                                C.POINTER(gs_kconfig), _I64P, _DP, _DP]),
     "gs_vecchia_mle": (C.c_int32, [_P, _DP, C.c_int32, C.c_double, C.POINTER(gs_model), _DP]),
 }
 

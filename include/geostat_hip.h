@@ -200,6 +200,35 @@ int32_t gs_kriging_set_timing(gs_kriging *k, int32_t enable);
 int32_t gs_kriging_timing_summary(gs_kriging *k, int64_t *calls, double *ms_search,
                                   double *ms_solve);
 
+/* ---- cross-validation (validation.rs) ------------------------------- */
+/* Each fold builds a predictor on the points outside it (the same checks
+ * as gs_kriging_new) and predicts its members; predicted / variance are n
+ * doubles in dataset order.  The first failing fold (and member) decides
+ * the returned error, as the reference's sequential collect does.  At
+ * least 3 points. */
+
+/* leave_one_out (validation.rs:136-159). */
+int32_t gs_cv_leave_one_out(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                            int64_t n, const gs_model *model, const gs_kconfig *config,
+                            double *predicted, double *variance);
+/* leave_one_out_with_drift (validation.rs:163-194): drift_data n*n_vars. */
+int32_t gs_cv_leave_one_out_with_drift(gs_ctx *ctx, int32_t dim, const double *coords,
+                                       const double *values, int64_t n, const gs_model *model,
+                                       const gs_kconfig *config, int32_t n_vars,
+                                       const double *drift_data, double *predicted,
+                                       double *variance);
+/* k_fold (validation.rs:272-292): folds from the seeded shuffle of
+ * fold_assignment (199-212), 2 <= k <= n. */
+int32_t gs_cv_k_fold(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                     int64_t n, const gs_model *model, const gs_kconfig *config, int64_t k,
+                     uint64_t seed, double *predicted, double *variance);
+/* block_cv (validation.rs:348-373): blocks_per_dim[dim] regular blocks over
+ * the bounding box (spatial_block_assignment, 294-323); empty blocks are
+ * skipped. */
+int32_t gs_cv_block(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
+                    int64_t n, const gs_model *model, const gs_kconfig *config,
+                    const int64_t *blocks_per_dim, double *predicted, double *variance);
+
 /* ---- Vecchia ------------------------------------------------------ */
 
 /* vecchia_plan (vecchia.rs:232-282): maxmin ordering (order == NULL) or the

@@ -388,3 +388,77 @@ def duplicate_pair(coords):
     found = lib().rc_duplicate_pair(coords.shape[1], _d(coords), coords.shape[0],
                                     pair.ctypes.data_as(_I64P))
     return (int(pair[0]), int(pair[1])) if found else None
+
+
+# ---- validation.rs ---------------------------------------------------------
+def shuffle(rng: "Rng", xs: list) -> None:
+    """Rng::shuffle (rng.rs:76-82): Fisher-Yates from the end."""
+    for i in range(len(xs) - 1, 0, -1):
+        j = rng.next_u64() % (i + 1)
+        xs[i], xs[j] = xs[j], xs[i]
+
+
+def fold_assignment(n: int, k: int, seed: int):
+    """fold_assignment (validation.rs:199-212)."""
+    order = list(range(n))
+    shuffle(Rng(seed), order)
+    fold_of = [0] * n
+    members = [[] for _ in range(k)]
+    for j, i in enumerate(order):
+        fold_of[i] = j % k
+        members[j % k].append(i)
+    return fold_of, members
+
+
+def spatial_block_assignment(coords, blocks_per_dim):
+    """spatial_block_assignment (validation.rs:294-323)."""
+    coords = _arr(coords)
+    D = coords.shape[1]
+    mn = list(coords[0])
+    mx = list(coords[0])
+    for p in coords:
+        for d in range(D):
+            mn[d] = min(mn[d], p[d])
+            mx[d] = max(mx[d], p[d])
+    nb = int(np.prod(blocks_per_dim))
+    fold_of = [0] * len(coords)
+    members = [[] for _ in range(nb)]
+    for i, p in enumerate(coords):
+        idx, stride = 0, 1
+        for d in range(D):
+            span = max(mx[d] - mn[d], 2.2250738585072014e-308)
+            frac = min(max((p[d] - mn[d]) / span, 0.0), 1.0 - 1e-12)
+            bd = min(int(frac * blocks_per_dim[d]), blocks_per_dim[d] - 1)
+            idx += bd * stride
+            stride *= blocks_per_dim[d]
+        fold_of[i] = idx
+        members[idx].append(i)
+    return fold_of, members
+
+
+def cv_with_folds(coords, values, m, cfg, fold_of, members, drift=None):
+    """cv_with_folds (validation.rs:214-256): (rc, predicted, variance); rc is
+    the first failing fold's error (build or per-member status)."""
+    coords = _arr(coords)
+    values = _arr(values)
+    n = len(values)
+    pred, var = np.zeros(n), np.zeros(n)
+    fold_of = np.asarray(fold_of)
+    for f, mem in enumerate(members):
+        if not mem:
+            continue
+        tr = np.nonzero(fold_of != f)[0]
+        if drift is None:
+            rc, v, vv, _, st, _ = predict_many(coords[tr], values[tr], m, cfg, coords[mem])
+        else:
+            dd = _arr(drift)
+            rc, v, vv, _, st, _ = predict_many_ex(coords[tr], values[tr], m, cfg, coords[mem],
+                                                  drift=dd[tr], tdrift=dd[mem])
+        if rc != 0:
+            return rc, pred, var
+        for j, i in enumerate(mem):
+            if st[j] != 0:
+                return int(st[j]), pred, var
+            pred[i] = v[j]
+            var[i] = vv[j]
+    return 0, pred, var

@@ -791,3 +791,79 @@ def test_block_kriging_3d_uses_z_separation(oracle):
     ver = oracle.predict_many_ex(c, v, m, KrigingConfig(), center,
                                  offsets=np.array([[0, 0, -5.0], [0, 0, 5.0]]), block=True)
     assert abs(ver[1][0] - deg[1][0]) > 1e-6
+
+
+# ---- cross-validation (validation.rs:136-400) ------------------------------
+def _smooth_field(oracle, n, seed, scale=20.0):
+    """validation.rs:992-1003 smooth_field."""
+    r = oracle.Rng(seed)
+    c, v = [], []
+    for _ in range(n):
+        x = r.uniform() * 100.0
+        y = r.uniform() * 100.0
+        c.append([x, y])
+        v.append(math.sin(x / scale) + math.cos(y / scale))
+    return np.array(c), np.array(v)
+
+
+def test_fold_assignment_is_balanced_and_shuffle_is_fisher_yates(oracle):
+    # validation.rs:1021-1039; the shuffle against an independent
+    # restatement drawing j = next % (i + 1) from the same stream
+    fold_of, members = oracle.fold_assignment(53, 5, 42)
+    sizes = [len(x) for x in members]
+    assert sum(sizes) == 53 and max(sizes) - min(sizes) <= 1
+    assert sorted(i for mm in members for i in mm) == list(range(53))
+    r = oracle.Rng(7)
+    draws = [r.next_u64() for _ in range(9)]
+    want = list(range(10))
+    for step, i in enumerate(range(9, 0, -1)):
+        j = draws[step] % (i + 1)
+        want[i], want[j] = want[j], want[i]
+    got = list(range(10))
+    oracle.shuffle(oracle.Rng(7), got)
+    assert got == want
+
+
+def test_loo_k_equals_n_and_skill(oracle):
+    # validation.rs:906-935, 1005-1019
+    from geostat_hip.validation import CvResult
+    c, v = _smooth_field(oracle, 40, 5)
+    m = VariogramModel(0.01, [Structure(ModelKind.SPHERICAL, 1.0, 50.0)])
+    cfg = KrigingConfig()
+    n = len(v)
+    rc, lp, lv = oracle.cv_with_folds(c, v, m, cfg, list(range(n)), [[i] for i in range(n)])
+    assert rc == 0
+    fo, mem = oracle.fold_assignment(n, n, 123)
+    rc2, kp, kv = oracle.cv_with_folds(c, v, m, cfg, fo, mem)
+    assert rc2 == 0 and np.abs(kp - lp).max() < 1e-12 and np.abs(kv - lv).max() < 1e-12
+    cv = CvResult(v, lp, lv)
+    std = math.sqrt(np.mean((v - v.mean()) ** 2))
+    assert cv.rmse() < 0.5 * std and abs(cv.mean_error()) < 0.1 and cv.mae() <= cv.rmse()
+
+
+def test_cv_result_anchor_points():
+    # validation.rs:939-981
+    from geostat_hip.validation import CvResult
+    o = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
+    one = np.ones(5)
+    perfect = CvResult(o, o.copy(), one)
+    assert abs(perfect.vecv() - 100) < 1e-12 and abs(perfect.e1() - 100) < 1e-12
+    assert abs(perfect.rrmse()) < 1e-12
+    mean_pred = CvResult(o, np.full(5, 3.0), one)
+    assert abs(mean_pred.vecv()) < 1e-9 and abs(mean_pred.e1()) < 1e-9
+    assert CvResult(o, np.full(5, 10.0), one).vecv() < 0
+    cv = CvResult(o, np.array([1.5, 2.5, 2.5, 4.5, 4.5]), one)
+    assert abs(cv.rrmse() - cv.rmse() / 3.0 * 100) < 1e-12
+    assert abs(cv.rmae() - cv.mae() / 3.0 * 100) < 1e-12
+    assert 0 < cv.vecv() < 100
+
+
+def test_block_assignment_covers_every_point(oracle):
+    # validation.rs:1052-1068 (assignment part) against numpy binning
+    c, _ = _smooth_field(oracle, 120, 7)
+    fo, mem = oracle.spatial_block_assignment(c, [4, 4])
+    lo, hi = c.min(0), c.max(0)
+    frac = np.clip((c - lo) / (hi - lo), 0.0, 1.0 - 1e-12)
+    b = np.minimum((frac * 4).astype(int), 3)
+    assert list(b[:, 0] + 4 * b[:, 1]) == fo
+    assert sum(len(x) for x in mem) == 120
