@@ -115,6 +115,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (CSRC / "gs_solve.hip", []),
         (CSRC / "gs_host.cpp", EXACT),
         (CSRC / "gs_mle.cpp", EXACT),
+        (CSRC / "gs_global.hip", EXACT),
     ]
     units += [(p, []) for p in _gen_sources()]
     jobs = jobs or max(1, min(16, os.cpu_count() or 4))

@@ -415,6 +415,9 @@ struct gs_kriging {
     // extended systems: measurement-error variances, normalized external
     // covariates (nv of them) and block-kriging workspace
     bool has_err = false;
+    // global neighbourhood above 64 points: one shared LU on the device
+    bool global = false;
+    DBuf g_lu, g_piv, g_flag, g_scale, g_w, g_b0;
     int nv = 0;
     double xmean[3] = {0, 0, 0}, xscale[3] = {1, 1, 1};
     DBuf err, xdrift, h_tdrift, blk;
@@ -609,6 +612,36 @@ int32_t check_pointset(gs_ctx *ctx, int32_t dim, const double *coords, const dou
     return validate_model(model, dim, true);
 }
 
+// Largest global neighbourhood: the shared system is (n + M)^2 doubles
+// (2 GiB at the limit) and its factorization n + M dependent steps.
+constexpr int64_t kGlobalMax = 16384;
+
+gs::GlobalParams global_params(gs_kriging *K) {
+    gs::GlobalParams g;
+    std::memset(&g, 0, sizeof(g));
+    g.coords = K->coords.as<double>();
+    g.values = K->values.as<double>();
+    g.n = K->n;
+    g.N = K->n + K->M;
+    g.M = K->M;
+    g.method = K->cfg.method;
+    g.sk_mean = K->cfg.simple_mean;
+    for (int d = 0; d < 3; d++) g.center[d] = K->center[d];
+    g.scale = K->scale;
+    g.err = K->has_err ? K->err.as<double>() : nullptr;
+    g.xdrift = K->nv > 0 ? K->xdrift.as<double>() : nullptr;
+    g.nv = K->nv;
+    for (int k = 0; k < 3; k++) {
+        g.xmean[k] = K->xmean[k];
+        g.xscale[k] = K->xscale[k];
+    }
+    g.model = K->dmodel.as<DevModel>();
+    g.lu = K->g_lu.as<double>();
+    g.piv = K->g_piv.as<int32_t>();
+    g.singular = K->g_flag.as<int32_t>();
+    return g;
+}
+
 // Kriging::build (kriging.rs:262-408).  errors: measurement-error variances
 // (n) or NULL; drift: n x nv external covariates or NULL.
 int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const double *values,
@@ -703,10 +736,11 @@ int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const doub
     // GPU bound and gets a NaN estimate with status GS_ERR_INVALID_PARAMETER,
     // while every other target is exact (its list never saturated).
     if (!K->has_tree) {
-        if (n > 64)
+        if (n > kGlobalMax)
             return fail(GS_ERR_INVALID_PARAMETER,
-                        "unsupported on GPU: global neighbourhood with more than 64 points");
-        K->stride = n;
+                        "unsupported on GPU: global neighbourhood with more than 16384 points");
+        K->global = n > 64;
+        K->stride = K->global ? 1 : n;
     } else if (cfg.max_per_octant != GS_NONE) {
         if (cfg.max_per_octant > 64 || (cfg.max_per_octant << dim) > 64)
             return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: max_per_octant * 2^dim > 64");
@@ -758,6 +792,19 @@ int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const doub
         HIP_TRY(hipMemcpy(K->tidx.p, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
         int32_t st2 = build_cell_grid(K.get(), tc);
         if (st2 != GS_OK) return st2;
+    }
+    if (K->global && n >= K->M) {
+        // factor the target-independent system once (kriging.rs:396-406)
+        const int64_t N = n + K->M;
+        HIP_TRY(K->g_lu.ensure(sizeof(double) * N * N));
+        HIP_TRY(K->g_piv.ensure(sizeof(int32_t) * N));
+        HIP_TRY(K->g_flag.ensure(sizeof(int32_t)));
+        HIP_TRY(K->g_scale.ensure(sizeof(double)));
+        hipStream_t s = ctx->stream;
+        HIP_TRY(gs::launch_zero_i32(K->g_flag.as<int32_t>(), 1, s));
+        gs::GlobalParams gp = global_params(K.get());
+        HIP_TRY(gs::launch_global_factor(dim, K->lean, gp, K->g_scale.as<double>(), s));
+        HIP_TRY(hipStreamSynchronize(s));
     }
     *out = K.release();
     return GS_OK;
@@ -834,7 +881,9 @@ void gs_kriging_free(gs_kriging *k) {
     delete k;
 }
 
-int64_t gs_kriging_max_neighbors(const gs_kriging *k) { return k ? k->stride : -1; }
+int64_t gs_kriging_max_neighbors(const gs_kriging *k) {
+    return k ? (k->global ? k->n : k->stride) : -1;
+}
 
 int32_t gs_kriging_set_timing(gs_kriging *k, int32_t enable) {
     if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
@@ -951,12 +1000,59 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
                    double *d_value, double *d_var, double *d_lag, int32_t *d_status,
                    hipStream_t s, const Extra &x);
 
+// Global neighbourhood above 64 points: per-target substitutions with the
+// shared factors, in chunks whose scratch (b0 and the solution, N doubles
+// each per target) stays within 1 GiB.
+int32_t run_global(gs_kriging *K, const double *d_targets, int64_t T, double *d_value,
+                   double *d_var, double *d_lag, int32_t *d_status, hipStream_t s,
+                   const Extra &x) {
+    const int64_t N = K->n + K->M;
+    if (K->n < K->M) {   // n < m drift terms: InsufficientData at every target (kriging.rs:579-584)
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        HIP_TRY(gs::launch_fill_f64(d_value, T, nan, s));
+        HIP_TRY(gs::launch_fill_f64(d_var, T, nan, s));
+        if (d_lag) HIP_TRY(gs::launch_fill_f64(d_lag, T, nan, s));
+        if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INSUFFICIENT_DATA, s));
+        return GS_OK;
+    }
+    int64_t tc = ((int64_t)1 << 30) / (2 * N * (int64_t)sizeof(double));
+    tc = std::max<int64_t>(64, tc / 64 * 64);
+    tc = std::min<int64_t>(tc, (T + 63) / 64 * 64);
+    HIP_TRY(K->g_w.ensure(sizeof(double) * N * tc));
+    HIP_TRY(K->g_b0.ensure(sizeof(double) * N * tc));
+    gs::GlobalParams g = global_params(K);
+    g.targets = d_targets;
+    g.T = T;
+    g.stride = tc;
+    g.w = K->g_w.as<double>();
+    g.b0 = K->g_b0.as<double>();
+    g.value = d_value;
+    g.variance = d_var;
+    g.lagrange = d_lag;
+    g.status = d_status;
+    g.tdrift = x.tdrift;
+    g.offsets = x.offsets;
+    g.n_off = x.n_off;
+    g.cbb = x.cbb;
+    for (int64_t t0 = 0; t0 < T; t0 += tc) {
+        g.t0 = t0;
+        g.Tc = std::min<int64_t>(tc, T - t0);
+        HIP_TRY(gs::launch_global_solve(K->D, K->lean, g, s));
+    }
+    return GS_OK;
+}
+
 int32_t run_predict(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
                     double *d_value, double *d_var, double *d_lag, int32_t *d_status,
                     hipStream_t s, const Extra &x = Extra()) {
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (T == 0) return GS_OK;
+    if (K->global) {
+        if (precision != GS_PREC_FP64)
+            return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
+        return run_global(K, d_targets, T, d_value, d_var, d_lag, d_status, s, x);
+    }
     std::array<hipEvent_t, 3> ev{};
     if (K->timing) {
         for (auto &e : ev) HIP_TRY(K->take_event(&e));
@@ -1223,7 +1319,15 @@ int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, in
     if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
     if (T == 0) return GS_OK;
     if (!targets || !idx || !count) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
-    if (max_out < k->stride) return fail(GS_ERR_INVALID_PARAMETER, "max_out smaller than the neighbourhood bound");
+    if (max_out < gs_kriging_max_neighbors(k))
+        return fail(GS_ERR_INVALID_PARAMETER, "max_out smaller than the neighbourhood bound");
+    if (k->global) {   // every datum, in index order (kriging.rs:441-444)
+        for (int64_t t = 0; t < T; t++) {
+            for (int64_t j = 0; j < max_out; j++) idx[t * max_out + j] = j < k->n ? (int32_t)j : -1;
+            count[t] = (int32_t)k->n;
+        }
+        return GS_OK;
+    }
     std::lock_guard<std::mutex> lk(k->mu);
     HIP_TRY(hipSetDevice(k->ctx->device));
     hipStream_t s = k->ctx->stream;

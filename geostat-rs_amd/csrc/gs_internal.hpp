@@ -109,6 +109,39 @@ struct SolveParams {
     const double *cbb;       // device scalar: within-block covariance
 };
 
+// Global neighbourhood above 64 points: one shared LU, per-target
+// substitutions (gs_global.hip).
+struct GlobalParams {
+    const double *coords;    // [n][D]
+    const double *values;    // [n]
+    int64_t n, N;            // data points, system size n + M
+    int32_t M, method;       // drift terms; 0 simple, 1 ordinary, 2 universal, 3 external
+    double sk_mean;
+    double center[3];
+    double scale;
+    const double *err;       // [n] measurement-error variances or null
+    const double *xdrift;    // [n][nv] normalized covariates or null
+    int32_t nv;
+    double xmean[3], xscale[3];
+    const double *tdrift;    // [T][nv] target covariates
+    const DevModel *model;
+    double *lu;              // [N][N] factors (row-major, L below the diagonal)
+    int32_t *piv;            // [N] row swapped with row k at step k
+    int32_t *singular;       // device flag: the factorization hit a tiny pivot
+    const double *targets;   // [T][D]
+    int64_t T, t0, Tc;       // all targets, first of this chunk, chunk size
+    int64_t stride;          // scratch row stride (>= Tc)
+    double *w, *b0;          // scratch [N][stride]
+    double *value, *variance, *lagrange;
+    int32_t *status;
+    const double *offsets;   // block discretization or null
+    int32_t n_off;
+    const double *cbb;
+};
+
+hipError_t launch_global_factor(int D, bool lean, GlobalParams p, double *d_scale, hipStream_t s);
+hipError_t launch_global_solve(int D, bool lean, const GlobalParams &p, hipStream_t s);
+
 struct VecchiaParams {
     const double *coords;    // [n][D]
     const double *zc;        // [n] centred values
