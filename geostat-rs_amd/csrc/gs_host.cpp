@@ -551,6 +551,8 @@ extern "C" {
 const char *gs_last_error(void) { return g_err.c_str(); }
 int32_t gs_abi_version(void) { return GS_ABI_VERSION; }
 
+int32_t gs_model_validate(const gs_model *model) { return validate_model(model, 2, true); }
+
 int32_t gs_ctx_create(int32_t device, gs_ctx **out) {
     if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
     *out = nullptr;

@@ -40,7 +40,9 @@ __all__ = [
 
 
 # ---------------------------------------------------------------- errors --
-class GeostatError(Exception):
+class GeostatError(ValueError):
+    """GeostatError (error.rs:15-40).  A ValueError, as the reference's
+    Python binding raises every GeostatError as one."""
     code = -1
 
 
@@ -152,6 +154,23 @@ class Structure:
         return cls(kind, sill, rng, Anisotropy(azimuth_deg, ratio, ratio_z, dip_deg, rake_deg))
 
 
+_KIND_NAME = {0: "spherical", 1: "exponential", 2: "gaussian", 3: "matern15", 4: "matern25",
+              5: "matern", 6: "circular", 7: "stable", 8: "hole", 9: "wave", 10: "power"}
+_KIND_CODE = {v: k for k, v in _KIND_NAME.items()}
+
+
+def _json_num(x: float) -> str:
+    """Shortest round-trip decimal, in serde_json's spelling (1.0, 1e-7)."""
+    x = float(x)
+    if not math.isfinite(x):
+        raise InvalidParameter("non-finite model parameter cannot be written as JSON")
+    r = repr(x)
+    if "e" in r:
+        mant, ex = r.split("e")
+        return f"{mant}e{int(ex)}"
+    return r
+
+
 @dataclass
 class VariogramModel:
     """VariogramModel (model.rs:543-549): nugget + nested structures."""
@@ -163,6 +182,65 @@ class VariogramModel:
         for st in self.structures:
             s += st.sill
         return self.nugget + s
+
+    # ---- JSON (serde form of VariogramModel, model.rs:47-48, 320-341,
+    # 438-449, 543-549): kinds in lowercase, parametrised kinds as
+    # {"matern": nu}; anisotropy fields at their defaults are omitted
+    @classmethod
+    def from_json(cls, text: str) -> "VariogramModel":
+        """VariogramModel.from_json (python/src/lib.rs:131-138)."""
+        import json
+        try:
+            d = json.loads(text)
+            structs = []
+            for st in d["structures"]:
+                k = st["kind"]
+                if isinstance(k, str):
+                    kind = (_KIND_CODE[k], 0.0)
+                else:
+                    (name, param), = k.items()
+                    if name not in ("matern", "stable", "power"):
+                        raise KeyError(name)
+                    kind = (_KIND_CODE[name], float(param))
+                a = st.get("anis")
+                an = None if a is None else Anisotropy(
+                    float(a["azimuth_deg"]), float(a["ratio"]), float(a.get("ratio_z", 1.0)),
+                    float(a.get("dip_deg", 0.0)), float(a.get("rake_deg", 0.0)))
+                structs.append(Structure(kind, float(st["sill"]), float(st["range"]), an))
+            m = cls(float(d["nugget"]), structs)
+        except (KeyError, TypeError, ValueError, AttributeError) as e:
+            raise InvalidParameter(f"cannot parse model JSON: {e}") from None
+        m.validate()
+        return m
+
+    def to_json(self) -> str:
+        """VariogramModel.to_json (python/src/lib.rs:140-143)."""
+        parts = []
+        for st in self.structures:
+            code, param = st.kind
+            name = _KIND_NAME[code]
+            kind = (f'{{"{name}":{_json_num(param)}}}' if code in (5, 7, 10) else f'"{name}"')
+            body = f'"kind":{kind},"sill":{_json_num(st.sill)},"range":{_json_num(st.range)}'
+            if st.anis is not None:
+                a = st.anis
+                an = f'"azimuth_deg":{_json_num(a.azimuth_deg)},"ratio":{_json_num(a.ratio)}'
+                if a.ratio_z != 1.0:
+                    an += f',"ratio_z":{_json_num(a.ratio_z)}'
+                if a.dip_deg != 0.0:
+                    an += f',"dip_deg":{_json_num(a.dip_deg)}'
+                if a.rake_deg != 0.0:
+                    an += f',"rake_deg":{_json_num(a.rake_deg)}'
+                body += f',"anis":{{{an}}}'
+            parts.append("{" + body + "}")
+        return f'{{"nugget":{_json_num(self.nugget)},"structures":[{",".join(parts)}]}}'
+
+    def validate(self) -> None:
+        """VariogramModel::new's checks (model.rs:554-618), as the C library
+        runs them."""
+        if len(self.structures) > _abi.GS_MAX_STRUCTURES:
+            raise InvalidParameter("unsupported on GPU: more than 8 nested structures")
+        cm = self._c()
+        _check(_abi.lib().gs_model_validate(C.byref(cm)))
 
     def _c(self) -> _abi.gs_model:
         if len(self.structures) > _abi.GS_MAX_STRUCTURES:

@@ -99,6 +99,7 @@ SIGNATURES = {
                                  C.POINTER(gs_kconfig), C.c_int64, C.c_uint64, _DP, _DP]),
     "gs_cv_block": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
                                 C.POINTER(gs_kconfig), _I64P, _DP, _DP]),
+    "gs_model_validate": (C.c_int32, [C.POINTER(gs_model)]),
     "gs_vecchia_mle": (C.c_int32, [_P, _DP, C.c_int32, C.c_double, C.POINTER(gs_model), _DP]),
 }
 

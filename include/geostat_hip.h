@@ -110,6 +110,10 @@ int32_t gs_ctx_synchronize(gs_ctx *ctx);
 /* Thread-local message of the last failing call on this thread. */
 const char *gs_last_error(void);
 int32_t gs_abi_version(void);
+/* VariogramModel::new's checks (model.rs:554-618): nugget, sills, ranges,
+ * kind parameters, anisotropy, positive total sill.  GS_OK or
+ * GS_ERR_INVALID_PARAMETER (more than 8 structures is rejected too). */
+int32_t gs_model_validate(const gs_model *model);
 
 /* ---- kriging ------------------------------------------------------ */
 
