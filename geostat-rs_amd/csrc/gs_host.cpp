@@ -392,6 +392,15 @@ struct gs_ctx {
     DBuf nodes;   // gs::BesselNodes
 };
 
+// Per-call device workspace of a predictor: neighbour lists, search order,
+// bucket lists, staging for host buffers, global-neighbourhood scratch.
+// The handle's own calls use the one inside the handle; the request loop
+// gives every slot its own, so requests on one dataset run concurrently.
+struct KWork {
+    DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
+    DBuf keys, keys_out, vals, perm, sort_tmp, flags, h_tdrift, blk, g_w, g_b0;
+};
+
 struct gs_kriging {
     gs_ctx *ctx = nullptr;
     int D = 2;
@@ -417,16 +426,15 @@ struct gs_kriging {
     bool has_err = false;
     // global neighbourhood above 64 points: one shared LU on the device
     bool global = false;
-    DBuf g_lu, g_piv, g_flag, g_scale, g_w, g_b0;
+    DBuf g_lu, g_piv, g_flag, g_scale;
     int nv = 0;
     double xmean[3] = {0, 0, 0}, xscale[3] = {1, 1, 1};
-    DBuf err, xdrift, h_tdrift, blk;
+    DBuf err, xdrift;
     // cell grid of the search-frame coordinates (tile search)
     DBuf cg_start, cg_pts, cg_idx;
     gs::CellGridView cg{};
-    // per-call workspace (grows)
-    DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
-    DBuf keys, keys_out, vals, perm, sort_tmp, flags;
+    // per-call workspace of the calls made through the handle itself
+    KWork w;
     std::mutex mu;
     // measurement hook: (search start, search end / solve start, solve end)
     bool timing = false;
@@ -923,52 +931,52 @@ namespace {
 int32_t stride_of(const gs_kriging *K) { return (int32_t)std::max<int64_t>(K->stride, 1); }
 
 // Neighbour lists for T device targets into the handle's nbr/count buffers.
-int32_t run_neighbors(gs_kriging *K, const double *d_targets, int64_t T, hipStream_t s) {
+int32_t run_neighbors(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, hipStream_t s) {
     const int32_t stride = stride_of(K);
-    HIP_TRY(K->nbr.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1) * stride));
-    HIP_TRY(K->count.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1)));
+    HIP_TRY(W.nbr.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1) * stride));
+    HIP_TRY(W.count.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1)));
     if (!K->has_tree) {
-        HIP_TRY(gs::launch_iota_nbr(K->nbr.as<int32_t>(), K->count.as<int32_t>(), T, (int32_t)K->n,
+        HIP_TRY(gs::launch_iota_nbr(W.nbr.as<int32_t>(), W.count.as<int32_t>(), T, (int32_t)K->n,
                                     stride, s));
         return GS_OK;
     }
     // 1) Hilbert order of the targets' search-frame cells
     const size_t tmp_bytes = gs::tile_sort_workspace(T);
-    HIP_TRY(K->keys.ensure(sizeof(uint64_t) * T));
-    HIP_TRY(K->keys_out.ensure(sizeof(uint64_t) * T));
-    HIP_TRY(K->vals.ensure(sizeof(int32_t) * T));
-    HIP_TRY(K->perm.ensure(sizeof(int32_t) * T));
-    HIP_TRY(K->sort_tmp.ensure(tmp_bytes));
-    HIP_TRY(K->flags.ensure(sizeof(int32_t) * (T + 4)));
-    HIP_TRY(gs::launch_tile_order(K->D, d_targets, T, K->xf, K->cg, K->keys.as<uint64_t>(),
-                                  K->keys_out.as<uint64_t>(), K->vals.as<int32_t>(),
-                                  K->perm.as<int32_t>(), K->sort_tmp.p, tmp_bytes, s));
+    HIP_TRY(W.keys.ensure(sizeof(uint64_t) * T));
+    HIP_TRY(W.keys_out.ensure(sizeof(uint64_t) * T));
+    HIP_TRY(W.vals.ensure(sizeof(int32_t) * T));
+    HIP_TRY(W.perm.ensure(sizeof(int32_t) * T));
+    HIP_TRY(W.sort_tmp.ensure(tmp_bytes));
+    HIP_TRY(W.flags.ensure(sizeof(int32_t) * (T + 4)));
+    HIP_TRY(gs::launch_tile_order(K->D, d_targets, T, K->xf, K->cg, W.keys.as<uint64_t>(),
+                                  W.keys_out.as<uint64_t>(), W.vals.as<int32_t>(),
+                                  W.perm.as<int32_t>(), W.sort_tmp.p, tmp_bytes, s));
     // 2) tile search: canonical lists; boundary ties and scattered
     //    wavefronts flagged
-    int32_t *flag_len = K->flags.as<int32_t>() + T;
+    int32_t *flag_len = W.flags.as<int32_t>() + T;
     HIP_TRY(gs::launch_zero_i32(flag_len, 1, s));
     gs::TileParams tp;
     std::memset(&tp, 0, sizeof(tp));
     tp.G = K->cg;
     tp.targets = d_targets;
     tp.T = T;
-    tp.perm = K->perm.as<int32_t>();
+    tp.perm = W.perm.as<int32_t>();
     tp.k = (int32_t)K->k_heap;
     tp.per_octant = K->per_oct;
     tp.max_nb = (int32_t)K->stride;
     tp.stride = stride;
     tp.r2 = K->r2;
     tp.xf = K->xf;
-    tp.out_idx = K->nbr.as<int32_t>();
-    tp.out_count = K->count.as<int32_t>();
-    tp.flag_list = K->flags.as<int32_t>();
+    tp.out_idx = W.nbr.as<int32_t>();
+    tp.out_count = W.count.as<int32_t>();
+    tp.flag_list = W.flags.as<int32_t>();
     tp.flag_len = flag_len;
     HIP_TRY(gs::launch_tile_knn(K->D, tp, s));
     // 3) exact kd-tree replay (the reference's traversal order) of the
     //    flagged targets
     gs::KnnParams p;
     std::memset(&p, 0, sizeof(p));
-    p.list = K->flags.as<int32_t>();
+    p.list = W.flags.as<int32_t>();
     p.list_len = flag_len;
     p.tcoords = K->tcoords.as<double>();
     p.tidx = K->tidx.as<uint32_t>();
@@ -981,8 +989,8 @@ int32_t run_neighbors(gs_kriging *K, const double *d_targets, int64_t T, hipStre
     p.depth = K->depth;
     p.r2 = K->r2;
     p.xf = K->xf;
-    p.out_idx = K->nbr.as<int32_t>();
-    p.out_count = K->count.as<int32_t>();
+    p.out_idx = W.nbr.as<int32_t>();
+    p.out_count = W.count.as<int32_t>();
     p.stride = stride;
     HIP_TRY(gs::launch_knn(K->D, p, s));
     return GS_OK;
@@ -998,14 +1006,14 @@ struct Extra {
     const double *cbb = nullptr;
 };
 
-int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
-                   double *d_value, double *d_var, double *d_lag, int32_t *d_status,
-                   hipStream_t s, const Extra &x);
+int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
+                   int32_t precision, double *d_value, double *d_var, double *d_lag,
+                   int32_t *d_status, hipStream_t s, const Extra &x);
 
 // Global neighbourhood above 64 points: per-target substitutions with the
 // shared factors, in chunks whose scratch (b0 and the solution, N doubles
 // each per target) stays within 1 GiB.
-int32_t run_global(gs_kriging *K, const double *d_targets, int64_t T, double *d_value,
+int32_t run_global(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, double *d_value,
                    double *d_var, double *d_lag, int32_t *d_status, hipStream_t s,
                    const Extra &x) {
     const int64_t N = K->n + K->M;
@@ -1020,14 +1028,14 @@ int32_t run_global(gs_kriging *K, const double *d_targets, int64_t T, double *d_
     int64_t tc = ((int64_t)1 << 30) / (2 * N * (int64_t)sizeof(double));
     tc = std::max<int64_t>(64, tc / 64 * 64);
     tc = std::min<int64_t>(tc, (T + 63) / 64 * 64);
-    HIP_TRY(K->g_w.ensure(sizeof(double) * N * tc));
-    HIP_TRY(K->g_b0.ensure(sizeof(double) * N * tc));
+    HIP_TRY(W.g_w.ensure(sizeof(double) * N * tc));
+    HIP_TRY(W.g_b0.ensure(sizeof(double) * N * tc));
     gs::GlobalParams g = global_params(K);
     g.targets = d_targets;
     g.T = T;
     g.stride = tc;
-    g.w = K->g_w.as<double>();
-    g.b0 = K->g_b0.as<double>();
+    g.w = W.g_w.as<double>();
+    g.b0 = W.g_b0.as<double>();
     g.value = d_value;
     g.variance = d_var;
     g.lagrange = d_lag;
@@ -1044,44 +1052,45 @@ int32_t run_global(gs_kriging *K, const double *d_targets, int64_t T, double *d_
     return GS_OK;
 }
 
-int32_t run_predict(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
-                    double *d_value, double *d_var, double *d_lag, int32_t *d_status,
-                    hipStream_t s, const Extra &x = Extra()) {
+int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
+                    int32_t precision, double *d_value, double *d_var, double *d_lag,
+                    int32_t *d_status, hipStream_t s, const Extra &x = Extra(),
+                    bool timed = true) {
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (T == 0) return GS_OK;
     if (K->global) {
         if (precision != GS_PREC_FP64)
             return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
-        return run_global(K, d_targets, T, d_value, d_var, d_lag, d_status, s, x);
+        return run_global(K, W, d_targets, T, d_value, d_var, d_lag, d_status, s, x);
     }
     std::array<hipEvent_t, 3> ev{};
-    if (K->timing) {
+    if (timed && K->timing) {
         for (auto &e : ev) HIP_TRY(K->take_event(&e));
         HIP_TRY(hipEventRecord(ev[0], s));
     }
-    int32_t st = run_neighbors(K, d_targets, T, s);
+    int32_t st = run_neighbors(K, W, d_targets, T, s);
     if (st != GS_OK) return st;
-    if (K->timing) HIP_TRY(hipEventRecord(ev[1], s));
-    st = run_solves(K, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x);
+    if (timed && K->timing) HIP_TRY(hipEventRecord(ev[1], s));
+    st = run_solves(K, W, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x);
     if (st != GS_OK) return st;
-    if (K->timing) {
+    if (timed && K->timing) {
         HIP_TRY(hipEventRecord(ev[2], s));
         K->marks.push_back(ev);
     }
     return GS_OK;
 }
 
-int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t precision,
-                   double *d_value, double *d_var, double *d_lag, int32_t *d_status,
-                   hipStream_t s, const Extra &x) {
+int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
+                   int32_t precision, double *d_value, double *d_var, double *d_lag,
+                   int32_t *d_status, hipStream_t s, const Extra &x) {
     gs::SolveParams p;
     std::memset(&p, 0, sizeof(p));
     p.coords = K->coords.as<double>();
     p.values = K->values.as<double>();
     p.targets = d_targets;
-    p.nbr = K->nbr.as<int32_t>();
-    p.count = K->count.as<int32_t>();
+    p.nbr = W.nbr.as<int32_t>();
+    p.count = W.count.as<int32_t>();
     p.T = T;
     p.stride = stride_of(K);
     p.min_nb = K->min_nb;
@@ -1120,7 +1129,7 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
         int variant = 0;
         int32_t st = tuned_variant(key, s, [&](int v) {
             if (pp.list) {
-                int32_t *tune_len = K->lens.as<int32_t>() + 4;
+                int32_t *tune_len = W.lens.as<int32_t>() + 4;
                 hipError_t e = gs::launch_min_i32(pp.list_len, kTuneSystems, tune_len, s);
                 if (e != hipSuccess) return e;
                 tp.list_len = tune_len;
@@ -1139,17 +1148,17 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
     };
     if (K->per_oct > 0 && K->nb_bound > 16) {
         // Uneven sizes: bucket by padded system size, one launch per bucket.
-        HIP_TRY(K->lists.ensure(sizeof(int32_t) * 3 * T));
-        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 8));
-        HIP_TRY(gs::launch_zero_i32(K->lens.as<int32_t>(), 4, s));
-        HIP_TRY(gs::launch_bucket(K->count.as<int32_t>(), T, K->lists.as<int32_t>(),
-                                  K->lens.as<int32_t>(), T, s));
+        HIP_TRY(W.lists.ensure(sizeof(int32_t) * 3 * T));
+        HIP_TRY(W.lens.ensure(sizeof(int32_t) * 8));
+        HIP_TRY(gs::launch_zero_i32(W.lens.as<int32_t>(), 4, s));
+        HIP_TRY(gs::launch_bucket(W.count.as<int32_t>(), T, W.lists.as<int32_t>(),
+                                  W.lens.as<int32_t>(), T, s));
         const int nbs[3] = {16, 32, 64};
         for (int b = 0; b < 3; b++) {
             if (nbs[b] > pad_nb(K->nb_bound)) break;
             gs::SolveParams pb = p;
-            pb.list = K->lists.as<int32_t>() + (int64_t)b * T;
-            pb.list_len = K->lens.as<int32_t>() + b;
+            pb.list = W.lists.as<int32_t>() + (int64_t)b * T;
+            pb.list_len = W.lens.as<int32_t>() + b;
             // persistent grid: enough groups to fill the device, grid-strided
             int64_t groups = std::min<int64_t>(T, (int64_t)256 * 64 * 4);
             int32_t st = solve(nbs[b], pb, groups);
@@ -1160,10 +1169,10 @@ int32_t run_solves(gs_kriging *K, const double *d_targets, int64_t T, int32_t pr
     if (K->has_tree) {
         // process targets in the search's Hilbert order: neighbouring systems
         // share data points, so the coordinate gathers hit cache
-        HIP_TRY(K->lens.ensure(sizeof(int32_t) * 8));
-        HIP_TRY(gs::launch_fill_i32(K->lens.as<int32_t>() + 3, 1, (int32_t)T, s));
-        p.list = K->perm.as<int32_t>();
-        p.list_len = K->lens.as<int32_t>() + 3;
+        HIP_TRY(W.lens.ensure(sizeof(int32_t) * 8));
+        HIP_TRY(gs::launch_fill_i32(W.lens.as<int32_t>() + 3, 1, (int32_t)T, s));
+        p.list = W.perm.as<int32_t>();
+        p.list_len = W.lens.as<int32_t>() + 3;
     }
     return solve(pad_nb(K->nb_bound), p, T);
 }
@@ -1192,7 +1201,8 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
         if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INVALID_PARAMETER, s));
         return GS_OK;
     }
-    return run_predict(k, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status, s);
+    return run_predict(k, k->w, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status,
+                       s);
 }
 
 }  // extern "C"
@@ -1215,30 +1225,30 @@ int32_t fail_all_host(int64_t T, double *value, double *variance, double *lagran
 
 // Host-buffer predict: uploads the targets (and the per-target covariates
 // when tdrift is given), runs the pass, downloads the estimates.
-int32_t predict_host(gs_kriging *k, const double *targets, int64_t T, int32_t precision,
+int32_t predict_host(gs_kriging *k, KWork &W, const double *targets, int64_t T, int32_t precision,
                      double *value, double *variance, double *lagrange, int32_t *status,
-                     const double *tdrift, Extra x) {
-    hipStream_t s = k->ctx->stream;
-    HIP_TRY(k->h_targets.ensure(sizeof(double) * T * k->D));
-    HIP_TRY(k->h_value.ensure(sizeof(double) * T));
-    HIP_TRY(k->h_var.ensure(sizeof(double) * T));
-    HIP_TRY(k->h_lag.ensure(sizeof(double) * T));
-    HIP_TRY(k->h_status.ensure(sizeof(int32_t) * T));
-    HIP_TRY(hipMemcpyAsync(k->h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
+                     const double *tdrift, Extra x, hipStream_t s = nullptr, bool timed = true) {
+    if (!s) s = k->ctx->stream;
+    HIP_TRY(W.h_targets.ensure(sizeof(double) * T * k->D));
+    HIP_TRY(W.h_value.ensure(sizeof(double) * T));
+    HIP_TRY(W.h_var.ensure(sizeof(double) * T));
+    HIP_TRY(W.h_lag.ensure(sizeof(double) * T));
+    HIP_TRY(W.h_status.ensure(sizeof(int32_t) * T));
+    HIP_TRY(hipMemcpyAsync(W.h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
     if (tdrift) {
-        HIP_TRY(k->h_tdrift.ensure(sizeof(double) * T * k->nv));
-        HIP_TRY(hipMemcpyAsync(k->h_tdrift.p, tdrift, sizeof(double) * T * k->nv,
+        HIP_TRY(W.h_tdrift.ensure(sizeof(double) * T * k->nv));
+        HIP_TRY(hipMemcpyAsync(W.h_tdrift.p, tdrift, sizeof(double) * T * k->nv,
                                hipMemcpyHostToDevice, s));
-        x.tdrift = k->h_tdrift.as<double>();
+        x.tdrift = W.h_tdrift.as<double>();
     }
-    int32_t st = run_predict(k, k->h_targets.as<double>(), T, precision, k->h_value.as<double>(),
-                             k->h_var.as<double>(), k->h_lag.as<double>(), k->h_status.as<int32_t>(),
-                             s, x);
+    int32_t st = run_predict(k, W, W.h_targets.as<double>(), T, precision, W.h_value.as<double>(),
+                             W.h_var.as<double>(), W.h_lag.as<double>(), W.h_status.as<int32_t>(),
+                             s, x, timed);
     if (st != GS_OK) return st;
-    HIP_TRY(hipMemcpyAsync(value, k->h_value.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(variance, k->h_var.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
-    if (lagrange) HIP_TRY(hipMemcpyAsync(lagrange, k->h_lag.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
-    if (status) HIP_TRY(hipMemcpyAsync(status, k->h_status.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(value, W.h_value.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(variance, W.h_var.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+    if (lagrange) HIP_TRY(hipMemcpyAsync(lagrange, W.h_lag.p, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+    if (status) HIP_TRY(hipMemcpyAsync(status, W.h_status.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return GS_OK;
 }
@@ -1259,7 +1269,7 @@ int32_t gs_kriging_predict_many(gs_kriging *k, const double *targets, int64_t T,
     HIP_TRY(hipSetDevice(k->ctx->device));
     if (k->nv > 0)   // predict needs predict_with_drift here (kriging.rs:494-501)
         return fail_all_host(T, value, variance, lagrange, status, GS_ERR_INVALID_PARAMETER);
-    return predict_host(k, targets, T, precision, value, variance, lagrange, status, nullptr,
+    return predict_host(k, k->w, targets, T, precision, value, variance, lagrange, status, nullptr,
                         Extra());
 }
 
@@ -1277,7 +1287,7 @@ int32_t gs_kriging_predict_many_with_drift(gs_kriging *k, const double *targets,
     // predict_with_drift on another method fails per target (kriging.rs:507-512)
     if (k->nv == 0) return fail_all_host(T, value, variance, lagrange, status, GS_ERR_INVALID_PARAMETER);
     if (!drifts) return fail(GS_ERR_DIMENSION_MISMATCH, "no drift rows for the targets");
-    return predict_host(k, targets, T, precision, value, variance, lagrange, status, drifts,
+    return predict_host(k, k->w, targets, T, precision, value, variance, lagrange, status, drifts,
                         Extra());
 }
 
@@ -1303,16 +1313,17 @@ int32_t gs_kriging_predict_block_many(gs_kriging *k, const double *centers, int6
         if (!std::isfinite(offsets[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite block offset");
     hipStream_t s = k->ctx->stream;
     const size_t ob = sizeof(double) * (size_t)n_off * k->D;
-    HIP_TRY(k->blk.ensure(ob + 64));
-    double *d_offs = k->blk.as<double>();
-    double *d_cbb = reinterpret_cast<double *>(static_cast<char *>(k->blk.p) + ob);
+    HIP_TRY(k->w.blk.ensure(ob + 64));
+    double *d_offs = k->w.blk.as<double>();
+    double *d_cbb = reinterpret_cast<double *>(static_cast<char *>(k->w.blk.p) + ob);
     HIP_TRY(hipMemcpyAsync(d_offs, offsets, ob, hipMemcpyHostToDevice, s));
     HIP_TRY(gs::launch_block_cbb(k->D, k->lean, k->dmodel.as<DevModel>(), d_offs, n_off, d_cbb, s));
     Extra x;
     x.offsets = d_offs;
     x.n_off = n_off;
     x.cbb = d_cbb;
-    return predict_host(k, centers, T, precision, value, variance, lagrange, status, nullptr, x);
+    return predict_host(k, k->w, centers, T, precision, value, variance, lagrange, status, nullptr,
+                        x);
 }
 
 int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, int64_t max_out,
@@ -1333,14 +1344,14 @@ int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, in
     std::lock_guard<std::mutex> lk(k->mu);
     HIP_TRY(hipSetDevice(k->ctx->device));
     hipStream_t s = k->ctx->stream;
-    HIP_TRY(k->h_targets.ensure(sizeof(double) * T * k->D));
-    HIP_TRY(hipMemcpyAsync(k->h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
-    int32_t st = run_neighbors(k, k->h_targets.as<double>(), T, s);
+    HIP_TRY(k->w.h_targets.ensure(sizeof(double) * T * k->D));
+    HIP_TRY(hipMemcpyAsync(k->w.h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
+    int32_t st = run_neighbors(k, k->w, k->w.h_targets.as<double>(), T, s);
     if (st != GS_OK) return st;
     const int32_t stride = stride_of(k);
     std::vector<int32_t> tmp((size_t)(T * stride));
-    HIP_TRY(hipMemcpyAsync(tmp.data(), k->nbr.p, sizeof(int32_t) * T * stride, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(count, k->count.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), k->w.nbr.p, sizeof(int32_t) * T * stride, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(count, k->w.count.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int64_t t = 0; t < T; t++)
         for (int64_t j = 0; j < max_out; j++)
@@ -1436,7 +1447,7 @@ int32_t cv_with_folds(gs_ctx *ctx, int32_t dim, const double *coords, const doub
         {
             std::lock_guard<std::mutex> lk(K->mu);
             HIP_TRY(hipSetDevice(ctx->device));
-            st = predict_host(K, mc.data(), T, GS_PREC_FP64, ov.data(), ovar.data(), olag.data(),
+            st = predict_host(K, K->w, mc.data(), T, GS_PREC_FP64, ov.data(), ovar.data(), olag.data(),
                               ost.data(), nv > 0 ? md.data() : nullptr, Extra());
         }
         if (st != GS_OK) return st;
@@ -2005,8 +2016,13 @@ int32_t gs_vecchia_plan_get(const gs_vplan *p, int64_t *order, int32_t *nb, int3
     return GS_OK;
 }
 
-int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_model *models,
-                                int32_t n_models, int32_t precision, double *out, int32_t *status) {
+}  // extern "C"
+
+namespace {
+
+int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
+                      int32_t n_models, int32_t precision, double *out, int32_t *status,
+                      hipStream_t stream) {
     if (!p) return fail(GS_ERR_INVALID_PARAMETER, "plan is NULL");
     if (n_models < 0) return fail(GS_ERR_INVALID_PARAMETER, "n_models < 0");
     if (n_models == 0) return GS_OK;
@@ -2024,7 +2040,7 @@ int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_mode
     }
     std::lock_guard<std::mutex> lk(p->mu);
     HIP_TRY(hipSetDevice(p->ctx->device));
-    hipStream_t s = p->ctx->stream;
+    hipStream_t s = stream ? stream : p->ctx->stream;
     // z = v - mean (vecchia.rs:290-292), mean as a left-to-right sum.
     double mean = -0.0;
     for (int64_t i = 0; i < n; i++) mean += values[i];
@@ -2085,6 +2101,15 @@ int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_mode
         if (status) status[i] = st[(size_t)i];
     }
     return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gs_vecchia_loglik_batch(gs_vplan *p, const double *values, const gs_model *models,
+                                int32_t n_models, int32_t precision, double *out, int32_t *status) {
+    return vecchia_batch(p, values, models, n_models, precision, out, status, nullptr);
 }
 
 int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs_model *models,
@@ -2200,6 +2225,250 @@ int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const 
     gs_vecchia_plan_free(p);
     if (st != GS_OK) return st;
     if (s1 != 0) return fail(s1, "singular conditioning system");
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Request loop: kriging and Vecchia-likelihood requests over resident
+// predictors and plans.  Each worker slot owns a HIP stream and a kriging
+// workspace; a request is admitted, first come first served, when a slot is
+// free and the device bytes it needs fit in the budget beside the requests
+// already running (a request larger than the whole budget runs alone).
+// Requests on different slots run concurrently, so their kernels overlap on
+// the device.  Every kernel on the path is deterministic and independent of
+// its launch grid, so a request's result is the same alone and in the mix.
+// ---------------------------------------------------------------------------
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct SReq {
+    int kind = 0;   // 0 kriging, 1 Vecchia likelihood
+    gs_kriging *k = nullptr;
+    const double *targets = nullptr;
+    int64_t T = 0;
+    double *value = nullptr, *variance = nullptr, *lagrange = nullptr;
+    int32_t *status = nullptr;
+    gs_vplan *p = nullptr;
+    const double *values = nullptr;
+    const gs_model *models = nullptr;
+    int32_t n_models = 0;
+    double *out = nullptr;
+    int32_t *vstatus = nullptr;
+    int64_t bytes = 0;
+    Clock::time_point t_submit, t_start, t_end;
+    int32_t result = GS_OK;
+    std::string error;
+    bool done = false;
+};
+
+// Device bytes a kriging request needs beyond the resident data: the
+// workspace and staging of predict_host (neighbour lists, search order and
+// sort scratch, outputs) or the global scratch.
+int64_t krige_bytes(const gs_kriging *k, int64_t T) {
+    if (k->global) {
+        const int64_t N = k->n + k->M;
+        int64_t tc = ((int64_t)1 << 30) / (2 * N * (int64_t)sizeof(double));
+        tc = std::max<int64_t>(64, tc / 64 * 64);
+        tc = std::min<int64_t>(tc, (T + 63) / 64 * 64);
+        return T * (k->D * 8 + 36) + 2 * N * tc * 8;
+    }
+    const int64_t stride = std::max<int64_t>(k->stride, 1);
+    return T * (k->D * 8 + 36 + stride * 4 + 4 + 16 + 8 + 4 + 16 + (k->per_oct > 0 ? 12 : 0)) +
+           (int64_t)gs::tile_sort_workspace(T);
+}
+
+int64_t vecchia_bytes(const gs_vplan *p, int32_t n_models) {
+    const int64_t n_blocks = (p->n + 63) / 64;
+    return p->n * 8 + (int64_t)n_models * (int64_t)(sizeof(DevModel) + 12) +
+           n_blocks * gs::kMaxModelsPerLaunch * 8;
+}
+
+}  // namespace
+
+struct gs_server {
+    gs_ctx *ctx = nullptr;
+    int64_t budget = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<SReq>> queue;
+    std::map<int64_t, std::shared_ptr<SReq>> tickets;
+    int64_t next_ticket = 1;
+    int64_t in_flight_bytes = 0;
+    int running = 0;
+    bool stopping = false;
+    struct Slot {
+        hipStream_t stream = nullptr;
+        KWork w;
+    };
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<std::thread> threads;
+
+    void worker(Slot *slot) {
+        (void)hipSetDevice(ctx->device);
+        for (;;) {
+            std::shared_ptr<SReq> r;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] {
+                    if (stopping && queue.empty()) return true;
+                    if (queue.empty()) return false;
+                    const int64_t b = queue.front()->bytes;
+                    return running == 0 || in_flight_bytes + b <= budget;
+                });
+                if (queue.empty()) return;   // stopping
+                r = queue.front();
+                queue.pop_front();
+                in_flight_bytes += r->bytes;
+                running++;
+                r->t_start = Clock::now();
+            }
+            int32_t st;
+            if (r->kind == 0)
+                st = predict_host(r->k, slot->w, r->targets, r->T, GS_PREC_FP64, r->value,
+                                  r->variance, r->lagrange, r->status, nullptr, Extra(),
+                                  slot->stream, false);
+            else
+                st = vecchia_batch(r->p, r->values, r->models, r->n_models, GS_PREC_FP64, r->out,
+                                   r->vstatus, slot->stream);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                r->t_end = Clock::now();
+                r->result = st;
+                if (st != GS_OK) r->error = g_err;
+                r->done = true;
+                in_flight_bytes -= r->bytes;
+                running--;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int32_t gs_server_create(gs_ctx *ctx, int32_t n_slots, int64_t mem_budget, gs_server **out) {
+    if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
+    *out = nullptr;
+    if (!ctx) return fail(GS_ERR_INVALID_PARAMETER, "ctx is NULL");
+    if (n_slots < 1 || n_slots > 16) return fail(GS_ERR_INVALID_PARAMETER, "n_slots must be 1..16");
+    if (mem_budget <= 0) return fail(GS_ERR_INVALID_PARAMETER, "mem_budget must be positive");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<gs_server> S(new gs_server());
+    S->ctx = ctx;
+    S->budget = mem_budget;
+    for (int i = 0; i < n_slots; i++) {
+        std::unique_ptr<gs_server::Slot> sl(new gs_server::Slot());
+        HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+        S->slots.push_back(std::move(sl));
+    }
+    for (auto &sl : S->slots) {
+        gs_server::Slot *raw = sl.get();
+        gs_server *srv = S.get();
+        S->threads.emplace_back([srv, raw] { srv->worker(raw); });
+    }
+    *out = S.release();
+    return GS_OK;
+}
+
+void gs_server_destroy(gs_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->threads) t.join();
+    (void)hipSetDevice(s->ctx->device);
+    for (auto &sl : s->slots)
+        if (sl->stream) (void)hipStreamDestroy(sl->stream);
+    delete s;
+}
+
+int32_t gs_server_submit_krige(gs_server *s, gs_kriging *k, const double *targets, int64_t T,
+                               double *value, double *variance, double *lagrange,
+                               int32_t *status, int64_t *ticket) {
+    if (!s || !k || !ticket) return fail(GS_ERR_INVALID_PARAMETER, "NULL argument");
+    if (T <= 0 || !targets || !value || !variance)
+        return fail(GS_ERR_INVALID_PARAMETER, "empty request or NULL buffer");
+    if (k->ctx != s->ctx) return fail(GS_ERR_INVALID_PARAMETER, "predictor belongs to another context");
+    if (k->nv > 0)
+        return fail(GS_ERR_INVALID_PARAMETER, "unsupported in the request loop: external drift");
+    auto r = std::make_shared<SReq>();
+    r->kind = 0;
+    r->k = k;
+    r->targets = targets;
+    r->T = T;
+    r->value = value;
+    r->variance = variance;
+    r->lagrange = lagrange;
+    r->status = status;
+    r->bytes = krige_bytes(k, T);
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping) return fail(GS_ERR_INVALID_PARAMETER, "server is stopping");
+        *ticket = s->next_ticket++;
+        s->tickets[*ticket] = r;
+        s->queue.push_back(r);
+    }
+    s->cv_work.notify_all();
+    return GS_OK;
+}
+
+int32_t gs_server_submit_vecchia(gs_server *s, gs_vplan *p, const double *values,
+                                 const gs_model *models, int32_t n_models, double *out,
+                                 int32_t *status, int64_t *ticket) {
+    if (!s || !p || !ticket) return fail(GS_ERR_INVALID_PARAMETER, "NULL argument");
+    if (n_models <= 0 || !values || !models || !out)
+        return fail(GS_ERR_INVALID_PARAMETER, "empty request or NULL buffer");
+    if (p->ctx != s->ctx) return fail(GS_ERR_INVALID_PARAMETER, "plan belongs to another context");
+    auto r = std::make_shared<SReq>();
+    r->kind = 1;
+    r->p = p;
+    r->values = values;
+    r->models = models;
+    r->n_models = n_models;
+    r->out = out;
+    r->vstatus = status;
+    r->bytes = vecchia_bytes(p, n_models);
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping) return fail(GS_ERR_INVALID_PARAMETER, "server is stopping");
+        *ticket = s->next_ticket++;
+        s->tickets[*ticket] = r;
+        s->queue.push_back(r);
+    }
+    s->cv_work.notify_all();
+    return GS_OK;
+}
+
+int32_t gs_server_wait(gs_server *s, int64_t ticket, double *queue_ms, double *service_ms) {
+    if (!s) return fail(GS_ERR_INVALID_PARAMETER, "server is NULL");
+    std::shared_ptr<SReq> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->tickets.find(ticket);
+        if (it == s->tickets.end()) return fail(GS_ERR_INVALID_PARAMETER, "unknown ticket");
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->done; });
+        s->tickets.erase(ticket);
+    }
+    if (queue_ms)
+        *queue_ms = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
+    if (service_ms)
+        *service_ms = std::chrono::duration<double, std::milli>(r->t_end - r->t_start).count();
+    if (r->result != GS_OK) return fail(r->result, r->error);
     return GS_OK;
 }
 

@@ -34,7 +34,7 @@ __all__ = [
     "InvalidParameter", "NoNeighbors", "DuplicatePoints", "DeviceError",
     "ModelKind", "Anisotropy", "Structure", "VariogramModel", "KrigingMethod",
     "KrigingConfig", "KrigingEstimate", "Context", "Kriging", "Grid2D", "Grid3D",
-    "block_offsets", "VecchiaPlan", "VecchiaFit", "vecchia_plan", "vecchia_loglik", "vecchia_mle", "PREC_FP64",
+    "block_offsets", "Server", "Ticket", "VecchiaPlan", "VecchiaFit", "vecchia_plan", "vecchia_loglik", "vecchia_mle", "PREC_FP64",
     "PREC_FP32_REFINED",
 ]
 
@@ -773,3 +773,71 @@ def vecchia_loglik(coords, values, model: VariogramModel, m: int, order=None,
                                         _abi.dptr(values), coords.shape[0], C.byref(cm), int(m),
                                         None if o is None else _abi.i64ptr(o), C.byref(out)))
     return float(out.value)
+
+
+# ----------------------------------------------------------- request loop --
+class Ticket:
+    """A submitted request: holds its buffers until Server.wait returns."""
+
+    def __init__(self, tid: int, kind: str, keep: tuple, outputs: tuple):
+        self.id = tid
+        self.kind = kind
+        self._keep = keep
+        self.outputs = outputs
+        self.queue_ms = None
+        self.service_ms = None
+
+
+class Server:
+    """Request loop over resident predictors and plans (gs_server_*):
+    n_slots concurrent slots, admission by device-memory footprint."""
+
+    def __init__(self, n_slots: int = 4, mem_budget: int = 16 << 30,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or Context.default()
+        h = C.c_void_p()
+        _check(_abi.lib().gs_server_create(self.ctx._h, int(n_slots), int(mem_budget),
+                                           C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h and _abi._lib is not None:
+            _abi.lib().gs_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit_krige(self, kriging: "Kriging", targets) -> Ticket:
+        t = _f64(targets).reshape(-1, kriging.dim)
+        T = t.shape[0]
+        v, var, lag = np.empty(T), np.empty(T), np.empty(T)
+        st = np.empty(T, dtype=np.int32)
+        tid = C.c_int64()
+        _check(_abi.lib().gs_server_submit_krige(self._h, kriging._h, _abi.dptr(t), T,
+                                                 _abi.dptr(v), _abi.dptr(var), _abi.dptr(lag),
+                                                 _abi.i32ptr(st), C.byref(tid)))
+        return Ticket(tid.value, "krige", (kriging, t), (v, var, lag, st))
+
+    def submit_vecchia(self, plan: "VecchiaPlan", values, models) -> Ticket:
+        v = _f64(values).reshape(-1)
+        arr = (_abi.gs_model * len(models))(*[mm._c() for mm in models])
+        out = np.empty(len(models))
+        st = np.empty(len(models), dtype=np.int32)
+        tid = C.c_int64()
+        _check(_abi.lib().gs_server_submit_vecchia(self._h, plan._h, _abi.dptr(v), arr,
+                                                   len(models), _abi.dptr(out),
+                                                   _abi.i32ptr(st), C.byref(tid)))
+        return Ticket(tid.value, "vecchia", (plan, v, arr), (out, st))
+
+    def wait(self, ticket: Ticket):
+        """Blocks until the request is done; returns its outputs ((value,
+        variance, lagrange, status) or (loglik, status)) and records
+        ticket.queue_ms / ticket.service_ms."""
+        q, sv = C.c_double(), C.c_double()
+        rc = _abi.lib().gs_server_wait(self._h, ticket.id, C.byref(q), C.byref(sv))
+        ticket.queue_ms, ticket.service_ms = q.value, sv.value
+        ticket._keep = None
+        _check(rc)
+        return ticket.outputs

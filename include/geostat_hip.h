@@ -287,6 +287,31 @@ int32_t gs_vecchia_loglik(gs_ctx *ctx, int32_t dim, const double *coords, const 
 int32_t gs_vecchia_mle(gs_vplan *p, const double *values, int32_t kind, double param,
                        gs_model *fit, double *loglik);
 
+/* ---- request loop ------------------------------------------------- */
+/* Serving: kriging and Vecchia-likelihood requests over predictors and
+ * plans that stay resident in HBM (no reference counterpart; the
+ * reference's callers run one request at a time).  n_slots worker slots
+ * (1..16), each with its own stream and workspace; requests are admitted
+ * first come first served when a slot is free and their device bytes fit
+ * in mem_budget beside the running ones, and run concurrently.  Results
+ * are identical to the same request made alone (deterministic kernels).
+ * The caller keeps a request's buffers alive until gs_server_wait returns
+ * for its ticket; destroy drains the queue. */
+typedef struct gs_server gs_server;
+int32_t gs_server_create(gs_ctx *ctx, int32_t n_slots, int64_t mem_budget, gs_server **out);
+void gs_server_destroy(gs_server *s);
+/* A predict_many request on a handle (not external drift), FP64. */
+int32_t gs_server_submit_krige(gs_server *s, gs_kriging *k, const double *targets, int64_t T,
+                               double *value, double *variance, double *lagrange,
+                               int32_t *status, int64_t *ticket);
+/* A gs_vecchia_loglik_batch request on a plan, FP64. */
+int32_t gs_server_submit_vecchia(gs_server *s, gs_vplan *p, const double *values,
+                                 const gs_model *models, int32_t n_models, double *out,
+                                 int32_t *status, int64_t *ticket);
+/* Waits for a ticket; returns the request's status; queue_ms / service_ms
+ * (may be NULL) receive its time in the queue and on a slot. */
+int32_t gs_server_wait(gs_server *s, int64_t ticket, double *queue_ms, double *service_ms);
+
 #ifdef __cplusplus
 }
 #endif
