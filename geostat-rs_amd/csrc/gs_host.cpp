@@ -1961,19 +1961,64 @@ int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int6
     G.build_all(opts.data(), n);
     P->nb.assign((size_t)(n * m), -1);
     P->nb_count.assign((size_t)n, 0);
+    // canonical predecessors on the GPU; ties at the m-th distance are
+    // replayed here in the reference's visiting order
+    std::vector<int32_t> gpos((size_t)(n * m)), gflag((size_t)n);
+    {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        DBuf d_pts, d_start, d_items, d_out, d_cnt, d_flag;
+        HIP_TRY(d_pts.ensure(sizeof(double) * n * dim));
+        HIP_TRY(d_start.ensure(sizeof(int64_t) * G.start.size()));
+        HIP_TRY(d_items.ensure(sizeof(int32_t) * n));
+        HIP_TRY(d_out.ensure(sizeof(int32_t) * n * m));
+        HIP_TRY(d_cnt.ensure(sizeof(int32_t) * n));
+        HIP_TRY(d_flag.ensure(sizeof(int32_t) * n));
+        HIP_TRY(hipMemcpyAsync(d_pts.p, opts.data(), sizeof(double) * n * dim, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_start.p, G.start.data(), sizeof(int64_t) * G.start.size(),
+                               hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_items.p, G.items.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        gs::PredParams pp;
+        std::memset(&pp, 0, sizeof(pp));
+        pp.pts = d_pts.as<double>();
+        pp.n = n;
+        pp.m = (int32_t)m;
+        pp.brute = 1024;
+        for (int d = 0; d < 3; d++) {
+            pp.origin[d] = d < dim ? G.origin[d] : 0.0;
+            pp.nn[d] = d < dim ? G.nn[d] : 1;
+            pp.strides[d] = d < dim ? G.strides[d] : 0;
+        }
+        pp.cell = G.cell;
+        pp.start = d_start.as<int64_t>();
+        pp.items = d_items.as<int32_t>();
+        pp.out = d_out.as<int32_t>();
+        pp.count = d_cnt.as<int32_t>();
+        pp.flag = d_flag.as<int32_t>();
+        HIP_TRY(gs::launch_pred_knn(dim, pp, s));
+        HIP_TRY(hipMemcpyAsync(gpos.data(), d_out.p, sizeof(int32_t) * n * m, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(P->nb_count.data(), d_cnt.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(gflag.data(), d_flag.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     std::atomic<int64_t> next(0);
     auto worker = [&]() {
         KHeap h;
         std::vector<uint32_t> res;
         for (;;) {
-            int64_t s = next.fetch_add(256);
+            int64_t s = next.fetch_add(4096);
             if (s >= n) break;
-            int64_t e = std::min(n, s + 256);
+            int64_t e = std::min(n, s + 4096);
             for (int64_t k = s; k < e; k++) {
-                grid_knn(G, &opts[(size_t)(k * dim)], m, k, h, res);
-                P->nb_count[(size_t)k] = (int32_t)res.size();
-                for (size_t q = 0; q < res.size(); q++)
-                    P->nb[(size_t)(k * m) + q] = (int32_t)P->order[res[q]];
+                int32_t *row = &P->nb[(size_t)(k * m)];
+                if (gflag[(size_t)k]) {
+                    grid_knn(G, &opts[(size_t)(k * dim)], m, k, h, res);
+                    P->nb_count[(size_t)k] = (int32_t)res.size();
+                    for (size_t q = 0; q < res.size(); q++) row[q] = (int32_t)P->order[res[q]];
+                } else {
+                    for (int32_t q = 0; q < P->nb_count[(size_t)k]; q++)
+                        row[q] = (int32_t)P->order[(size_t)gpos[(size_t)(k * m + q)]];
+                }
             }
         }
     };

@@ -142,6 +142,25 @@ struct GlobalParams {
 hipError_t launch_global_factor(int D, bool lean, GlobalParams p, double *d_scale, hipStream_t s);
 hipError_t launch_global_solve(int D, bool lean, const GlobalParams &p, hipStream_t s);
 
+// Nearest predecessors of the Vecchia plan (gs_plan.hip): canonical m
+// nearest among positions < k for every k, over the reference's bucket grid.
+struct PredParams {
+    const double *pts;        // [n][D] coordinates in ordered position
+    int64_t n;
+    int32_t m;
+    int64_t brute;            // positions k <= brute scan all predecessors
+    double origin[3];
+    double cell;
+    int64_t nn[3], strides[3];
+    const int64_t *start;     // [buckets + 1] CSR bucket offsets
+    const int32_t *items;     // positions, ascending inside each bucket
+    int32_t *out;             // [n][m] positions, ascending (d2, position), -1 padded
+    int32_t *count;           // [n]
+    int32_t *flag;            // [n] 1: a tie at the m-th distance (replay on the host)
+};
+
+hipError_t launch_pred_knn(int D, const PredParams &p, hipStream_t s);
+
 struct VecchiaParams {
     const double *coords;    // [n][D]
     const double *zc;        // [n] centred values

@@ -46,6 +46,17 @@ def test_plan_with_ties_and_given_order(oracle):
     plan_equal(oracle, c, 7, order)
 
 
+@pytest.mark.parametrize("m", [5, 12, 30])
+def test_plan_lattice_beyond_brute_force_prefix(oracle, m):
+    # 3000 lattice points: positions past the GPU's all-predecessor prefix
+    # go through the bucket-grid rings, with many ties at the m-th distance
+    # (replayed in the reference's visiting order)
+    c, _ = cs.lattice(60, 50)
+    plan_equal(oracle, c, m)
+    order = np.random.default_rng(m).permutation(len(c))
+    plan_equal(oracle, c, m, order)
+
+
 def test_plan_3d(oracle):
     c, v = workloads.points(63, 2500, [100.0, 100.0, 30.0], recipe=1)
     p, o, nb, cnt = plan_equal(oracle, c, 20)
