@@ -1821,39 +1821,52 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
     BGrid G;                               // ~1 point per bucket
     G.init(D, mn, mx, n);
     G.build_all(coords, n);
+    // Everything below lives in bucket order (slot q holds point items[q]),
+    // so a range query reads contiguous coordinates, keys and flags.  Keys
+    // are sums of squares (never -0.0 or NaN), for which total_cmp is the
+    // plain numeric order.
+    const std::vector<uint32_t> &id = G.items;
+    std::vector<double> cc((size_t)(n * D));
+    std::vector<uint32_t> slot((size_t)n);
+    for (int64_t q = 0; q < n; q++) {
+        slot[id[(size_t)q]] = (uint32_t)q;
+        for (int d = 0; d < D; d++) cc[(size_t)(q * D + d)] = coords[(int64_t)id[(size_t)q] * D + d];
+    }
     std::vector<double> key((size_t)n);    // min squared distance to the selected set
     std::vector<char> taken((size_t)n, 0);
-    taken[(size_t)first] = 1;
-    for (int64_t i = 0; i < n; i++) key[(size_t)i] = dist2(coords + i * D, coords + first * D, D);
-    // indexed binary max-heap over the remaining points, ordered by
-    // (total_key(key), index) as MaxminKey (vecchia.rs:114-125)
+    const uint32_t qf = slot[(size_t)first];
+    taken[qf] = 1;
+    for (int64_t q = 0; q < n; q++)
+        key[(size_t)q] = dist2(&cc[(size_t)(q * D)], coords + first * D, D);
+    // indexed binary max-heap over the remaining slots, ordered by
+    // (key, original index) as MaxminKey (vecchia.rs:114-125)
     std::vector<uint32_t> heap;
     std::vector<int64_t> where((size_t)n, -1);
     heap.reserve((size_t)n);
     auto above = [&](uint32_t a, uint32_t b) {
-        const int64_t ka = total_key(key[a]), kb = total_key(key[b]);
-        return ka > kb || (ka == kb && a > b);
+        const double ka = key[a], kb = key[b];
+        return ka > kb || (ka == kb && id[a] > id[b]);
     };
-    auto place = [&](size_t e, uint32_t id) {
-        heap[e] = id;
-        where[id] = (int64_t)e;
+    auto place = [&](size_t e, uint32_t q) {
+        heap[e] = q;
+        where[q] = (int64_t)e;
     };
     auto sift_down = [&](size_t e) {
-        const uint32_t id = heap[e];
+        const uint32_t q = heap[e];
         const size_t sz = heap.size();
         for (;;) {
             size_t l = 2 * e + 1;
             if (l >= sz) break;
             size_t b = l;
             if (l + 1 < sz && above(heap[l + 1], heap[l])) b = l + 1;
-            if (!above(heap[b], id)) break;
+            if (!above(heap[b], q)) break;
             place(e, heap[b]);
             e = b;
         }
-        place(e, id);
+        place(e, q);
     };
-    for (int64_t i = 0; i < n; i++)
-        if (i != first) { where[(size_t)i] = (int64_t)heap.size(); heap.push_back((uint32_t)i); }
+    for (int64_t q = 0; q < n; q++)
+        if ((uint32_t)q != qf) { where[(size_t)q] = (int64_t)heap.size(); heap.push_back((uint32_t)q); }
     for (size_t e = heap.size() / 2; e-- > 0;) sift_down(e);
     while (!heap.empty()) {
         const uint32_t p = heap[0];
@@ -1865,11 +1878,11 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
             sift_down(0);
         }
         taken[p] = 1;
-        order.push_back((int64_t)p);
+        order.push_back((int64_t)id[p]);
         if (heap.empty()) break;
         // points whose key can drop lie within sqrt(key[p]) of p (every
         // remaining key is <= key[p]); the box is widened for rounding
-        const double *pp = coords + (int64_t)p * D;
+        const double *pp = &cc[(size_t)p * D];
         const double rad = std::sqrt(key[p]) * (1.0 + 1e-9) + 1e-300;
         int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         for (int d = 0; d < D; d++) {
@@ -1880,17 +1893,16 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
         }
         int64_t cur[3] = {lo[0], lo[1], lo[2]};
         for (;;) {
-            // one x-run of buckets is one contiguous CSR range
+            // one x-run of buckets is one contiguous slot range
             int64_t b0 = 0;
             for (int d = 1; d < D; d++) b0 += cur[d] * G.strides[d];
             const int64_t q0 = G.start[(size_t)(b0 + lo[0])], q1 = G.start[(size_t)(b0 + hi[0]) + 1];
             for (int64_t q = q0; q < q1; q++) {
-                const uint32_t i = G.items[(size_t)q];
-                if (taken[i]) continue;
-                const double d2 = dist2(coords + (int64_t)i * D, pp, D);
-                if (total_key(d2) < total_key(key[i])) {
-                    key[i] = d2;
-                    sift_down((size_t)where[i]);
+                if (taken[(size_t)q]) continue;
+                const double d2 = dist2(&cc[(size_t)(q * D)], pp, D);
+                if (d2 < key[(size_t)q]) {
+                    key[(size_t)q] = d2;
+                    sift_down((size_t)where[(size_t)q]);
                 }
             }
             int d = 1;
