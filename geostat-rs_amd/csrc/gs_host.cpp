@@ -1151,8 +1151,8 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         HIP_TRY(W.lists.ensure(sizeof(int32_t) * 3 * T));
         HIP_TRY(W.lens.ensure(sizeof(int32_t) * 8));
         HIP_TRY(gs::launch_zero_i32(W.lens.as<int32_t>(), 4, s));
-        HIP_TRY(gs::launch_bucket(W.count.as<int32_t>(), T, W.lists.as<int32_t>(),
-                                  W.lens.as<int32_t>(), T, s));
+        HIP_TRY(gs::launch_bucket(W.count.as<int32_t>(), K->has_tree ? W.perm.as<int32_t>() : nullptr,
+                                  T, W.lists.as<int32_t>(), W.lens.as<int32_t>(), T, s));
         const int nbs[3] = {16, 32, 64};
         for (int b = 0; b < 3; b++) {
             if (nbs[b] > pad_nb(K->nb_bound)) break;

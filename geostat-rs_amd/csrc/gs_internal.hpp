@@ -204,8 +204,9 @@ hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, int64_t n, double value, hipStream_t s);
 // *dst = min(*src, cap) on the device (stream-ordered)
 hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStream_t s);
-hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
-                         int64_t list_cap, hipStream_t s);
+// perm: processing order of the targets (NULL: index order)
+hipError_t launch_bucket(const int32_t *count, const int32_t *perm, int64_t T, int32_t *lists,
+                         int32_t *lens, int64_t list_cap, hipStream_t s);
 hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaParams &p,
                           int64_t n_blocks, hipStream_t s);
 // Vecchia log-likelihood gradient (partial holds 4 columns per model:

@@ -8,16 +8,31 @@ namespace gs {
 
 constexpr int kW = 64;
 
-// Size buckets for uneven neighbourhoods (octant search): targets are
-// appended to the list of the smallest padded size that holds them.
-__global__ void bucket_kernel(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
-                              int64_t cap) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int c = count[t];
+// Size buckets for uneven neighbourhoods (octant search): targets, taken in
+// the search's processing order (perm, or index order), are appended to the
+// list of the smallest padded size that holds them.  Each wavefront claims
+// one run per bucket with a single atomic and keeps its targets' order, so
+// a bucket list is a concatenation of 64-target runs in processing order.
+__global__ void bucket_kernel(const int32_t *count, const int32_t *perm, int64_t T,
+                              int32_t *lists, int32_t *lens, int64_t cap) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < T;
+    const int32_t t = live ? (perm ? perm[i] : (int32_t)i) : 0;
+    const int c = live ? count[t] : 0;
     const int b = c <= 16 ? 0 : (c <= 32 ? 1 : 2);
-    const int pos = atomicAdd(&lens[b], 1);
-    lists[(int64_t)b * cap + pos] = (int32_t)t;
+    const unsigned lane = threadIdx.x & 63u;
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const uint64_t mask = __ballot(live && b == q);
+        if (!mask) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int base = 0;
+        if ((int)lane == leader) base = atomicAdd(&lens[q], __popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (live && b == q)
+            lists[(int64_t)q * cap + base + __popcll(mask & below)] = t;
+    }
 }
 
 // Per-model sum of the per-block partials in a fixed order: lane-strided
@@ -204,11 +219,11 @@ hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_bucket(const int32_t *count, int64_t T, int32_t *lists, int32_t *lens,
-                         int64_t list_cap, hipStream_t s) {
+hipError_t launch_bucket(const int32_t *count, const int32_t *perm, int64_t T, int32_t *lists,
+                         int32_t *lens, int64_t list_cap, hipStream_t s) {
     if (T == 0) return hipSuccess;
     hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, count,
-                       T, lists, lens, list_cap);
+                       perm, T, lists, lens, list_cap);
     return hipGetLastError();
 }
 
