@@ -133,24 +133,32 @@ __global__ void hilbert_kernel(const double *targets, int64_t T, SearchXf xf, Ce
 
 // Bounded max-heap over (d2, idx) in LDS, lane-interleaved (entry e of
 // lane l at [e][l]: conflict-free).
+// Bounded heap per lane, lane-interleaved in LDS.  A slot holds the
+// candidate's squared distance rounded to float (the ordering key) and its
+// position in the cell-ordered point arrays: 8 bytes instead of 12, so more
+// wavefronts fit in LDS.  Rounding is monotone, so the float order agrees
+// with the exact (d2, index) order except between candidates whose keys
+// round to the same float: the kernel flags a target when such a tie sits
+// at the selection boundary (the exact kd-tree replay then decides) and
+// re-orders equal-key runs of the selected set by the exact (d2, index).
 struct TileHeap {
-    double *d;
+    float *d;
     uint32_t *ix;
     int lane;
-    __device__ __forceinline__ double D2(int e) const { return d[e * kTile + lane]; }
+    __device__ __forceinline__ float D2(int e) const { return d[e * kTile + lane]; }
     __device__ __forceinline__ uint32_t IX(int e) const { return ix[e * kTile + lane]; }
-    __device__ __forceinline__ void set(int e, double v, uint32_t i) {
+    __device__ __forceinline__ void set(int e, float v, uint32_t i) {
         d[e * kTile + lane] = v;
         ix[e * kTile + lane] = i;
     }
-    static __device__ __forceinline__ bool gt(double ad, uint32_t ai, double bd, uint32_t bi) {
+    static __device__ __forceinline__ bool gt(float ad, uint32_t ai, float bd, uint32_t bi) {
         return ad > bd || (ad == bd && ai > bi);
     }
-    __device__ __forceinline__ void push(int &len, double v, uint32_t i) {
+    __device__ __forceinline__ void push(int &len, float v, uint32_t i) {
         int e = len++;
         while (e > 0) {
             const int p = (e - 1) >> 1;
-            const double pd = D2(p);
+            const float pd = D2(p);
             const uint32_t pi = IX(p);
             if (!gt(v, i, pd, pi)) break;
             set(e, pd, pi);
@@ -159,7 +167,7 @@ struct TileHeap {
         set(e, v, i);
     }
     // Sift (v, i) down from slot e; returns the value left in slot e.
-    __device__ __forceinline__ void sift_down(int len, int e, double v, uint32_t i, double &top_d,
+    __device__ __forceinline__ void sift_down(int len, int e, float v, uint32_t i, float &top_d,
                                               uint32_t &top_i) {
         top_d = v;
         top_i = i;
@@ -168,10 +176,10 @@ struct TileHeap {
             const int l = 2 * e + 1;
             if (l >= len) break;
             int c = l;
-            double cd = D2(l);
+            float cd = D2(l);
             uint32_t ci = IX(l);
             if (l + 1 < len) {
-                const double rd = D2(l + 1);
+                const float rd = D2(l + 1);
                 const uint32_t ri = IX(l + 1);
                 if (gt(rd, ri, cd, ci)) { c = l + 1; cd = rd; ci = ri; }
             }
@@ -184,23 +192,48 @@ struct TileHeap {
     }
     // Floyd's bottom-up construction over slots [0, len).
     __device__ __forceinline__ void heapify(int len) {
-        double td;
+        float td;
         uint32_t ti;
         for (int e = len / 2 - 1; e >= 0; e--) sift_down(len, e, D2(e), IX(e), td, ti);
     }
     __device__ __forceinline__ void sort(int len) {
-        double td;
+        float td;
         uint32_t ti;
         for (int end = len - 1; end > 0; end--) {
-            const double rd = D2(0);
+            const float rd = D2(0);
             const uint32_t ri = IX(0);
-            const double vd = D2(end);
+            const float vd = D2(end);
             const uint32_t vi = IX(end);
             set(end, rd, ri);
             sift_down(end, 0, vd, vi, td, ti);
         }
     }
 };
+
+// Upper bound of the exact squared distance behind a float key (round to
+// nearest: within half an ulp).
+__device__ __forceinline__ double key_upper(float f) { return (double)f * (1.0 + 1.2e-7); }
+
+// Exact search-frame squared distance of the point at cell-order position
+// p, in the reference's operation order.
+template <int D>
+__device__ __forceinline__ double exact_d2(const CellGridView &G, uint32_t p, const double *st) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const double dd = G.pts[(int64_t)p * D + d] - st[d];
+        d2 += dd * dd;
+    }
+    return d2;
+}
+
+// (d2, original index) order of two cell-order positions.
+template <int D>
+__device__ __forceinline__ bool exact_gt(const CellGridView &G, uint32_t a, uint32_t b,
+                                         const double *st) {
+    const double da = exact_d2<D>(G, a, st), db = exact_d2<D>(G, b, st);
+    return da > db || (da == db && G.idx[a] > G.idx[b]);
+}
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -242,10 +275,10 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
     const int lane = threadIdx.x;
     const int slots = OCT ? (P.per_octant << D) : P.k;
     TileHeap H;
-    H.d = reinterpret_cast<double *>(smem_raw);
+    H.d = reinterpret_cast<float *>(smem_raw);
     H.ix = reinterpret_cast<uint32_t *>(H.d + (size_t)slots * kTile);
     H.lane = lane;
-    RowTable *RT = reinterpret_cast<RowTable *>(smem_raw + (size_t)slots * kTile * 12);
+    RowTable *RT = reinterpret_cast<RowTable *>(smem_raw + (size_t)slots * kTile * 8);
     const CellGridView &G = P.G;
 
     const int64_t pos = (int64_t)blockIdx.x * kTile + lane;
@@ -301,12 +334,16 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
     const int k = P.k;
     const int q = P.per_octant;
     int len = 0;
-    double top_d = INFINITY;        // heap mode: root of the full heap
+    float top_d = INFINITY;         // heap mode: root of the full heap
     uint32_t top_i = 0;
     int olen[1 << D];
+    float orej[1 << D];             // octant mode: smallest key left out, per octant
 #pragma unroll
-    for (int o = 0; o < (1 << D); o++) olen[o] = 0;
-    double rej = INFINITY;
+    for (int o = 0; o < (1 << D); o++) {
+        olen[o] = 0;
+        orej[o] = INFINITY;
+    }
+    float rej = INFINITY;
     bool done = !active;
 
     int vlo[3] = {0, 0, 0}, vhi[3] = {0, 0, 0};   // visited rectangle
@@ -361,7 +398,7 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
                 double pc[D];
 #pragma unroll
                 for (int d = 0; d < D; d++) pc[d] = 0.0;
-                uint32_t pid = 0;
+                uint32_t pid = 0;   // cell-order position
                 if (g < total) {
                     int at = 0;    // last row whose exclusive prefix is <= g
 #pragma unroll
@@ -372,7 +409,7 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
                     const int64_t p = o < al ? (int64_t)RT->a0[at] + o : (int64_t)RT->b0[at] + (o - al);
 #pragma unroll
                     for (int d = 0; d < D; d++) pc[d] = G.pts[p * D + d];
-                    pid = G.idx[p];
+                    pid = (uint32_t)p;
                 }
                 const int cnt = min(kTile, total - cb);
                 for (int j = 0; j < cnt; j++) {
@@ -388,23 +425,24 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
                         d2 += dd * dd;
                     }
                     if (!(d2 <= r2)) continue;
+                    const float kf = __double2float_rn(d2);
                     if (!OCT) {
                         if (len < k) {
                             // fill unordered; one bottom-up heapify when full
                             // (candidates arrive roughly by increasing
                             // distance, so sift-up pushes would climb to the
                             // root every time)
-                            H.set(len, d2, id);
+                            H.set(len, kf, id);
                             if (++len == k) {
                                 H.heapify(k);
                                 top_d = H.D2(0);
                                 top_i = H.IX(0);
                             }
-                        } else if (TileHeap::gt(top_d, top_i, d2, id)) {
-                            rej = fmin(rej, top_d);
-                            H.sift_down(len, 0, d2, id, top_d, top_i);
+                        } else if (TileHeap::gt(top_d, top_i, kf, id)) {
+                            rej = fminf(rej, top_d);
+                            H.sift_down(len, 0, kf, id, top_d, top_i);
                         } else {
-                            rej = fmin(rej, d2);
+                            rej = fminf(rej, kf);
                         }
                     } else {
                         int oct = 0;
@@ -418,13 +456,21 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
 #pragma unroll
                         for (int o = 1; o < (1 << D); o++) if (oct == o) ol = olen[o];
                         if (ol < q) {
-                            O.push(ol, d2, id);
+                            O.push(ol, kf, id);
 #pragma unroll
                             for (int o = 0; o < (1 << D); o++) if (oct == o) olen[o] = ol;
-                        } else if (TileHeap::gt(O.D2(0), O.IX(0), d2, id)) {
-                            double td;
-                            uint32_t ti;
-                            O.sift_down(ol, 0, d2, id, td, ti);
+                        } else {
+                            const float od = O.D2(0);
+                            float left = kf;
+                            if (TileHeap::gt(od, O.IX(0), kf, id)) {
+                                float td;
+                                uint32_t ti;
+                                O.sift_down(ol, 0, kf, id, td, ti);
+                                left = od;
+                            }
+#pragma unroll
+                            for (int o = 0; o < (1 << D); o++)
+                                if (oct == o) orej[o] = fminf(orej[o], left);
                         }
                     }
                 }
@@ -451,14 +497,15 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
                 lbc = lbc * (1.0 - 1e-9) - 1e-6;
                 double bound = r2;
                 if (!OCT) {
-                    if (len == k) bound = fmin(bound, top_d);
+                    if (len == k) bound = fmin(bound, key_upper(top_d));
                 } else {
                     bool full = true;
                     double worst = 0.0;
 #pragma unroll
                     for (int o = 0; o < (1 << D); o++) {
                         full = full && olen[o] == q;
-                        if (olen[o] > 0) worst = fmax(worst, H.d[((size_t)o * q) * kTile + lane]);
+                        if (olen[o] > 0)
+                            worst = fmax(worst, key_upper(H.d[((size_t)o * q) * kTile + lane]));
                     }
                     if (full) bound = fmin(bound, worst);
                 }
@@ -473,19 +520,31 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
     if (!live) return;
 
     int32_t *out = P.out_idx + t * P.stride;
+    auto flag_target = [&]() {
+        const int slot = atomicAdd(P.flag_len, 1);
+        P.flag_list[slot] = (int32_t)t;
+    };
     if (!OCT) {
+        // a candidate left out with the boundary key: the exact order decides
         const bool tie = len == k && rej == top_d;
         if (len < k) H.heapify(len);     // a partial fill is still unordered
         H.sort(len);
-        for (int e = 0; e < len; e++) out[e] = (int32_t)H.IX(e);
+        // runs of equal keys in exact (d2, index) order (insertion sort)
+        for (int e = 1; e < len; e++) {
+            for (int j = e; j > 0 && H.D2(j - 1) == H.D2(j) &&
+                            exact_gt<D>(G, H.IX(j - 1), H.IX(j), st); j--) {
+                const uint32_t a = H.IX(j - 1);
+                H.set(j - 1, H.D2(j), H.IX(j));
+                H.set(j, H.D2(j), a);
+            }
+        }
+        for (int e = 0; e < len; e++) out[e] = (int32_t)G.idx[H.IX(e)];
         for (int e = len; e < P.stride; e++) out[e] = -1;
         P.out_count[t] = len;
-        if (tie) {
-            const int slot = atomicAdd(P.flag_len, 1);
-            P.flag_list[slot] = (int32_t)t;
-        }
+        if (tie) flag_target();
         return;
     }
+    bool tie = false;
     int head[1 << D];
 #pragma unroll
     for (int o = 0; o < (1 << D); o++) {
@@ -493,30 +552,54 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
         O.d = H.d + (size_t)o * q * kTile;
         O.ix = H.ix + (size_t)o * q * kTile;
         O.sort(olen[o]);
+        // a full octant whose boundary key is shared by a left-out candidate
+        if (olen[o] == q && q > 0 && orej[o] == O.D2(q - 1)) tie = true;
         head[o] = 0;
     }
     int total = 0;
 #pragma unroll
     for (int o = 0; o < (1 << D); o++) total += olen[o];
     const int outn = total < P.max_nb ? total : P.max_nb;
-    for (int e = 0; e < outn; e++) {
+    // merge by key; positions go to out first, then equal-key runs are put in
+    // exact order and the positions mapped to indices
+    float last = -1.0f;
+    for (int e = 0; e < total; e++) {
         int best = -1;
-        double bd = 0.0;
+        float bd = 0.0f;
         uint32_t bi = 0;
 #pragma unroll
         for (int o = 0; o < (1 << D); o++) {
             if (head[o] < olen[o]) {
-                const double od = H.d[((size_t)o * q + head[o]) * kTile + lane];
+                const float od = H.d[((size_t)o * q + head[o]) * kTile + lane];
                 const uint32_t oi = H.ix[((size_t)o * q + head[o]) * kTile + lane];
                 if (best < 0 || TileHeap::gt(bd, bi, od, oi)) { best = o; bd = od; bi = oi; }
             }
         }
 #pragma unroll
         for (int o = 0; o < (1 << D); o++) if (best == o) head[o]++;
-        out[e] = (int32_t)bi;
+        if (e < outn) {
+            out[e] = (int32_t)bi;
+            last = bd;
+        } else {
+            // truncation inside a run of equal keys: the exact order decides
+            if (bd == last) tie = true;
+            break;
+        }
     }
+    for (int e = 1; e < outn; e++) {
+        for (int j = e; j > 0; j--) {
+            const uint32_t a = (uint32_t)out[j - 1], b = (uint32_t)out[j];
+            if (__double2float_rn(exact_d2<D>(G, a, st)) != __double2float_rn(exact_d2<D>(G, b, st)))
+                break;
+            if (!exact_gt<D>(G, a, b, st)) break;
+            out[j - 1] = (int32_t)b;
+            out[j] = (int32_t)a;
+        }
+    }
+    for (int e = 0; e < outn; e++) out[e] = (int32_t)G.idx[(uint32_t)out[e]];
     for (int e = outn; e < P.stride; e++) out[e] = -1;
     P.out_count[t] = outn;
+    if (tie) flag_target();
 }
 
 }  // namespace
@@ -557,7 +640,7 @@ hipError_t launch_tile_order(int D, const double *targets, int64_t T, const Sear
 
 static size_t tile_lds_bytes(int D, int k, int per_octant) {
     const int slots = per_octant > 0 ? (per_octant << D) : k;
-    return (size_t)slots * kTile * (sizeof(double) + sizeof(uint32_t)) + sizeof(RowTable);
+    return (size_t)slots * kTile * (sizeof(float) + sizeof(uint32_t)) + sizeof(RowTable);
 }
 
 hipError_t launch_tile_knn(int D, const TileParams &p, hipStream_t s) {
