@@ -330,9 +330,11 @@ __device__ __forceinline__ double entry(double c0, double nug, double s) {
 }
 
 // Staging slots per lane: the NB-1 entries of a block pair go through the
-// staging column in two passes.
+// staging column in passes of this many slots (two passes, four for 64-point
+// systems, whose larger staging column would otherwise halve the wavefronts
+// that fit in LDS).
 template <int NB>
-constexpr int stage_slots() { return NB / 2; }
+constexpr int stage_slots() { return NB == 64 ? 16 : NB / 2; }
 
 // Right-looking Cholesky of the row-distributed C held in a (lane g owns
 // row row_of(r, g) of block r, columns before the diagonal), fused with the
@@ -439,6 +441,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
     double a[R][NB];
     constexpr int NE = NB - 1;              // entries per lane and block pair
     constexpr int HS = stage_slots<NB>();   // slots per staging pass
+    constexpr int NPS = (NE + HS - 1) / HS; // staging passes
     static_for<0, (R + 1) / 2>([&](auto PP) {
         constexpr int rA = decltype(PP)::value;
         constexpr int rB = R - 1 - rA;
@@ -451,7 +454,7 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
         // Entries right of the diagonal receive unused values: the
         // factorization never reads them (its multipliers for rows <= j are
         // selected away) and the diagonal is tracked in dg below.
-        static_for<0, 2>([&](auto PS) {
+        static_for<0, NPS>([&](auto PS) {
             constexpr int e0 = decltype(PS)::value * HS;
             constexpr int e1 = e0 + HS < NE ? e0 + HS : NE;
             staged_entries<D, NB, LEAN, decltype(PS)::value == 0>(Mdl, e0, e1, iA, rowx[rA],
@@ -1005,7 +1008,7 @@ __global__ __launch_bounds__(kWave) void vecchia_grad_kernel(VecchiaParams P) {
                     const int iB = NB - 1 - iA;
                     const bool liveA = iA < cnt, liveB = iB < cnt;
                     double *pcol = s_p + (size_t)bp * NE * kWave;
-                    static_for<0, 2>([&](auto PS) {
+                    static_for<0, (NE + HS - 1) / HS>([&](auto PS) {
                         constexpr int e0 = decltype(PS)::value * HS;
                         constexpr int e1 = e0 + HS < NE ? e0 + HS : NE;
                         if (sph) grad_slots<D, NB, true>(GR, e0, e1, iA, liveA, liveB, rowx[rA], rowx[rB], xy, s_stage, pcol, lane);
