@@ -210,6 +210,32 @@ struct TileHeap {
     }
 };
 
+// Ascending bitonic sorting network over the first N entries of a register
+// array (N a power of two; indices are compile-time constants, so the array
+// stays in VGPRs).  Used for the final order of small selections, where a
+// heap sort in LDS would be a chain of dependent LDS round trips.
+template <int N, int CAP>
+__device__ __forceinline__ void bitonic_sort(uint64_t (&a)[CAP]) {
+    static_assert(N <= CAP && (N & (N - 1)) == 0, "bitonic size");
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t x = a[i], y = a[l];
+                    const bool up = (i & k) == 0;
+                    const bool sw = up ? x > y : x < y;
+                    a[i] = sw ? y : x;
+                    a[l] = sw ? x : y;
+                }
+            }
+        }
+    }
+}
+
 // Upper bound of the exact squared distance behind a float key (round to
 // nearest: within half an ulp).
 __device__ __forceinline__ double key_upper(float f) { return (double)f * (1.0 + 1.2e-7); }
@@ -270,7 +296,7 @@ struct RowTable {
 };
 
 template <int D, bool OCT>
-__global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void tile_knn_kernel(TileParams P) {
     extern __shared__ unsigned char smem_raw[];
     const int lane = threadIdx.x;
     const int slots = OCT ? (P.per_octant << D) : P.k;
@@ -527,8 +553,38 @@ __global__ __launch_bounds__(kTile) void tile_knn_kernel(TileParams P) {
     if (!OCT) {
         // a candidate left out with the boundary key: the exact order decides
         const bool tie = len == k && rej == top_d;
-        if (len < k) H.heapify(len);     // a partial fill is still unordered
-        H.sort(len);
+        if (k <= 32) {
+            // (key, position) packed so that one unsigned compare orders
+            // them (keys are non-negative floats); empty slots sort last
+            uint64_t a[32];
+#pragma unroll
+            for (int e = 0; e < 32; e++) {
+                a[e] = ~0ull;
+                if (e < len) a[e] = ((uint64_t)__float_as_uint(H.D2(e)) << 32) | H.IX(e);
+            }
+            if (k <= 16) bitonic_sort<16>(a);
+            else bitonic_sort<32>(a);
+            // adjacent equal keys (rare) need the exact order: back to LDS
+            bool run = false;
+#pragma unroll
+            for (int e = 1; e < 32; e++)
+                run = run || (e < len && (uint32_t)(a[e - 1] >> 32) == (uint32_t)(a[e] >> 32));
+            if (!run) {
+#pragma unroll
+                for (int e = 0; e < 32; e++)
+                    if (e < len) out[e] = (int32_t)G.idx[(uint32_t)a[e]];
+                for (int e = len; e < P.stride; e++) out[e] = -1;
+                P.out_count[t] = len;
+                if (tie) flag_target();
+                return;
+            }
+#pragma unroll
+            for (int e = 0; e < 32; e++)
+                if (e < len) H.set(e, __uint_as_float((uint32_t)(a[e] >> 32)), (uint32_t)a[e]);
+        } else {
+            if (len < k) H.heapify(len);     // a partial fill is still unordered
+            H.sort(len);
+        }
         // runs of equal keys in exact (d2, index) order (insertion sort)
         for (int e = 1; e < len; e++) {
             for (int j = e; j > 0 && H.D2(j - 1) == H.D2(j) &&
