@@ -1791,11 +1791,12 @@ void grid_knn(const BGrid &G, const double *target, int64_t k, int64_t limit, KH
 // to the larger index.  The reference refreshes stale heap keys lazily;
 // here keys are kept exact instead: when p is selected with key l2 (the
 // largest current min-distance), only points closer to p than sqrt(l2) can
-// lower their key, so one range query on a grid of all points and a
-// decrease-key per improved point keep every key equal to the min squared
-// distance to the selected set.  The order is the same (argmax of
-// (total_key(min d2), index) at every step); the work is ~n log n instead
-// of the lazy refreshes' 1-NN queries on a sparse grid.
+// lower their key, so one range query on a grid of all points updates every
+// key to the min squared distance to the selected set.  The next point is
+// the argmax of (key, index), found through a tournament tree over blocks of
+// 64 bucket-ordered slots: a lowered key only costs a block rescan when it
+// was its block's maximum.  The order is the same as the lazy heap's; the
+// work is ~n log n without the lazy refreshes' 1-NN queries.
 void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &order) {
     order.clear();
     if (n == 0) return;
@@ -1822,9 +1823,9 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
     G.init(D, mn, mx, n);
     G.build_all(coords, n);
     // Everything below lives in bucket order (slot q holds point items[q]),
-    // so a range query reads contiguous coordinates, keys and flags.  Keys
-    // are sums of squares (never -0.0 or NaN), for which total_cmp is the
-    // plain numeric order.
+    // so a range query reads contiguous coordinates and keys.  Keys are sums
+    // of squares (never -0.0 or NaN), for which total_cmp is the plain
+    // numeric order; a selected slot's key is -1.
     const std::vector<uint32_t> &id = G.items;
     std::vector<double> cc((size_t)(n * D));
     std::vector<uint32_t> slot((size_t)n);
@@ -1833,57 +1834,69 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
         for (int d = 0; d < D; d++) cc[(size_t)(q * D + d)] = coords[(int64_t)id[(size_t)q] * D + d];
     }
     std::vector<double> key((size_t)n);    // min squared distance to the selected set
-    std::vector<char> taken((size_t)n, 0);
     const uint32_t qf = slot[(size_t)first];
-    taken[qf] = 1;
     for (int64_t q = 0; q < n; q++)
         key[(size_t)q] = dist2(&cc[(size_t)(q * D)], coords + first * D, D);
-    // indexed binary max-heap over the remaining slots, ordered by
-    // (key, original index) as MaxminKey (vecchia.rs:114-125)
-    std::vector<uint32_t> heap;
-    std::vector<int64_t> where((size_t)n, -1);
-    heap.reserve((size_t)n);
-    auto above = [&](uint32_t a, uint32_t b) {
-        const double ka = key[a], kb = key[b];
-        return ka > kb || (ka == kb && id[a] > id[b]);
+    key[qf] = -1.0;
+    // per block: the slot of its (key, index) maximum, as MaxminKey
+    // (vecchia.rs:114-125), with that key and index
+    constexpr int64_t kB = 64;
+    const int64_t nblk = (n + kB - 1) / kB;
+    std::vector<uint32_t> bmax((size_t)nblk), bid((size_t)nblk);
+    std::vector<double> bkey((size_t)nblk);
+    auto rescan = [&](int64_t b) {
+        const int64_t q0 = b * kB, len = std::min<int64_t>(kB, n - q0);
+        const double *kk = key.data() + q0;
+        double m = kk[0];
+        for (int64_t j = 1; j < len; j++) m = kk[j] > m ? kk[j] : m;
+        uint32_t best = (uint32_t)q0, bi = 0;
+        bool found = false;
+        for (int64_t j = 0; j < len; j++)
+            if (kk[j] == m && (!found || id[(size_t)(q0 + j)] > bi)) {
+                best = (uint32_t)(q0 + j);
+                bi = id[(size_t)(q0 + j)];
+                found = true;
+            }
+        bmax[(size_t)b] = best;
+        bkey[(size_t)b] = m;
+        bid[(size_t)b] = bi;
     };
-    auto place = [&](size_t e, uint32_t q) {
-        heap[e] = q;
-        where[q] = (int64_t)e;
+    for (int64_t b = 0; b < nblk; b++) rescan(b);
+    // tournament tree: node i holds the winning block of its range
+    int64_t L = 1;
+    while (L < nblk) L <<= 1;
+    std::vector<int32_t> win((size_t)(2 * L), -1);
+    auto better = [&](int32_t a, int32_t b) -> int32_t {
+        if (a < 0) return b;
+        if (b < 0) return a;
+        const double ka = bkey[(size_t)a], kb = bkey[(size_t)b];
+        return (kb > ka || (kb == ka && bid[(size_t)b] > bid[(size_t)a])) ? b : a;
     };
-    auto sift_down = [&](size_t e) {
-        const uint32_t q = heap[e];
-        const size_t sz = heap.size();
-        for (;;) {
-            size_t l = 2 * e + 1;
-            if (l >= sz) break;
-            size_t b = l;
-            if (l + 1 < sz && above(heap[l + 1], heap[l])) b = l + 1;
-            if (!above(heap[b], q)) break;
-            place(e, heap[b]);
-            e = b;
+    for (int64_t b = 0; b < nblk; b++) win[(size_t)(L + b)] = (int32_t)b;
+    for (int64_t i = L - 1; i >= 1; i--) win[(size_t)i] = better(win[(size_t)(2 * i)], win[(size_t)(2 * i + 1)]);
+    // block b's maximum went down: replay its path towards the root, up to
+    // the first node that neither changes nor was won by b
+    auto lowered = [&](int64_t b) {
+        rescan(b);
+        for (int64_t i = (L + b) >> 1; i >= 1; i >>= 1) {
+            const int32_t w = better(win[(size_t)(2 * i)], win[(size_t)(2 * i + 1)]);
+            if (w == win[(size_t)i] && w != (int32_t)b) break;
+            win[(size_t)i] = w;
         }
-        place(e, q);
     };
-    for (int64_t q = 0; q < n; q++)
-        if ((uint32_t)q != qf) { where[(size_t)q] = (int64_t)heap.size(); heap.push_back((uint32_t)q); }
-    for (size_t e = heap.size() / 2; e-- > 0;) sift_down(e);
-    while (!heap.empty()) {
-        const uint32_t p = heap[0];
-        const uint32_t last = heap.back();
-        heap.pop_back();
-        where[p] = -1;
-        if (!heap.empty()) {
-            place(0, last);
-            sift_down(0);
-        }
-        taken[p] = 1;
+    lowered(qf / kB);
+    for (int64_t sel = 1; sel < n; sel++) {
+        const int32_t wb = win[1];
+        const uint32_t p = bmax[(size_t)wb];
+        const double kp = key[p];
         order.push_back((int64_t)id[p]);
-        if (heap.empty()) break;
+        key[p] = -1.0;
+        lowered(wb);
+        if (sel + 1 == n) break;
         // points whose key can drop lie within sqrt(key[p]) of p (every
         // remaining key is <= key[p]); the box is widened for rounding
         const double *pp = &cc[(size_t)p * D];
-        const double rad = std::sqrt(key[p]) * (1.0 + 1e-9) + 1e-300;
+        const double rad = std::sqrt(kp) * (1.0 + 1e-9) + 1e-300;
         int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         for (int d = 0; d < D; d++) {
             lo[d] = std::min(std::max<int64_t>(BGrid::sat((pp[d] - rad - G.origin[d]) / G.cell), 0),
@@ -1897,14 +1910,21 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
             int64_t b0 = 0;
             for (int d = 1; d < D; d++) b0 += cur[d] * G.strides[d];
             const int64_t q0 = G.start[(size_t)(b0 + lo[0])], q1 = G.start[(size_t)(b0 + hi[0]) + 1];
+            int64_t blk = -1;
+            bool hit = false;   // the block's maximum was lowered
             for (int64_t q = q0; q < q1; q++) {
-                if (taken[(size_t)q]) continue;
+                if (q / kB != blk) {
+                    if (hit) lowered(blk);
+                    blk = q / kB;
+                    hit = false;
+                }
                 const double d2 = dist2(&cc[(size_t)(q * D)], pp, D);
                 if (d2 < key[(size_t)q]) {
                     key[(size_t)q] = d2;
-                    sift_down((size_t)where[(size_t)q]);
+                    hit = hit || (uint32_t)q == bmax[(size_t)blk];
                 }
             }
+            if (hit) lowered(blk);
             int d = 1;
             for (; d < D; d++) {
                 if (++cur[d] <= hi[d]) break;
