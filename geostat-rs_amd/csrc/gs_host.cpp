@@ -253,30 +253,40 @@ DevModel resolve_model(const gs_model &m, int D, const gs::BesselNodes *d_nodes)
     return M;
 }
 
-// First duplicate location (data.rs:150-163): keys are (c + 0.0).to_bits().
+// First duplicate location (data.rs:150-163): keys are (c + 0.0).to_bits();
+// the first point whose key was seen before, paired with that earlier point.
+// Open addressing over point indices (linear probing, load <= 1/2).
 bool duplicate_pair(int D, const double *coords, int64_t n, int64_t pair[2]) {
-    struct KeyHash {
-        size_t operator()(const std::array<uint64_t, 3> &k) const {
-            uint64_t h = 1469598103934665603ULL;
-            for (uint64_t v : k) h = (h ^ v) * 1099511628211ULL ^ (v >> 29);
-            return (size_t)h;
-        }
+    size_t cap = 16;
+    while (cap < (size_t)n * 2) cap <<= 1;
+    std::vector<int64_t> tab(cap, -1);
+    auto bits = [&](int64_t i, int d) {
+        const double c = coords[i * D + d] + 0.0;
+        uint64_t b;
+        std::memcpy(&b, &c, 8);
+        return b;
     };
-    std::unordered_map<std::array<uint64_t, 3>, int64_t, KeyHash> seen;
-    seen.reserve((size_t)n * 2);
     for (int64_t i = 0; i < n; i++) {
-        std::array<uint64_t, 3> key{0, 0, 0};
+        uint64_t h = 0x9E3779B97F4A7C15ULL;
         for (int d = 0; d < D; d++) {
-            double c = coords[i * D + d] + 0.0;
-            std::memcpy(&key[d], &c, 8);
+            h ^= bits(i, d);
+            h *= 0xBF58476D1CE4E5B9ULL;
+            h ^= h >> 31;
         }
-        auto it = seen.find(key);
-        if (it != seen.end()) {
-            pair[0] = it->second;
-            pair[1] = i;
-            return true;
+        for (size_t s = (size_t)h & (cap - 1);; s = (s + 1) & (cap - 1)) {
+            const int64_t j = tab[s];
+            if (j < 0) {
+                tab[s] = i;
+                break;
+            }
+            bool eq = true;
+            for (int d = 0; d < D; d++) eq = eq && bits(j, d) == bits(i, d);
+            if (eq) {
+                pair[0] = j;
+                pair[1] = i;
+                return true;
+            }
         }
-        seen.emplace(key, i);
     }
     return false;
 }
