@@ -370,6 +370,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         orej[o] = INFINITY;
     }
     float rej = INFINITY;
+    float wmax = INFINITY;          // octant mode: largest root once all heaps are full
     bool done = !active;
 
     int vlo[3] = {0, 0, 0}, vhi[3] = {0, 0, 0};   // visited rectangle
@@ -420,6 +421,19 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
             RT->b0[lane] = b0;
             __syncthreads();
             for (int cb = 0; cb < total; cb += kTile) {
+                if (OCT) {
+                    // once every octant heap is full, a candidate whose key
+                    // exceeds all their roots enters none of them (the bound
+                    // is refreshed per chunk: roots only fall in between)
+                    bool full = true;
+                    float m = 0.0f;
+#pragma unroll
+                    for (int o = 0; o < (1 << D); o++) {
+                        full = full && olen[o] == q;
+                        if (olen[o] > 0) m = fmaxf(m, H.d[((size_t)o * q) * kTile + lane]);
+                    }
+                    wmax = full ? m : INFINITY;
+                }
                 const int g = cb + lane;
                 double pc[D];
 #pragma unroll
@@ -452,6 +466,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
                     }
                     if (!(d2 <= r2)) continue;
                     const float kf = __double2float_rn(d2);
+                    if (OCT && kf > wmax) continue;
                     if (!OCT) {
                         if (len < k) {
                             // fill unordered; one bottom-up heapify when full
