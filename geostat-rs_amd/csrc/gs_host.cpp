@@ -2455,7 +2455,11 @@ int32_t gs_server_create(gs_ctx *ctx, int32_t n_slots, int64_t mem_budget, gs_se
     S->budget = mem_budget;
     for (int i = 0; i < n_slots; i++) {
         std::unique_ptr<gs_server::Slot> sl(new gs_server::Slot());
-        HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+        const hipError_t e = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (auto &made : S->slots) (void)hipStreamDestroy(made->stream);
+            return fail(GS_ERR_DEVICE, std::string("stream creation: ") + hipGetErrorString(e));
+        }
         S->slots.push_back(std::move(sl));
     }
     for (auto &sl : S->slots) {
