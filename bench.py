@@ -276,7 +276,10 @@ def main() -> int:
     d_var = torch.empty(T, dtype=torch.float64, device=dev)
     d_lag = torch.empty(T, dtype=torch.float64, device=dev)
     d_st = torch.empty(T, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    # the predictor's own stream (default priority), ordered after the
+    # uploads and fills above
+    stream = torch.cuda.Stream(device=dev)
+    stream.wait_stream(torch.cuda.current_stream(dev))
     sh = stream.cuda_stream
 
     def step():

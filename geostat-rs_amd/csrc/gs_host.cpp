@@ -1200,7 +1200,7 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
         return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
     std::lock_guard<std::mutex> lk(k->mu);
     HIP_TRY(hipSetDevice(k->ctx->device));
-    hipStream_t s = stream ? (hipStream_t)stream : k->ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
     if (k->nv > 0) {
         // predict on an external-drift predictor fails per target
         // (kriging.rs:494-501) and predict_many maps that to NaN

@@ -512,7 +512,9 @@ class Kriging:
     def predict_many_device(self, targets, value, variance, lagrange=None, status=None,
                             precision="fp64", stream=None):
         """Enqueues predict_many on device tensors (torch CUDA tensors or raw
-        pointers); returns immediately."""
+        pointers) on `stream` (a hipStream_t as an int, e.g. a torch stream's
+        cuda_stream; None or 0 is the null stream, torch's default stream);
+        returns immediately."""
         def ptr(x):
             if x is None:
                 return None

@@ -177,9 +177,11 @@ int32_t gs_kriging_predict_block_many(gs_kriging *k, const double *centers, int6
                                       int32_t *status);
 
 /* Same as gs_kriging_predict_many with every buffer already in device
- * memory, enqueued on `stream` (a hipStream_t; NULL = the context stream).
- * Asynchronous: returns after enqueueing.  Workspace comes from the
- * handle's cache, so calls on one handle must not overlap. */
+ * memory, enqueued on `stream` (a hipStream_t; NULL is the null stream, as
+ * in HIP's own calls, so work the caller queued there, such as filling
+ * the buffers, comes first).  Asynchronous: returns after enqueueing.
+ * Workspace comes from the handle's cache, so calls on one handle must not
+ * overlap. */
 int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, int64_t T,
                                        int32_t precision, double *d_value, double *d_variance,
                                        double *d_lagrange, int32_t *d_status, void *stream);
