@@ -3,8 +3,10 @@
 Each case draws a dimension, a point set (uniform, clustered, or snapped to
 a lattice so that distances tie), a conditioning size m from 1 to 64, the
 maxmin or a given order, and a batch of models of both the lean and the
-general class.  Bars as in the Vecchia tests: the plan bit-exact, every
-log-likelihood within 1e-10 relative, status codes equal.
+general class; a second sweep checks the likelihood gradient of random
+lean models.  Bars as in the Vecchia tests: the plan bit-exact, every
+log-likelihood within 1e-10 relative, status codes equal, gradients within
+1e-8 normwise.
 """
 import numpy as np
 import pytest
@@ -61,3 +63,19 @@ def test_random_plan_and_models_match_oracle(oracle, seed):
             continue
         assert st[i] == 0, (i, st[i])
         assert abs(ll[i] - oll) <= TOL64 * abs(oll), (i, ll[i], oll)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_plan_gradient_matches_oracle(oracle, seed):
+    from test_gpu_vecchia import GRAD_KINDS, _check_grad
+    D, c, v, _, order, _ = _case(100 + seed)
+    r = np.random.default_rng(7000 + seed)
+    m = int(r.integers(1, 33))          # the gradient kernels take sets up to 32
+    models = [gh.VariogramModel(float(r.uniform(0.01, 0.3)),
+                                [gh.Structure(GRAD_KINDS[int(r.integers(len(GRAD_KINDS)))],
+                                              float(r.uniform(0.3, 2.0)),
+                                              float(r.uniform(0.03, 0.4)))])
+              for _ in range(int(r.integers(1, 9)))]
+    print(f"gradient sweep seed {seed}: D={D} n={len(c)} m={m} models={len(models)}", flush=True)
+    p, o, nb, cnt = plan_equal(oracle, c, m, order)
+    _check_grad(oracle, c, v, o, nb, cnt, p, models)
