@@ -120,6 +120,17 @@ __device__ __forceinline__ double eff_h(const DevStructure &s, const double *dh)
     return sqrt(acc);
 }
 
+// sill * g(|T dh| / range) of one structure, kept out of line: the
+// libm-grade functions and the Bessel quadrature of shape_g would otherwise
+// be inlined at every entry of the unrolled solve kernels, which then need
+// far more than 256 VGPRs and spill hundreds of SGPRs.
+template <int D>
+__device__ __noinline__ double general_term(const DevStructure *s, const BesselNodes *gl,
+                                            double dx, double dy, double dz) {
+    const double dh[3] = {dx, dy, dz};
+    return s->sill * shape_g(*s, gl, eff_h<D>(*s, dh));
+}
+
 // ---- lean model class: spherical, exponential, Gaussian, Matern 3/2, 5/2 --
 // These five kinds need only exp and sqrt.  Kernels specialised on this
 // class use the two routines below instead of the general libm-grade ones,
@@ -205,7 +216,7 @@ __device__ __forceinline__ double cov_offdiag(const DevModel &M, const double *d
     double gam = 0.0;
     for (int i = 0; i < M.ns; i++) {
         if constexpr (LEAN) gam += M.s[i].sill * shape_lean(M.s[i], eff_h2<D>(M.s[i], dh));
-        else gam += M.s[i].sill * shape_g(M.s[i], M.nodes, eff_h<D>(M.s[i], dh));
+        else gam += general_term<D>(&M.s[i], M.nodes, dh[0], D > 1 ? dh[1] : 0.0, D > 2 ? dh[2] : 0.0);
     }
     return M.c0 - (M.nugget + gam);
 }

@@ -301,21 +301,21 @@ __device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int e0
             }
 #undef GS_LEAN_CASE
         } else {
-            const DevStructure S = Mdl.s[s];
+            const DevStructure *S = &Mdl.s[s];
             const bool first = s == 0;
             for (int e = e0; e < e1; e++) {
                 const bool inA = e < iA;
                 const int c = inA ? e : NB - 2 - e;
-                double dh[D];
+                double dh[3] = {0.0, 0.0, 0.0};
 #pragma unroll
                 for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
-                const double v = S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dh));
+                const double v = general_term<D>(S, Mdl.nodes, dh[0], dh[1], dh[2]);
                 double *slot = stage + (e - e0) * kWave + lane;
                 *slot = first ? v : *slot + v;
             }
             if constexpr (TGT) {
-                tA += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dA));
-                tB += S.sill * shape_g(S, Mdl.nodes, eff_h<D>(S, dB));
+                tA += general_term<D>(S, Mdl.nodes, dA[0], dA[1], D > 2 ? dA[D - 1] : 0.0);
+                tB += general_term<D>(S, Mdl.nodes, dB[0], dB[1], D > 2 ? dB[D - 1] : 0.0);
             }
         }
     }

@@ -1,0 +1,122 @@
+"""Seeded sweep over random extended predictors against the oracle:
+measurement error, external drift, block kriging, and block kriging with
+measurement error, each over random data, nested models (lean and general
+kinds, optional anisotropy) and neighbourhoods (k nearest, radius with a
+minimum, octant quotas).  Bars as in the extended tests: status codes
+equal, value and variance within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import geostat_hip as gh
+from test_gpu_extended import compare
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [gh.ModelKind.SPHERICAL, gh.ModelKind.EXPONENTIAL, gh.ModelKind.MATERN15,
+         gh.ModelKind.MATERN25, gh.ModelKind.Matern(0.8), gh.ModelKind.Stable(1.5)]
+VARIANTS = ("error", "drift", "block", "block_error")
+
+
+def _anis(r, D):
+    if D == 2:
+        return gh.Anisotropy(float(r.uniform(0, 180)), float(r.uniform(0.4, 1.0)))
+    return gh.Anisotropy(float(r.uniform(0, 180)), float(r.uniform(0.4, 1.0)),
+                         ratio_z=float(r.uniform(0.3, 1.0)), dip_deg=float(r.uniform(-20, 20)))
+
+
+def _neighbourhood(r, D, method, m_drift):
+    # at least three neighbours per drift column: fewer leave the drift block
+    # so ill-conditioned that rounding alone separates two correct solvers
+    mode = int(r.integers(3))
+    kmin = max(6, 3 * m_drift)
+    if mode == 0:
+        return gh.KrigingConfig(method=method, max_neighbors=int(r.integers(kmin, 65)))
+    if mode == 1:
+        return gh.KrigingConfig(method=method, max_neighbors=int(r.integers(kmin, 49)),
+                                search_radius=float(r.uniform(60, 250)),
+                                min_neighbors=max(3, 3 * m_drift))
+    q = int(r.integers(max(1, (kmin + (1 << D) - 1) >> D), (64 >> D) + 1))
+    return gh.KrigingConfig(method=method, max_per_octant=q, search_radius=float(r.uniform(150, 400)),
+                            min_neighbors=max(3, 3 * m_drift))
+
+
+def _block_offsets(r, D):
+    size = [float(r.uniform(5, 40)) for _ in range(D)]
+    counts = [int(r.integers(1, 5)) for _ in range(D)]
+    if D == 2:
+        return gh.block_offsets(size, counts)
+    axes = [((np.arange(k) + 0.5) / k - 0.5) * s for s, k in zip(size, counts)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return np.column_stack([x.ravel(), y.ravel(), z.ravel()])
+
+
+def _case(seed):
+    """(variant, D, coords, values, model, config, targets, extras) of one case."""
+    r = np.random.default_rng(9000 + seed)
+    variant = VARIANTS[seed % 4]
+    D = 2 if (seed // 4) % 3 else 3
+    n = int(r.integers(500, 3000))
+    ext = np.array([1000.0, 1000.0, 150.0][:D])
+    c = r.uniform(0.0, 1.0, size=(n, D)) * ext
+    v = np.sin(c[:, 0] / 140.0) + np.cos(c[:, 1] / 170.0) + 0.2 * r.standard_normal(n)
+    structs = [gh.Structure(KINDS[int(r.integers(len(KINDS)))], float(r.uniform(0.4, 1.0)),
+                            float(r.uniform(100, 450)), _anis(r, D) if r.uniform() < 0.4 else None)
+               for _ in range(int(r.integers(1, 3)))]
+    model = gh.VariogramModel(float(r.uniform(0.05, 0.2)), structs)
+    t = r.uniform(-0.1, 1.1, size=(400, D)) * ext
+    t[:10] = c[:10]
+    extra = {"errors": np.abs(r.standard_normal(n)) * float(r.uniform(0.01, 0.2))}
+    if variant == "error":
+        method = [gh.KrigingMethod.Simple(0.2), gh.KrigingMethod.Ordinary(),
+                  gh.KrigingMethod.Universal(1), gh.KrigingMethod.Universal(2)][int(r.integers(4))]
+        m_drift = {0: 0, 1: 1}.get(method.kind,
+                                   1 + D if method.degree == 1 else 1 + D + D * (D + 1) // 2)
+        cfg = _neighbourhood(r, D, method, m_drift)
+    elif variant == "drift":
+        nv = int(r.integers(1, 4))
+        # covariates varying along different directions, so that they stay
+        # distinct inside one neighbourhood (nearly collinear covariates make
+        # the drift block too ill-conditioned for a 1e-10 comparison)
+        ang = [np.deg2rad(60.0 * j + 15.0) for j in range(nv)]
+        cov = lambda p: np.column_stack([np.sin((p[:, 0] * np.cos(a) + p[:, 1] * np.sin(a)) / (60.0 + 15 * j))
+                                         for j, a in enumerate(ang)])
+        extra["drift"], extra["tdrift"] = cov(c), cov(t)
+        v = v + extra["drift"].sum(1)
+        cfg = _neighbourhood(r, D, gh.KrigingMethod.ExternalDrift(nv), 1 + nv)
+    else:
+        method = [gh.KrigingMethod.Simple(0.1), gh.KrigingMethod.Ordinary()][int(r.integers(2))]
+        cfg = _neighbourhood(r, D, method, method.kind)
+        extra["offsets"] = _block_offsets(r, D)
+    return variant, D, c, v, model, cfg, t, extra
+
+
+def run_case(oracle, seed):
+    """(product result, oracle result) of one case."""
+    variant, D, c, v, model, cfg, t, x = _case(seed)
+    if variant == "error":
+        k = gh.Kriging.with_measurement_error(c, v, model, cfg, x["errors"])
+        return (k.predict_many(t), oracle.predict_many_ex(c, v, model, cfg, t, errors=x["errors"]))
+    if variant == "drift":
+        k = gh.Kriging.with_external_drift(c, v, model, cfg, x["drift"])
+        return (k.predict_many_with_drift(t, x["tdrift"]),
+                oracle.predict_many_ex(c, v, model, cfg, t, drift=x["drift"], tdrift=x["tdrift"]))
+    if variant == "block":
+        k = gh.Kriging(c, v, model, cfg)
+        return (k.predict_block_many(t, x["offsets"]),
+                oracle.predict_many_ex(c, v, model, cfg, t, offsets=x["offsets"], block=True))
+    k = gh.Kriging.with_measurement_error(c, v, model, cfg, x["errors"])
+    return (k.predict_block_many(t, x["offsets"]),
+            oracle.predict_many_ex(c, v, model, cfg, t, errors=x["errors"], offsets=x["offsets"],
+                                   block=True))
+
+
+@pytest.mark.parametrize("seed", range(32))
+def test_random_extended_predictor_matches_oracle(oracle, seed):
+    variant, D, c, _, model, cfg, _, _ = _case(seed)
+    print(f"extended sweep seed {seed}: {variant} D={D} n={len(c)} "
+          f"structures={len(model.structures)} method={cfg.method.kind}/{cfg.method.degree} "
+          f"k={cfg.max_neighbors} octant={cfg.max_per_octant} radius={cfg.search_radius}",
+          flush=True)
+    got, want = run_case(oracle, seed)
+    compare(model, got, want)
