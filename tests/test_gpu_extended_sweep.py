@@ -120,3 +120,27 @@ def test_random_extended_predictor_matches_oracle(oracle, seed):
           flush=True)
     got, want = run_case(oracle, seed)
     compare(model, got, want)
+
+
+@pytest.mark.parametrize("kind", [gh.ModelKind.Matern(0.8), gh.ModelKind.Stable(1.5),
+                                  gh.ModelKind.HOLE], ids=["matern0.8", "stable", "hole"])
+def test_general_class_small_systems_repeatable(oracle, kind):
+    # Regression: general-class extended kernels once returned run-to-run
+    # spurious singular statuses on 3-7 point systems.  Ten repeated calls
+    # must agree bit for bit and with the oracle's statuses.
+    r = np.random.default_rng(5)
+    ext = [1000.0, 1000.0, 150.0]
+    c = r.uniform(0, 1, size=(2000, 3)) * ext
+    v = np.sin(c[:, 0] / 100.0)
+    t = r.uniform(-0.1, 1.1, size=(300, 3)) * ext
+    err = np.full(2000, 0.05)
+    m = gh.VariogramModel(0.1, [gh.Structure(kind, 0.8, 200.0)])
+    cfg = gh.KrigingConfig(method=gh.KrigingMethod.Simple(0.2), max_neighbors=4)
+    print(f"general-class repeat: kind={kind} k=4 T={len(t)} x10", flush=True)
+    k = gh.Kriging.with_measurement_error(c, v, m, cfg, err)
+    first = k.predict_many(t)
+    compare(m, first, oracle.predict_many_ex(c, v, m, cfg, t, errors=err))
+    for _ in range(9):
+        again = k.predict_many(t)
+        assert np.array_equal(again[3], first[3])
+        assert np.array_equal(again[0], first[0]) and np.array_equal(again[1], first[1])
