@@ -328,6 +328,7 @@ class Context:
         h = C.c_void_p()
         _check(_abi.lib().gs_ctx_create(device, C.byref(h)))
         self._h = h
+        self._release = _abi.lib().gs_ctx_destroy   # kept: module globals may be gone at exit
         self.device = device
 
     @classmethod
@@ -341,8 +342,9 @@ class Context:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _abi._lib is not None:
-            _abi.lib().gs_ctx_destroy(h)
+        release = getattr(self, "_release", None)
+        if h and release is not None:
+            release(h)
             self._h = None
 
 
@@ -412,6 +414,7 @@ class Kriging:
                                     _abi.i64ptr(dup))
         _check(st, dup)
         self._h = h
+        self._release = _abi.lib().gs_kriging_free   # kept: module globals may be gone at exit
 
     @classmethod
     def with_measurement_error(cls, coords, values, model: VariogramModel,
@@ -428,8 +431,9 @@ class Kriging:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _abi._lib is not None:
-            _abi.lib().gs_kriging_free(h)
+        release = getattr(self, "_release", None)
+        if h and release is not None:
+            release(h)
             self._h = None
 
     @property
@@ -662,11 +666,13 @@ class VecchiaPlan:
                                               self.m, None if o is None else _abi.i64ptr(o),
                                               C.byref(h)))
         self._h = h
+        self._release = _abi.lib().gs_vecchia_plan_free   # kept: module globals may be gone at exit
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _abi._lib is not None:
-            _abi.lib().gs_vecchia_plan_free(h)
+        release = getattr(self, "_release", None)
+        if h and release is not None:
+            release(h)
             self._h = None
 
     def arrays(self):
@@ -801,11 +807,13 @@ class Server:
         _check(_abi.lib().gs_server_create(self.ctx._h, int(n_slots), int(mem_budget),
                                            C.byref(h)))
         self._h = h
+        self._release = _abi.lib().gs_server_destroy   # kept: module globals may be gone at exit
 
     def close(self) -> None:
         h = getattr(self, "_h", None)
-        if h and _abi._lib is not None:
-            _abi.lib().gs_server_destroy(h)
+        release = getattr(self, "_release", None)
+        if h and release is not None:
+            release(h)
             self._h = None
 
     def __del__(self):
