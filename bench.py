@@ -250,8 +250,8 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(targets shard trivially; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (targets would "
+              "shard by contiguous ranges with the data replicated on every GPU)", file=sys.stderr)
         return 2
 
     if args.workload in ("vecchia_c3", "serving"):
