@@ -2,8 +2,8 @@
  * refcpu — CPU oracle for the geostat-hip kriging / Vecchia hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
- * algorithm (franciscoparrao/geostat-rs, crates/geostat-core) used as the
- * checker for the HIP implementation.  Only tests/, __graft_entry__.smoke()
+ * algorithm (franciscoparrao/geostat-rs, crates/geostat-core) that the HIP
+ * implementation's results are compared with.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load it.  The product library
  * (libgeostat_hip.so) never links or calls it.
  *

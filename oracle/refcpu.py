@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/build/librefcpu.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, always as the checker or the timed CPU
-baseline, never as part of the product path.
+bench.py's cpu_baseline leg, always as the reference the GPU results are
+compared with or the timed CPU baseline, never as part of the product path.
 
 Model / config arguments are read by attribute (kind, sill, range, anis,
 nugget, structures, method, ...), so the same description objects the

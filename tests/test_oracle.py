@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/refcpu.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/refcpu.c) before the GPU results are compared with it.
 
 Each test names the reference test or the mathematical fact it pins.
 Reference constants (gstat, R besselK, published xoshiro vectors) are
