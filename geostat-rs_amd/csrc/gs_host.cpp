@@ -406,9 +406,42 @@ struct gs_ctx {
 // bucket lists, staging for host buffers, global-neighbourhood scratch.
 // The handle's own calls use the one inside the handle; the request loop
 // gives every slot its own, so requests on one dataset run concurrently.
+// A default-priority stream of the workspace's own, with the two events that
+// let the kd-tree replay of flagged targets run beside the solve of the
+// other targets (created at first use, with the caller's device current).
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t searched = nullptr, replayed = nullptr;
+    SideStream() = default;
+    SideStream(const SideStream &) = delete;
+    SideStream &operator=(const SideStream &) = delete;
+    ~SideStream() {
+        if (s) (void)hipStreamDestroy(s);
+        if (searched) (void)hipEventDestroy(searched);
+        if (replayed) (void)hipEventDestroy(replayed);
+    }
+    hipError_t ensure() {
+        if (!s) {
+            hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+            if (e != hipSuccess) { s = nullptr; return e; }
+        }
+        if (!searched) {
+            hipError_t e = hipEventCreateWithFlags(&searched, hipEventDisableTiming);
+            if (e != hipSuccess) { searched = nullptr; return e; }
+        }
+        if (!replayed) {
+            hipError_t e = hipEventCreateWithFlags(&replayed, hipEventDisableTiming);
+            if (e != hipSuccess) { replayed = nullptr; return e; }
+        }
+        return hipSuccess;
+    }
+};
+
 struct KWork {
     DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
     DBuf keys, keys_out, vals, perm, sort_tmp, flags, h_tdrift, blk, g_w, g_b0;
+    DBuf mark;          // [T] bytes: 1 for targets flagged for the kd-tree replay
+    SideStream side;
 };
 
 struct gs_kriging {
@@ -941,7 +974,11 @@ namespace {
 int32_t stride_of(const gs_kriging *K) { return (int32_t)std::max<int64_t>(K->stride, 1); }
 
 // Neighbour lists for T device targets into the handle's nbr/count buffers.
-int32_t run_neighbors(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, hipStream_t s) {
+// side: the kd-tree replay of the flagged targets goes onto the workspace's
+// side stream after the tile search (W.side.replayed marks its end) and the
+// flagged targets are marked in W.mark; otherwise it follows on s.
+int32_t run_neighbors(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, hipStream_t s,
+                      bool side = false) {
     const int32_t stride = stride_of(K);
     HIP_TRY(W.nbr.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1) * stride));
     HIP_TRY(W.count.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1)));
@@ -981,6 +1018,11 @@ int32_t run_neighbors(gs_kriging *K, KWork &W, const double *d_targets, int64_t 
     tp.out_count = W.count.as<int32_t>();
     tp.flag_list = W.flags.as<int32_t>();
     tp.flag_len = flag_len;
+    if (side) {
+        HIP_TRY(W.mark.ensure((size_t)((T + 3) / 4) * 4));
+        HIP_TRY(gs::launch_zero_i32(W.mark.as<int32_t>(), (T + 3) / 4, s));
+        tp.flag_mark = W.mark.as<uint8_t>();
+    }
     HIP_TRY(gs::launch_tile_knn(K->D, tp, s));
     // 3) exact kd-tree replay (the reference's traversal order) of the
     //    flagged targets
@@ -1002,6 +1044,14 @@ int32_t run_neighbors(gs_kriging *K, KWork &W, const double *d_targets, int64_t 
     p.out_idx = W.nbr.as<int32_t>();
     p.out_count = W.count.as<int32_t>();
     p.stride = stride;
+    if (side) {
+        HIP_TRY(W.side.ensure());
+        HIP_TRY(hipEventRecord(W.side.searched, s));
+        HIP_TRY(hipStreamWaitEvent(W.side.s, W.side.searched, 0));
+        HIP_TRY(gs::launch_knn(K->D, p, W.side.s));
+        HIP_TRY(hipEventRecord(W.side.replayed, W.side.s));
+        return GS_OK;
+    }
     HIP_TRY(gs::launch_knn(K->D, p, s));
     return GS_OK;
 }
@@ -1018,7 +1068,7 @@ struct Extra {
 
 int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                    int32_t precision, double *d_value, double *d_var, double *d_lag,
-                   int32_t *d_status, hipStream_t s, const Extra &x);
+                   int32_t *d_status, hipStream_t s, const Extra &x, bool side = false);
 
 // Global neighbourhood above 64 points: per-target substitutions with the
 // shared factors, in chunks whose scratch (b0 and the solution, N doubles
@@ -1079,10 +1129,14 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         for (auto &e : ev) HIP_TRY(K->take_event(&e));
         HIP_TRY(hipEventRecord(ev[0], s));
     }
-    int32_t st = run_neighbors(K, W, d_targets, T, s);
+    // k-nearest and radius neighbourhoods: the kd-tree replay of flagged
+    // targets overlaps the solve of the others (octant neighbourhoods are
+    // solved in size buckets, which take every target at once)
+    const bool side = K->has_tree && K->per_oct == 0;
+    int32_t st = run_neighbors(K, W, d_targets, T, s, side);
     if (st != GS_OK) return st;
     if (timed && K->timing) HIP_TRY(hipEventRecord(ev[1], s));
-    st = run_solves(K, W, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x);
+    st = run_solves(K, W, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x, side);
     if (st != GS_OK) return st;
     if (timed && K->timing) {
         HIP_TRY(hipEventRecord(ev[2], s));
@@ -1093,7 +1147,7 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
 
 int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                    int32_t precision, double *d_value, double *d_var, double *d_lag,
-                   int32_t *d_status, hipStream_t s, const Extra &x) {
+                   int32_t *d_status, hipStream_t s, const Extra &x, bool side) {
     gs::SolveParams p;
     std::memset(&p, 0, sizeof(p));
     p.coords = K->coords.as<double>();
@@ -1183,6 +1237,19 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         HIP_TRY(gs::launch_fill_i32(W.lens.as<int32_t>() + 3, 1, (int32_t)T, s));
         p.list = W.perm.as<int32_t>();
         p.list_len = W.lens.as<int32_t>() + 3;
+        if (side) {
+            // the unflagged targets while the replay runs, then the flagged
+            // ones once their lists are in
+            p.skip = W.mark.as<uint8_t>();
+            int32_t st = solve(pad_nb(K->nb_bound), p, T);
+            if (st != GS_OK) return st;
+            HIP_TRY(hipStreamWaitEvent(s, W.side.replayed, 0));
+            gs::SolveParams pf = p;
+            pf.skip = nullptr;
+            pf.list = W.flags.as<int32_t>();
+            pf.list_len = W.flags.as<int32_t>() + T;
+            return solve(pad_nb(K->nb_bound), pf, std::min<int64_t>(T, (int64_t)256 * 64 * 4));
+        }
     }
     return solve(pad_nb(K->nb_bound), p, T);
 }

@@ -57,6 +57,7 @@ struct TileParams {
     int32_t *out_count;
     int32_t *flag_list;      // targets for the kd-tree replay (ties, scattered tiles)
     int32_t *flag_len;
+    uint8_t *flag_mark;      // optional [T]: set to 1 for every flagged target
 };
 
 struct KnnParams {
@@ -86,6 +87,7 @@ struct SolveParams {
     const int32_t *count;    // [T]
     const int32_t *list;     // optional bucket list of target ids
     const int32_t *list_len; // device counter for list
+    const uint8_t *skip;     // optional per-target mask: nonzero targets are left alone
     int64_t T;
     int32_t stride;
     int32_t min_nb;

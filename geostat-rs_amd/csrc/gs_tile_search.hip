@@ -352,6 +352,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         if (live) {
             const int rank = (int)__popcll(live_mask & ((1ull << lane) - 1ull));
             P.flag_list[base + rank] = (int32_t)t;
+            if (P.flag_mark) P.flag_mark[t] = 1;
         }
         return;
     }
@@ -564,6 +565,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     auto flag_target = [&]() {
         const int slot = atomicAdd(P.flag_len, 1);
         P.flag_list[slot] = (int32_t)t;
+        if (P.flag_mark) P.flag_mark[t] = 1;
     };
     if (!OCT) {
         // a candidate left out with the boundary key: the exact order decides
