@@ -441,6 +441,7 @@ struct KWork {
     DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
     DBuf keys, keys_out, vals, perm, sort_tmp, flags, h_tdrift, blk, g_w, g_b0;
     DBuf mark;          // [T] bytes: 1 for targets flagged for the kd-tree replay
+    DBuf sel_tmp;       // selection workspace of the unflagged targets
     SideStream side;
 };
 
@@ -1131,8 +1132,10 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     }
     // k-nearest and radius neighbourhoods: the kd-tree replay of flagged
     // targets overlaps the solve of the others (octant neighbourhoods are
-    // solved in size buckets, which take every target at once)
-    const bool side = K->has_tree && K->per_oct == 0;
+    // solved in size buckets, which take every target at once).  Small
+    // calls keep it in line: the extra launches and events would cost more
+    // than the overlap saves.
+    const bool side = K->has_tree && K->per_oct == 0 && T >= 65536;
     int32_t st = run_neighbors(K, W, d_targets, T, s, side);
     if (st != GS_OK) return st;
     if (timed && K->timing) HIP_TRY(hipEventRecord(ev[1], s));
@@ -1240,12 +1243,18 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         if (side) {
             // the unflagged targets while the replay runs, then the flagged
             // ones once their lists are in
-            p.skip = W.mark.as<uint8_t>();
+            HIP_TRY(W.lists.ensure(sizeof(int32_t) * T));
+            const size_t sel_bytes = gs::select_workspace(T);
+            HIP_TRY(W.sel_tmp.ensure(sel_bytes));
+            HIP_TRY(gs::launch_select_unmarked(W.perm.as<int32_t>(), W.mark.as<uint8_t>(), T,
+                                               W.lists.as<int32_t>(), W.lens.as<int32_t>() + 5,
+                                               W.sel_tmp.p, sel_bytes, s));
+            p.list = W.lists.as<int32_t>();
+            p.list_len = W.lens.as<int32_t>() + 5;
             int32_t st = solve(pad_nb(K->nb_bound), p, T);
             if (st != GS_OK) return st;
             HIP_TRY(hipStreamWaitEvent(s, W.side.replayed, 0));
             gs::SolveParams pf = p;
-            pf.skip = nullptr;
             pf.list = W.flags.as<int32_t>();
             pf.list_len = W.flags.as<int32_t>() + T;
             return solve(pad_nb(K->nb_bound), pf, std::min<int64_t>(T, (int64_t)256 * 64 * 4));

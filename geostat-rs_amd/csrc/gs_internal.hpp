@@ -87,7 +87,6 @@ struct SolveParams {
     const int32_t *count;    // [T]
     const int32_t *list;     // optional bucket list of target ids
     const int32_t *list_len; // device counter for list
-    const uint8_t *skip;     // optional per-target mask: nonzero targets are left alone
     int64_t T;
     int32_t stride;
     int32_t min_nb;
@@ -209,6 +208,12 @@ hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStre
 // perm: processing order of the targets (NULL: index order)
 hipError_t launch_bucket(const int32_t *count, const int32_t *perm, int64_t T, int32_t *lists,
                          int32_t *lens, int64_t list_cap, hipStream_t s);
+// out = the entries t of perm[0..T) with mark[t] == 0, in perm's order;
+// *out_len their number.  tmp: select_workspace(T) bytes.
+size_t select_workspace(int64_t T);
+hipError_t launch_select_unmarked(const int32_t *perm, const uint8_t *mark, int64_t T,
+                                  int32_t *out, int32_t *out_len, void *tmp, size_t tmp_bytes,
+                                  hipStream_t s);
 hipError_t launch_vecchia(int D, int NB, int variant, bool lean, const VecchiaParams &p,
                           int64_t n_blocks, hipStream_t s);
 // Vecchia log-likelihood gradient (partial holds 4 columns per model:

@@ -636,7 +636,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
     for (int64_t sys = (int64_t)blockIdx.x * GPW + grp; sys < nsys;
          sys += (int64_t)gridDim.x * GPW) {
         const int64_t t = P.list ? (int64_t)P.list[sys] : sys;
-        if (P.skip && P.skip[t]) continue;   // solved by a later launch
         const int cnt = P.count[t];
         if (cnt < P.min_nb || cnt < M || cnt > NB) {
             if (g == 0) {
