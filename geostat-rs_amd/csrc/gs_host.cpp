@@ -1130,12 +1130,10 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         for (auto &e : ev) HIP_TRY(K->take_event(&e));
         HIP_TRY(hipEventRecord(ev[0], s));
     }
-    // k-nearest and radius neighbourhoods: the kd-tree replay of flagged
-    // targets overlaps the solve of the others (octant neighbourhoods are
-    // solved in size buckets, which take every target at once).  Small
-    // calls keep it in line: the extra launches and events would cost more
-    // than the overlap saves.
-    const bool side = K->has_tree && K->per_oct == 0 && T >= 65536;
+    // The kd-tree replay of flagged targets overlaps the solve of the
+    // others.  Small calls keep it in line: the extra launches and events
+    // would cost more than the overlap saves.
+    const bool side = K->has_tree && T >= 65536;
     int32_t st = run_neighbors(K, W, d_targets, T, s, side);
     if (st != GS_OK) return st;
     if (timed && K->timing) HIP_TRY(hipEventRecord(ev[1], s));
@@ -1217,21 +1215,33 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         // Uneven sizes: bucket by padded system size, one launch per bucket.
         HIP_TRY(W.lists.ensure(sizeof(int32_t) * 3 * T));
         HIP_TRY(W.lens.ensure(sizeof(int32_t) * 8));
-        HIP_TRY(gs::launch_zero_i32(W.lens.as<int32_t>(), 4, s));
-        HIP_TRY(gs::launch_bucket(W.count.as<int32_t>(), K->has_tree ? W.perm.as<int32_t>() : nullptr,
-                                  T, W.lists.as<int32_t>(), W.lens.as<int32_t>(), T, s));
-        const int nbs[3] = {16, 32, 64};
-        for (int b = 0; b < 3; b++) {
-            if (nbs[b] > pad_nb(K->nb_bound)) break;
-            gs::SolveParams pb = p;
-            pb.list = W.lists.as<int32_t>() + (int64_t)b * T;
-            pb.list_len = W.lens.as<int32_t>() + b;
-            // persistent grid: enough groups to fill the device, grid-strided
-            int64_t groups = std::min<int64_t>(T, (int64_t)256 * 64 * 4);
-            int32_t st = solve(nbs[b], pb, groups);
+        auto bucket_and_solve = [&](const int32_t *perm, const int32_t *n_dev,
+                                    const uint8_t *skip) -> int32_t {
+            HIP_TRY(gs::launch_zero_i32(W.lens.as<int32_t>(), 4, s));
+            HIP_TRY(gs::launch_bucket(W.count.as<int32_t>(), perm, T, W.lists.as<int32_t>(),
+                                      W.lens.as<int32_t>(), T, s, n_dev, skip));
+            const int nbs[3] = {16, 32, 64};
+            for (int b = 0; b < 3; b++) {
+                if (nbs[b] > pad_nb(K->nb_bound)) break;
+                gs::SolveParams pb = p;
+                pb.list = W.lists.as<int32_t>() + (int64_t)b * T;
+                pb.list_len = W.lens.as<int32_t>() + b;
+                // persistent grid: enough groups to fill the device, grid-strided
+                int64_t groups = std::min<int64_t>(T, (int64_t)256 * 64 * 4);
+                int32_t st = solve(nbs[b], pb, groups);
+                if (st != GS_OK) return st;
+            }
+            return GS_OK;
+        };
+        if (side) {
+            // the unflagged targets while the replay runs, then the flagged
+            // ones (same lists, reused in stream order)
+            int32_t st = bucket_and_solve(W.perm.as<int32_t>(), nullptr, W.mark.as<uint8_t>());
             if (st != GS_OK) return st;
+            HIP_TRY(hipStreamWaitEvent(s, W.side.replayed, 0));
+            return bucket_and_solve(W.flags.as<int32_t>(), W.flags.as<int32_t>() + T, nullptr);
         }
-        return GS_OK;
+        return bucket_and_solve(K->has_tree ? W.perm.as<int32_t>() : nullptr, nullptr, nullptr);
     }
     if (K->has_tree) {
         // process targets in the search's Hilbert order: neighbouring systems

@@ -205,9 +205,12 @@ hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, int64_t n, double value, hipStream_t s);
 // *dst = min(*src, cap) on the device (stream-ordered)
 hipError_t launch_min_i32(const int32_t *src, int32_t cap, int32_t *dst, hipStream_t s);
-// perm: processing order of the targets (NULL: index order)
+// perm: processing order of the targets (NULL: index order); n_dev: device
+// count of perm's entries (NULL: T); skip: per-target mask of targets to
+// leave out (NULL: none).
 hipError_t launch_bucket(const int32_t *count, const int32_t *perm, int64_t T, int32_t *lists,
-                         int32_t *lens, int64_t list_cap, hipStream_t s);
+                         int32_t *lens, int64_t list_cap, hipStream_t s,
+                         const int32_t *n_dev = nullptr, const uint8_t *skip = nullptr);
 // out = the entries t of perm[0..T) with mark[t] == 0, in perm's order;
 // *out_len their number.  tmp: select_workspace(T) bytes.
 size_t select_workspace(int64_t T);

@@ -16,10 +16,13 @@ constexpr int kW = 64;
 // one run per bucket with a single atomic and keeps its targets' order, so
 // a bucket list is a concatenation of 64-target runs in processing order.
 __global__ void bucket_kernel(const int32_t *count, const int32_t *perm, int64_t T,
-                              int32_t *lists, int32_t *lens, int64_t cap) {
+                              const int32_t *n_dev, const uint8_t *skip, int32_t *lists,
+                              int32_t *lens, int64_t cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < T;
+    const int64_t n = n_dev ? (int64_t)*n_dev : T;
+    bool live = i < n;
     const int32_t t = live ? (perm ? perm[i] : (int32_t)i) : 0;
+    if (live && skip && skip[t]) live = false;
     const int c = live ? count[t] : 0;
     const int b = c <= 16 ? 0 : (c <= 32 ? 1 : 2);
     const unsigned lane = threadIdx.x & 63u;
@@ -228,10 +231,11 @@ hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_
 }
 
 hipError_t launch_bucket(const int32_t *count, const int32_t *perm, int64_t T, int32_t *lists,
-                         int32_t *lens, int64_t list_cap, hipStream_t s) {
+                         int32_t *lens, int64_t list_cap, hipStream_t s, const int32_t *n_dev,
+                         const uint8_t *skip) {
     if (T == 0) return hipSuccess;
     hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, count,
-                       perm, T, lists, lens, list_cap);
+                       perm, T, n_dev, skip, lists, lens, list_cap);
     return hipGetLastError();
 }
 
