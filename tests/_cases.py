@@ -94,7 +94,9 @@ def drift_basis(coords, method, p):
 def exact_estimate(oracle, coords, values, model, method, target, nb, digits=40):
     """The bordered kriging system (kriging.rs:527-644) with entries from the
     oracle's gamma, solved in `digits`-digit arithmetic: the exact answer the
-    double-precision solvers approximate.  Returns (value, variance)."""
+    double-precision solvers approximate.  Returns (value, variance,
+    multiplier), the multiplier being the solution's first drift entry (NaN
+    without drift)."""
     import mpmath as mp
     mp.mp.dps = digits
     c0 = model.total_sill()
@@ -119,7 +121,8 @@ def exact_estimate(oracle, coords, values, model, method, target, nb, digits=40)
     else:
         val = sum(w[a] * values[nb[a]] for a in range(n))
         var = c0 - sum(w[a] * b[a] for a in range(n + m))
-    return float(val), max(float(var), 0.0)
+    lag = float(w[n]) if m > 0 else float("nan")
+    return float(val), max(float(var), 0.0), lag
 
 
 MODELS_2D = {

@@ -111,7 +111,7 @@ def run_case(oracle, seed):
                                    block=True))
 
 
-@pytest.mark.parametrize("seed", range(32))
+@pytest.mark.parametrize("seed", range(128))
 def test_random_extended_predictor_matches_oracle(oracle, seed):
     variant, D, c, _, model, cfg, _, _ = _case(seed)
     print(f"extended sweep seed {seed}: {variant} D={D} n={len(c)} "

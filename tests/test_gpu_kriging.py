@@ -36,7 +36,7 @@ def closer_to_exact(oracle, coords, values, model, config, targets, got, want, w
     least as close as the oracle (the reference's LU) to the exact solution
     of the same system, computed in 40-digit arithmetic."""
     c0 = model.total_sill()
-    floor = 1e-3 * (np.sqrt(c0) if which == 0 else c0)
+    floor = 1e-3 * (np.sqrt(c0) if which == 0 else c0)     # value; variance or multiplier
     scale = max(np.abs(want[~np.isnan(want)]).max(), floor)
     worst = np.argsort(-np.nan_to_num(np.abs(got - want)))[:8]
     _, idx, cnt = oracle.neighbors(coords, values, model, config, targets[worst],
@@ -60,9 +60,9 @@ def check_predict(oracle, coords, values, model, config, targets, prec="fp64", t
         if rel_inf(got, want, floor) >= tol:
             closer_to_exact(oracle, coords, values, model, config, targets, got, want, which)
     if config.method.kind != 0 and rel_inf(lag, olag) >= tol * 10:
-        # the multiplier of an ill-conditioned drift block: bound by the
-        # same conditioning the value check established
-        assert rel_inf(lag, olag) < 1e-6
+        # the multiplier of an ill-conditioned drift block: at least as close
+        # as the oracle's to the exact solution's
+        closer_to_exact(oracle, coords, values, model, config, targets, lag, olag, 2)
     return k, v, var
 
 

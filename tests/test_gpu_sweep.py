@@ -68,7 +68,7 @@ def _case(seed):
     return D, c, v, model, cfg, t
 
 
-@pytest.mark.parametrize("seed", range(96))
+@pytest.mark.parametrize("seed", range(256))
 def test_random_configuration_matches_oracle(oracle, seed):
     D, c, v, model, cfg, t = _case(seed)
     print(f"sweep seed {seed}: D={D} n={len(c)} structures={len(model.structures)} "
