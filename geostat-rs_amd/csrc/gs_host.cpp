@@ -2040,6 +2040,19 @@ int host_threads() {
 
 extern "C" {
 
+int32_t gs_maxmin_order(int32_t dim, const double *coords, int64_t n, int64_t *order) {
+    if (dim != 2 && dim != 3) return fail(GS_ERR_DIMENSION_MISMATCH, "dim must be 2 or 3");
+    if (n < 0) return fail(GS_ERR_INVALID_PARAMETER, "n < 0");
+    if (n == 0) return GS_OK;
+    if (!coords || !order) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    for (int64_t i = 0; i < n * dim; i++)
+        if (!std::isfinite(coords[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite coordinate");
+    std::vector<int64_t> o;
+    maxmin_order(dim, coords, n, o);
+    std::copy(o.begin(), o.end(), order);
+    return GS_OK;
+}
+
 int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int64_t n, int64_t m,
                             const int64_t *order, gs_vplan **out) {
     if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");

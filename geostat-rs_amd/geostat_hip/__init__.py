@@ -763,6 +763,17 @@ def vecchia_mle(coords, values, kind, m: int, order=None,
     return VecchiaPlan(coords, m, order, ctx).mle(values, kind)
 
 
+def maxmin_order(coords) -> np.ndarray:
+    """maxmin_order (vecchia.rs:144-215): the maxmin ordering of the points
+    (host computation)."""
+    c = _f64(coords)
+    if c.ndim != 2 or c.shape[1] not in (2, 3):
+        raise DimensionMismatch(f"coords must be (n, 2|3), got {c.shape}")
+    out = np.empty(c.shape[0], dtype=np.int64)
+    _check(_abi.lib().gs_maxmin_order(c.shape[1], _abi.dptr(c), c.shape[0], _abi.i64ptr(out)))
+    return out
+
+
 def vecchia_plan(coords, m: int, order=None, ctx: Optional[Context] = None) -> VecchiaPlan:
     """vecchia_plan (vecchia.rs:232-282)."""
     return VecchiaPlan(coords, m, order, ctx)

@@ -237,6 +237,13 @@ int32_t gs_cv_block(gs_ctx *ctx, int32_t dim, const double *coords, const double
 
 /* ---- Vecchia ------------------------------------------------------ */
 
+/* maxmin_order (vecchia.rs:144-215): the maxmin (farthest-point) ordering of
+ * n finite points, written to order[n].  Host computation, no device work:
+ * the first point is the one nearest to the centroid, then each next point
+ * is the one farthest from those already ordered, ties to the larger
+ * index. */
+int32_t gs_maxmin_order(int32_t dim, const double *coords, int64_t n, int64_t *order);
+
 /* vecchia_plan (vecchia.rs:232-282): maxmin ordering (order == NULL) or the
  * given permutation, then each ordered point's m nearest predecessors. */
 int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int64_t n, int64_t m,
