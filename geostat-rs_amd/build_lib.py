@@ -89,7 +89,7 @@ def _gen_sources() -> list[Path]:
 
 
 def _deps_mtime() -> float:
-    hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h")]
+    hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".inc")]
     hdrs.append(HERE.parent / "include" / "geostat_hip.h")
     return max(p.stat().st_mtime for p in hdrs)
 

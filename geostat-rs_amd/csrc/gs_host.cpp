@@ -20,7 +20,6 @@
 #include <mutex>
 #include <string>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/geostat_hip.h"
@@ -326,72 +325,60 @@ int tree_depth(int64_t n) {
 
 int pad_nb(int64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
 
-// ---- group-size tuner ------------------------------------------------------
-// Each solve kernel is compiled for two group sizes (lanes per system).  The
-// first launch of a configuration times both on the call's own inputs and
-// caches the faster one per (device, kernel family, dim, NB, drift terms,
-// model signature); GS_SOLVE_VARIANT=0|1 pins the choice.
-std::mutex g_tune_mu;
-std::unordered_map<std::string, int> g_tune;
+// ---- launch schedule table -------------------------------------------------
+// Each solve kernel is compiled for two group sizes (lanes per system), and
+// list-driven solves run on a persistent grid.  The choice per (kernel
+// family, dim, padded size, drift terms, model class, extended systems) comes
+// from a table written by the project's autotuner (geostat_hip/tune.py, run
+// on an MI355X; gs_tune_table.inc), so it is the same in every process and a
+// result never depends on which process measured what: the two group sizes
+// reduce in different orders, and their outputs differ in the last bits.
+// Shapes missing from the table take group-size variant 0 and 16
+// wavefronts per CU.  GS_SOLVE_VARIANT=0|1 pins the variant (the tuner's
+// own measurements use it).
+struct TuneEntry {
+    const char *family;     // "krige" or "vecchia"
+    int D, NB, M, lean, ext;
+    int variant;            // group-size variant
+    int waves_per_cu;       // persistent grid of list-driven solves
+};
 
-std::string model_signature(const gs_model &m) {
-    std::string s;
-    for (int i = 0; i < m.n_structures; i++)
-        s += std::to_string(m.structures[i].kind) + (m.structures[i].has_anis ? "a" : "i") + ",";
-    return s;
+const TuneEntry kTuneTable[] = {
+#include "gs_tune_table.inc"
+    {nullptr, 0, 0, 0, 0, 0, 0, 0},
+};
+
+struct Schedule {
+    int variant = 0;
+    int waves_per_cu = 16;
+};
+
+Schedule schedule_for(const char *family, int D, int NB, int M, bool lean, bool ext) {
+    Schedule sc;
+    for (const TuneEntry *e = kTuneTable; e->family; e++) {
+        if (std::strcmp(e->family, family) == 0 && e->D == D && e->NB == NB && e->M == M &&
+            e->lean == (int)lean && e->ext == (int)ext) {
+            sc.variant = e->variant;
+            sc.waves_per_cu = e->waves_per_cu;
+            break;
+        }
+    }
+    // only the lean, non-extended kernels are compiled for both group sizes
+    if (!lean || ext) sc.variant = 0;
+    if (const char *env = std::getenv("GS_SOLVE_VARIANT")) sc.variant = std::atoi(env) ? 1 : 0;
+    if (const char *env = std::getenv("GS_WAVES_PER_CU")) {
+        const int w = std::atoi(env);
+        if (w > 0 && w <= 64) sc.waves_per_cu = w;
+    }
+    return sc;
 }
 
-template <class Launch>
-int32_t tuned_variant(const std::string &key, hipStream_t s, Launch &&launch, int *out) {
-    if (const char *env = std::getenv("GS_SOLVE_VARIANT")) {
-        *out = std::atoi(env) ? 1 : 0;
-        return GS_OK;
-    }
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
-            *out = it->second;
-            return GS_OK;
-        }
-    }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    float best = -1.f;
-    int pick = 0;
-    for (int v = 0; v < gs::kVariants; v++) {
-        float ms = 0.f;
-        bool built = true;
-        for (int rep = 0; rep < 2 && built; rep++) {   // the first run warms caches and code
-            HIP_TRY(hipEventRecord(e0, s));
-            hipError_t e = launch(v);
-            if (e == hipErrorInvalidDeviceFunction) {   // variant not compiled for this class
-                (void)hipGetLastError();
-                built = false;
-                break;
-            }
-            if (e != hipSuccess) {
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                return fail(e == hipErrorInvalidValue ? GS_ERR_INVALID_PARAMETER : GS_ERR_DEVICE,
-                            std::string("solve launch: ") + hipGetErrorString(e));
-            }
-            HIP_TRY(hipEventRecord(e1, s));
-            HIP_TRY(hipEventSynchronize(e1));
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        }
-        if (built && (best < 0.f || ms < best)) {
-            best = ms;
-            pick = v;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = pick;
-    *out = pick;
-    return GS_OK;
+int device_cus(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return cus;
 }
 
 }  // namespace
@@ -1182,30 +1169,15 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     const bool ext = p.err || p.xdrift || p.offsets;
     if (precision != GS_PREC_FP64)
         return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
-    int device = K->ctx->device;
-    auto solve = [&](int NB, const gs::SolveParams &pp, int64_t groups) -> int32_t {
-        std::string key = "krige:" + std::to_string(device) + ":" + std::to_string(K->D) + ":" +
-                          std::to_string(NB) + ":" + std::to_string(K->M) + ":" +
-                          model_signature(K->model) + (pp.list ? "list" : "") + (ext ? "x" : "");
-        // the tuner times a bounded prefix of the systems (their outputs
-        // are rewritten by the real launch below)
-        constexpr int32_t kTuneSystems = 131072;
-        gs::SolveParams tp = pp;
-        int variant = 0;
-        int32_t st = tuned_variant(key, s, [&](int v) {
-            if (pp.list) {
-                int32_t *tune_len = W.lens.as<int32_t>() + 4;
-                hipError_t e = gs::launch_min_i32(pp.list_len, kTuneSystems, tune_len, s);
-                if (e != hipSuccess) return e;
-                tp.list_len = tune_len;
-            } else {
-                tp.T = std::min<int64_t>(pp.T, kTuneSystems);
-            }
-            return gs::launch_krige_fp64(K->D, NB, K->M, v, K->lean, ext, tp,
-                                         std::min<int64_t>(groups, kTuneSystems), s);
-        }, &variant);
-        if (st != GS_OK) return st;
-        hipError_t e = gs::launch_krige_fp64(K->D, NB, K->M, variant, K->lean, ext, pp, groups, s);
+    const int cus = device_cus(K->ctx->device);
+    // groups = 0: one group per system; otherwise a persistent grid of the
+    // table's wavefronts per CU, grid-strided over the list
+    auto solve = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
+        const Schedule sc = schedule_for("krige", K->D, NB, K->M, K->lean, ext);
+        const int gpw = 64 / gs::group_size(NB, sc.variant);
+        int64_t groups = pp.T;
+        if (persistent) groups = std::min<int64_t>(pp.T, (int64_t)cus * sc.waves_per_cu * gpw);
+        hipError_t e = gs::launch_krige_fp64(K->D, NB, K->M, sc.variant, K->lean, ext, pp, groups, s);
         if (e == hipErrorInvalidValue)
             return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size");
         HIP_TRY(e);
@@ -1226,9 +1198,7 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                 gs::SolveParams pb = p;
                 pb.list = W.lists.as<int32_t>() + (int64_t)b * T;
                 pb.list_len = W.lens.as<int32_t>() + b;
-                // persistent grid: enough groups to fill the device, grid-strided
-                int64_t groups = std::min<int64_t>(T, (int64_t)256 * 64 * 4);
-                int32_t st = solve(nbs[b], pb, groups);
+                int32_t st = solve(nbs[b], pb, true);
                 if (st != GS_OK) return st;
             }
             return GS_OK;
@@ -1261,16 +1231,16 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                                                W.sel_tmp.p, sel_bytes, s));
             p.list = W.lists.as<int32_t>();
             p.list_len = W.lens.as<int32_t>() + 5;
-            int32_t st = solve(pad_nb(K->nb_bound), p, T);
+            int32_t st = solve(pad_nb(K->nb_bound), p, false);
             if (st != GS_OK) return st;
             HIP_TRY(hipStreamWaitEvent(s, W.side.replayed, 0));
             gs::SolveParams pf = p;
             pf.list = W.flags.as<int32_t>();
             pf.list_len = W.flags.as<int32_t>() + T;
-            return solve(pad_nb(K->nb_bound), pf, std::min<int64_t>(T, (int64_t)256 * 64 * 4));
+            return solve(pad_nb(K->nb_bound), pf, true);
         }
     }
-    return solve(pad_nb(K->nb_bound), p, T);
+    return solve(pad_nb(K->nb_bound), p, false);
 }
 
 }  // namespace
@@ -2264,15 +2234,7 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
         vp.partial = p->partial.as<double>();
         vp.status = p->status.as<int32_t>() + b0;
         vp.points_per_block = ppb;
-        std::string key = "vecchia:" + std::to_string(p->ctx->device) + ":" + std::to_string(p->D) +
-                          ":" + std::to_string(NB) + ":" + std::to_string(nm) + ":" +
-                          model_signature(models[b0]) + (lean ? "lean" : "gen");
-        int variant = 0;
-        const int64_t tune_blocks = std::min<int64_t>(n_blocks, 4096);
-        int32_t stt = tuned_variant(key, s, [&](int v) {
-            return gs::launch_vecchia(p->D, NB, v, lean, vp, tune_blocks, s);
-        }, &variant);
-        if (stt != GS_OK) return stt;
+        const int variant = schedule_for("vecchia", p->D, NB, 0, lean, false).variant;
         hipError_t e = gs::launch_vecchia(p->D, NB, variant, lean, vp, n_blocks, s);
         if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
         HIP_TRY(e);

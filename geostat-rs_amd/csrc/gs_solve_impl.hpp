@@ -341,13 +341,15 @@ constexpr int stage_slots() { return NB == 64 ? 16 : NB / 2; }
 // forward substitution of the Q right-hand sides in y.  On return y holds
 // L^-1 b, invd[r] = 1/L_ii of the owned row of block r, and a[r][c] for c
 // below the row holds L_ic * L_cc (the multipliers before scaling).
-// Returns false when a pivot falls below 1e-12 * c0 (the reference's
-// singular threshold relative to the matrix scale, linalg.rs:36-57,
-// 135-140).
-template <int NB, int G, int Q, bool DADD = false>
+// Returns false when a pivot fails the singular test against `thresh`:
+// LU (kriging, linalg.rs:42-57) fails when the pivot is below 1e-12 of the
+// matrix scale, so GE keeps pivots >= thresh; the Cholesky of the Vecchia
+// conditionals (linalg.rs:135-140) fails unless the pivot exceeds 1e-12 of
+// the largest diagonal entry (GE = false).
+template <int NB, int G, int Q, bool DADD = false, bool GE = true>
 __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (&y)[Q][NB / G],
-                                               const double c0, double *col, const int g,
-                                               double (&invd)[NB / G],
+                                               const double c0, const double thresh, double *col,
+                                               const int g, double (&invd)[NB / G],
                                                const double *dadd = nullptr) {
     constexpr int R = NB / G;
     // dg[r]: the diagonal entry of the owned row of block r, updated as
@@ -364,14 +366,13 @@ __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (
         else dg[r] = c0;
         invd[r] = 0.0;
     }
-    const double thresh = 1e-12 * c0;
     bool ok = true;
     static_for<0, NB>([&](auto JJ) {
         constexpr int j = decltype(JJ)::value;
         constexpr int oj = owner_of<G>(j);
         constexpr int rj = j / G;
         const double d = bcast<G, oj>(dg[rj]);
-        ok = ok && (d > thresh);
+        ok = ok && (GE ? d >= thresh : d > thresh);
         const double inv = rsqrt_nr(d);
         invd[rj] = (g == oj) ? inv : invd[rj];
         double l[R];
@@ -419,18 +420,18 @@ __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (
 // target; y[0] is filled here with the covariance to the target.  Entries
 // are c0 - gamma(dh) with gamma = nugget + sum sill g (model.rs:636-646);
 // gamma(0) = 0, so a target on a data point gets c0.  Returns false when a
-// pivot falls below 1e-12 * c0 (the reference's singular threshold relative
-// to the matrix scale, linalg.rs:36-57, 135-140).
+// pivot fails the singular test against `thresh` (factor_forward).
 //
 // With X: ea[r] is added to the diagonal of the owned row of block r
 // (measurement error), and when n_off > 0 the target column is the block
 // average (1/U) sum_u C(x_i - (tg + off_u)) instead (kriging.rs:709-725).
-template <int D, int NB, int G, int Q, bool LEAN, bool X = false>
+template <int D, int NB, int G, int Q, bool LEAN, bool X = false, bool GE = true>
 __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const int cnt,
                                                    const double (&rowx)[NB / G][D],
                                                    const double *xy, const double *tg,
                                                    double *stage, double *col, const int lane,
-                                                   const int g_in, double (&y)[Q][NB / G],
+                                                   const int g_in, const double thresh,
+                                                   double (&y)[Q][NB / G],
                                                    const double *ea = nullptr,
                                                    const double *offs = nullptr,
                                                    const int n_off = 0) {
@@ -508,8 +509,26 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
     }
 
     double invd[R];
-    if constexpr (X) return factor_forward<NB, G, Q, true>(a, y, c0, col, g, invd, ea);
-    else return factor_forward<NB, G, Q>(a, y, c0, col, g, invd);
+    if constexpr (X) return factor_forward<NB, G, Q, true, GE>(a, y, c0, thresh, col, g, invd, ea);
+    else return factor_forward<NB, G, Q, false, GE>(a, y, c0, thresh, col, g, invd);
+}
+
+// Group maximum; every lane of the group ends with the same value.
+template <int G>
+__device__ __forceinline__ double gmax(double v) {
+    if constexpr (G >= 2) v = fmax(v, dpp_d<0xB1>(v));
+    if constexpr (G >= 4) v = fmax(v, dpp_d<0x4E>(v));
+    if constexpr (G >= 8) v = fmax(v, swz_d<0x1F | (4 << 10)>(v));
+    if constexpr (G >= 16) v = fmax(v, swz_d<0x1F | (8 << 10)>(v));
+    if constexpr (G >= 32) v = fmax(v, swz_d<0x1F | (16 << 10)>(v));
+    if constexpr (G >= 64) {
+        const double lo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 0));
+        const double hi = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 32),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 32));
+        v = fmax(lo, hi);
+    }
+    return v;
 }
 
 // Drift block by modified Gram-Schmidt on the augmented columns
@@ -522,9 +541,13 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
 // This never forms YF'YF, so the drift block's conditioning is not squared.
 // A column whose orthogonal remainder falls below 1e-12 of the largest
 // drift column norm makes the system singular (rank-deficient drift).
+// The reference's LU (linalg.rs:42-57) meets the drift block as the Schur
+// complement -S, S = F'C^-1 F = R'R: its partial-pivoting pivots of S must
+// reach `thresh` (1e-12 of the bordered matrix's largest entry) as well.
 template <int R, int G, int M, int Q>
 __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&fT)[M], double c0,
-                                              bool &ok, double &var, double &lag) {
+                                              const double thresh, bool &ok, double &var,
+                                              double &lag) {
     double rr[M][M], qb[M], qz[M];
     double n0 = 0.0;
     static_for<0, M>([&](auto JJ) {
@@ -572,6 +595,46 @@ __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&
             y[1][r] -= tz * y[2 + j][r];
         }
     });
+    {
+        double S[M][M];
+#pragma unroll
+        for (int i = 0; i < M; i++)
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k <= (i < j ? i : j); k++) acc += rr[k][i] * rr[k][j];
+                S[i][j] = acc;
+            }
+        static_for<0, M>([&](auto KK) {
+            constexpr int k = decltype(KK)::value;
+            double mx = fabs(S[k][k]);
+            int p = k;
+#pragma unroll
+            for (int i = k + 1; i < M; i++) {
+                const double v = fabs(S[i][k]);
+                p = v > mx ? i : p;
+                mx = v > mx ? v : mx;
+            }
+            ok = ok && !(mx < thresh);
+#pragma unroll
+            for (int i = k + 1; i < M; i++) {
+                const bool sw = p == i;
+#pragma unroll
+                for (int j = k; j < M; j++) {
+                    const double t = S[k][j];
+                    S[k][j] = sw ? S[i][j] : t;
+                    S[i][j] = sw ? t : S[i][j];
+                }
+            }
+#pragma unroll
+            for (int i = k + 1; i < M; i++) {
+                const double f = S[i][k] / S[k][k];
+#pragma unroll
+                for (int j = k + 1; j < M; j++) S[i][j] -= f * S[k][j];
+            }
+        });
+    }
     double pbz = 0.0, pbb = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -696,12 +759,32 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
                 for (int q = 0; q < M; q++) y[2 + q][r] = live ? f[q] : 0.0;
             }
         }
+        // Scale of the reference's singular test (linalg.rs:36-39): the
+        // largest |entry| of the bordered matrix, max(c0 + err_i, |F_iq|)
+        // over the live rows (covariances never exceed c0 in magnitude).
+        // Simple and ordinary kriging: c0, or max(c0, 1) for the constant
+        // drift column, uniform over the wavefront.
+        double amax = M == 1 ? fmax(c0, 1.0) : c0;
+        if constexpr (M > 1 || X) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (row_of<G>(r, g) < cnt) {
+                    if constexpr (X) amax = fmax(amax, c0 + ea[r]);
+#pragma unroll
+                    for (int q = 1; q < M; q++) amax = fmax(amax, fabs(y[2 + q][r]));
+                }
+            }
+            amax = gmax<G>(amax);
+        }
+        const double thresh = 1e-12 * amax;
         bool ok;
         if constexpr (X)
             ok = build_factor_solve<D, NB, G, Q, LEAN, true>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf,
-                                                             lane, g, y, ea, P.offsets, P.n_off);
+                                                             lane, g, thresh, y, ea, P.offsets,
+                                                             P.n_off);
         else
-            ok = build_factor_solve<D, NB, G, Q, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, y);
+            ok = build_factor_solve<D, NB, G, Q, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane,
+                                                       g, thresh, y);
         wave_sync();  // the group's LDS rows are rewritten by its next system
         // variance base: c0, or the within-block covariance for a block
         double vbase = c0;
@@ -732,7 +815,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
             } else {
                 basis<D, M>(P, tg, fT);
             }
-            value = drift_solve<R, G, M>(y, fT, vbase, ok, var, lag);
+            value = drift_solve<R, G, M>(y, fT, vbase, thresh, ok, var, lag);
         }
         if (g == 0) {
             if (ok) {
@@ -815,7 +898,8 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
                 double y[2][R];
 #pragma unroll
                 for (int r = 0; r < R; r++) y[1][r] = zr[r];
-                ok = build_factor_solve<D, NB, G, 2, LEAN>(Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, y);
+                ok = build_factor_solve<D, NB, G, 2, LEAN, false, false>(
+                    Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, 1e-12 * sill, y);
                 double s01 = 0.0, s00 = 0.0;
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -1041,7 +1125,7 @@ __global__ __launch_bounds__(kWave) void vecchia_grad_kernel(VecchiaParams P) {
                     rho[r] = live ? rv : 0.0;
                 }
                 double invd[R];
-                ok = factor_forward<NB, G, 2>(a, y, sill, colbuf, gg, invd);
+                ok = factor_forward<NB, G, 2, false, false>(a, y, sill, 1e-12 * sill, colbuf, gg, invd);
                 // back substitution L' x = y for both right-hand sides; a lane
                 // only needs x of its own rows
                 double x[2][R];

@@ -513,19 +513,51 @@ class Kriging:
         v, var, _, _ = self.predict_many(grid.centers(), precision)
         return v, var
 
+    def _check_device_args(self, targets, value, variance, lagrange, status):
+        """Shape, dtype, layout and device checks of the device-path buffers,
+        made before anything is enqueued: the kernels index them as
+        contiguous float64 (T, dim) / (T,) arrays and int32 statuses on the
+        handle's GPU."""
+        import torch
+        dev = self.ctx.device
+
+        def tensor(x, name, dtype):
+            if not isinstance(x, torch.Tensor):
+                raise InvalidParameter(f"{name} must be a torch tensor on cuda:{dev}")
+            if x.dtype != dtype:
+                raise InvalidParameter(f"{name} must be {dtype}, got {x.dtype}")
+            if x.device.type != "cuda" or x.device.index != dev:
+                raise InvalidParameter(f"{name} must be on cuda:{dev}, got {x.device}")
+            if not x.is_contiguous():
+                raise InvalidParameter(f"{name} must be contiguous")
+
+        tensor(targets, "targets", torch.float64)
+        if targets.dim() != 2 or targets.shape[1] != self.dim:
+            raise DimensionMismatch(f"targets must be (T, {self.dim}), got {tuple(targets.shape)}")
+        T = int(targets.shape[0])
+        for x, name, dtype in ((value, "value", torch.float64),
+                               (variance, "variance", torch.float64),
+                               (lagrange, "lagrange", torch.float64),
+                               (status, "status", torch.int32)):
+            if x is None and name in ("lagrange", "status"):
+                continue
+            tensor(x, name, dtype)
+            if x.numel() < T:
+                raise InvalidParameter(f"{name} holds {x.numel()} entries, need {T}")
+        return T
+
     def predict_many_device(self, targets, value, variance, lagrange=None, status=None,
                             precision="fp64", stream=None):
-        """Enqueues predict_many on device tensors (torch CUDA tensors or raw
-        pointers) on `stream` (a hipStream_t as an int, e.g. a torch stream's
-        cuda_stream; None or 0 is the null stream, torch's default stream);
-        returns immediately."""
-        def ptr(x):
-            if x is None:
-                return None
-            return x if isinstance(x, int) else x.data_ptr()
-        T = targets.shape[0] if hasattr(targets, "shape") else None
-        if T is None:
-            raise InvalidParameter("targets must have a shape")
+        """Enqueues predict_many on torch tensors resident on the handle's GPU
+        (targets (T, dim) float64, value / variance / lagrange float64 with at
+        least T entries, status int32) on `stream` (a hipStream_t as an int,
+        e.g. a torch stream's cuda_stream; None or 0 is the null stream,
+        torch's default stream); returns immediately.  Bad buffers raise
+        before any work is enqueued."""
+        if precision not in _PREC:
+            raise InvalidParameter(f"unknown precision {precision!r}")
+        T = self._check_device_args(targets, value, variance, lagrange, status)
+        ptr = (lambda x: None if x is None else x.data_ptr())
         _check(_abi.lib().gs_kriging_predict_many_device(
             self._h, ptr(targets), T, _PREC[precision], ptr(value), ptr(variance),
             ptr(lagrange), ptr(status), stream))

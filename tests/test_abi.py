@@ -101,3 +101,35 @@ def test_oracle_is_not_imported_by_the_product():
     for p in (ROOT / "geostat-rs_amd").rglob("*"):
         if p.suffix in (".py", ".cpp", ".hip", ".hpp", ".c", ".h"):
             assert "refcpu" not in p.read_text(errors="ignore"), p
+
+
+def test_device_path_rejects_bad_buffers_before_any_call():
+    # The checks run before the C call: a handle-less object is enough, and
+    # every case raises without touching a GPU.
+    import types
+
+    import torch
+
+    import geostat_hip as gh
+    k = gh.Kriging.__new__(gh.Kriging)
+    k.dim = 2
+    k.ctx = types.SimpleNamespace(device=0)
+    f64 = torch.float64
+    T = 8
+    good_t = torch.zeros((T, 2), dtype=f64)
+    outs = [torch.zeros(T, dtype=f64) for _ in range(3)] + [torch.zeros(T, dtype=torch.int32)]
+    cases = [
+        (good_t, outs),                                          # CPU tensors
+        (good_t.float(), outs),                                  # float32 targets
+        (good_t.reshape(-1), outs),                              # flat targets
+        (torch.zeros((T, 3), dtype=f64), outs),                  # 3-D targets on a 2-D handle
+        (good_t, [outs[0][:T - 1]] + outs[1:]),                  # short output
+        (good_t, outs[:3] + [torch.zeros(T, dtype=torch.int64)]),  # int64 status
+        (torch.zeros((2, T), dtype=f64).t(), outs),              # non-contiguous targets
+        ((good_t.numpy()), outs),                                # not a tensor
+    ]
+    for t, (v, var, lag, st) in cases:
+        with pytest.raises((gh.InvalidParameter, gh.DimensionMismatch)):
+            k.predict_many_device(t, v, var, lag, st)
+    with pytest.raises(gh.InvalidParameter):
+        k.predict_many_device(good_t, *outs, precision="fp16")

@@ -269,6 +269,23 @@ def test_singular_targets_are_nan_with_status(oracle):
     assert (ost == 3).all() and (st == 3).all() and np.isnan(val).all()
 
 
+@pytest.mark.parametrize("sill", [1e-6, 1e-4, 1e6, 3e6, 1e7, 1e8])
+@pytest.mark.parametrize("method", [gh.KrigingMethod.Simple(0.0), gh.KrigingMethod.Ordinary(),
+                                    gh.KrigingMethod.Universal(1), gh.KrigingMethod.Universal(2)])
+def test_singular_test_follows_matrix_scale(oracle, sill, method):
+    # The reference's LU is singular when a pivot falls below 1e-12 of the
+    # bordered matrix's largest entry (linalg.rs:36-57): with sills far from
+    # 1, the drift block's Schur pivots decide it (every target singular
+    # for ordinary kriging from a sill of about 1e7, for universal from
+    # about 1e6), and the NaN pattern must be the reference's.
+    c, v = workloads.points(5, 300, [1000.0, 1000.0])
+    t = workloads.uniforms(9, 400).reshape(200, 2) * 1000.0
+    for kind in (gh.ModelKind.SPHERICAL, gh.ModelKind.EXPONENTIAL):
+        m = gh.VariogramModel(0.1 * sill, [gh.Structure(kind, sill, 300.0)])
+        check_predict(oracle, c, v * np.sqrt(sill), m,
+                      gh.KrigingConfig(method=method, max_neighbors=32), t)
+
+
 # ------------------------------------------- size-independent properties --
 def test_full_size_properties():
     # At the full 200k x 4M configuration: determinism across calls, batch
