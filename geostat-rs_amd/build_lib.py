@@ -50,6 +50,10 @@ KRIGE += [(d, nb, m, GROUPS[nb][0], lean, 1)
           for nb in (16, 32, 64) for m in ms for lean in (1, 0)]
 VECCHIA = [(d, nb, g, lean) for d in (2, 3) for nb in (16, 32, 64) for lean in (1, 0)
            for g in (GROUPS[nb] if lean else GROUPS[nb][:1])]
+# FP32-refined kernels: lean models; kriging with M = 0 (simple) or 1
+# (ordinary); Vecchia conditionals.
+KRIGE32R = [(d, nb, m) for d in (2, 3) for nb in (16, 32, 64) for m in (0, 1)]
+VECCHIA32R = [(d, nb) for d in (2, 3) for nb in (16, 32, 64)]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
 # two row blocks per lane
 VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]
@@ -85,18 +89,41 @@ def _gen_sources() -> list[Path]:
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
         out.append(p)
+    for d, nb, m in KRIGE32R:
+        p = GEN / f"krige32r_d{d}_nb{nb}_m{m}.hip"
+        txt = (f'#include "{CSRC}/gs_fp32r_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_krige_fp32r_inst<{d}, {nb}, {m}>"
+               f"(const SolveParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
+    for d, nb in VECCHIA32R:
+        p = GEN / f"vecchia32r_d{d}_nb{nb}.hip"
+        txt = (f'#include "{CSRC}/gs_fp32r_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_vecchia_fp32r_inst<{d}, {nb}>"
+               f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
     return out
 
 
-def _deps_mtime() -> float:
-    hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".inc")]
-    hdrs.append(HERE.parent / "include" / "geostat_hip.h")
+def _deps_mtime(src: Path) -> float:
+    """Newest header a unit depends on: a generated solve unit includes
+    only the kernel templates; every other unit may include any header."""
+    kernel = ["gs_solve_impl.hpp", "gs_internal.hpp", "gs_device.hpp"]
+    if src.parent == GEN:
+        names = kernel + (["gs_fp32r_impl.hpp"] if "32r" in src.stem else [])
+        hdrs = [CSRC / n for n in names]
+    else:
+        hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".inc")]
+        hdrs.append(HERE.parent / "include" / "geostat_hip.h")
     return max(p.stat().st_mtime for p in hdrs)
 
 
 def _compile(src: Path, extra: list[str], incremental: bool = False) -> Path:
     obj = OBJ / (src.stem + ".o")
-    if incremental and obj.exists() and obj.stat().st_mtime > max(_deps_mtime(), src.stat().st_mtime):
+    if incremental and obj.exists() and obj.stat().st_mtime > max(_deps_mtime(src), src.stat().st_mtime):
         return obj
     cmd = [HIPCC, *COMMON, *extra, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)

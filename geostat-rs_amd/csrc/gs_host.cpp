@@ -429,6 +429,7 @@ struct KWork {
     DBuf keys, keys_out, vals, perm, sort_tmp, flags, h_tdrift, blk, g_w, g_b0;
     DBuf mark;          // [T] bytes: 1 for targets flagged for the kd-tree replay
     DBuf sel_tmp;       // selection workspace of the unflagged targets
+    DBuf fb;            // [T + 1] FP32-refined fallback list and its length
     SideStream side;
 };
 
@@ -495,6 +496,8 @@ struct gs_vplan {
     int32_t max_count = 0;
     double extent = 0.0;     // length of the bounding-box diagonal (MLE start scale)
     DBuf coords, d_order, d_nb, d_count, zc, models, partial, out, status;
+    // FP32-refined batches: per-model fallback flags and the FP64 re-run
+    DBuf refine_fail, models_redo, out_redo, status_redo;
     std::mutex mu;
 };
 
@@ -1107,11 +1110,8 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (T == 0) return GS_OK;
-    if (K->global) {
-        if (precision != GS_PREC_FP64)
-            return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
+    if (K->global)   // one shared FP64 factorization serves either precision
         return run_global(K, W, d_targets, T, d_value, d_var, d_lag, d_status, s, x);
-    }
     std::array<hipEvent_t, 3> ev{};
     if (timed && K->timing) {
         for (auto &e : ev) HIP_TRY(K->take_event(&e));
@@ -1132,6 +1132,21 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     }
     return GS_OK;
 }
+
+template <class Solve>
+int32_t solve_all(gs_kriging *K, KWork &W, gs::SolveParams p, int64_t T, hipStream_t s, bool side,
+                  Solve &&solve);
+
+// Smallest FP32 pivot trusted by the refined kriging path, relative to the
+// system size times the matrix scale: the FP32 solve error is about
+// cond * 6e-8, the refinement takes it to its square, and cond is at most
+// about N * scale / (smallest pivot), so 6e-4 keeps cond * 6e-8 <= 1e-4 and
+// the refined error near 1e-8.
+constexpr double kFallbackRatioKrige = 6e-4;
+// The Vecchia conditionals only enter the likelihood through sums of logs
+// and squares over many points: cond * 6e-8 <= 1e-3 (refined per-point
+// error near 1e-6, averaged over the points).
+constexpr double kFallbackRatioVecchia = 6e-5;
 
 int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                    int32_t precision, double *d_value, double *d_var, double *d_lag,
@@ -1167,12 +1182,21 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     p.n_off = x.n_off;
     p.cbb = x.cbb;
     const bool ext = p.err || p.xdrift || p.offsets;
-    if (precision != GS_PREC_FP64)
-        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined kriging is not built");
+    // FP32-refined solves cover lean models with simple or ordinary
+    // kriging; every other configuration is solved in FP64, which meets the
+    // refined path's tolerance by itself.
+    const bool r32 = precision == GS_PREC_FP32_REFINED && K->lean && !ext && K->M <= 1;
+    if (r32) {
+        HIP_TRY(W.fb.ensure(sizeof(int32_t) * (T + 1)));
+        p.fb_list = W.fb.as<int32_t>();
+        p.fb_len = W.fb.as<int32_t>() + T;
+        p.fb_ratio = kFallbackRatioKrige;
+        HIP_TRY(gs::launch_zero_i32(p.fb_len, 1, s));
+    }
     const int cus = device_cus(K->ctx->device);
-    // groups = 0: one group per system; otherwise a persistent grid of the
-    // table's wavefronts per CU, grid-strided over the list
-    auto solve = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
+    // persistent = false: one group per system; otherwise a persistent grid
+    // of the table's wavefronts per CU, grid-strided over the list
+    auto solve64 = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
         const Schedule sc = schedule_for("krige", K->D, NB, K->M, K->lean, ext);
         const int gpw = 64 / gs::group_size(NB, sc.variant);
         int64_t groups = pp.T;
@@ -1183,6 +1207,34 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         HIP_TRY(e);
         return GS_OK;
     };
+    auto solve = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
+        if (!r32) return solve64(NB, pp, persistent);
+        const Schedule sc = schedule_for("krige32r", K->D, NB, K->M, true, false);
+        const int gpw = 64 / gs::fp32r_group(NB);
+        int64_t groups = pp.T;
+        if (persistent) groups = std::min<int64_t>(pp.T, (int64_t)cus * sc.waves_per_cu * gpw);
+        hipError_t e = gs::launch_krige_fp32r(K->D, NB, K->M, pp, groups, s);
+        if (e == hipErrorInvalidValue)
+            return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size");
+        HIP_TRY(e);
+        return GS_OK;
+    };
+    int32_t st = solve_all(K, W, p, T, s, side, solve);
+    if (st != GS_OK || !r32) return st;
+    // systems the FP32 factorization did not trust, in FP64 (written in
+    // place; the list order does not affect any result)
+    gs::SolveParams pf = p;
+    pf.list = p.fb_list;
+    pf.list_len = p.fb_len;
+    return solve64(pad_nb(K->nb_bound), pf, true);
+}
+
+// The launch sequence of run_solves for one precision: size buckets for
+// uneven octant neighbourhoods, Hilbert order otherwise, and the flagged
+// targets after the kd-tree replay when it runs on the side stream.
+template <class Solve>
+int32_t solve_all(gs_kriging *K, KWork &W, gs::SolveParams p, int64_t T, hipStream_t s, bool side,
+                  Solve &&solve) {
     if (K->per_oct > 0 && K->nb_bound > 16) {
         // Uneven sizes: bucket by padded system size, one launch per bucket.
         HIP_TRY(W.lists.ensure(sizeof(int32_t) * 3 * T));
@@ -2185,8 +2237,6 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
     if (!values || !models || !out) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
-    if (precision != GS_PREC_FP64)
-        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined Vecchia");
     const int64_t n = p->n;
     for (int64_t i = 0; i < n; i++)
         if (!std::isfinite(values[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite value");
@@ -2235,6 +2285,57 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
         vp.status = p->status.as<int32_t>() + b0;
         vp.points_per_block = ppb;
         const int variant = schedule_for("vecchia", p->D, NB, 0, lean, false).variant;
+        if (precision == GS_PREC_FP32_REFINED && lean) {
+            // FP32-refined conditionals; a model with an untrusted one is
+            // evaluated again in FP64 below
+            HIP_TRY(p->refine_fail.ensure(sizeof(int32_t) * gs::kMaxModelsPerLaunch));
+            HIP_TRY(gs::launch_zero_i32(p->refine_fail.as<int32_t>(), nm, s));
+            vp.refine_fail = p->refine_fail.as<int32_t>();
+            vp.fb_ratio = kFallbackRatioVecchia;
+            hipError_t e = gs::launch_vecchia_fp32r(p->D, NB, vp, n_blocks, s);
+            if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
+            HIP_TRY(e);
+            HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm, p->out.as<double>() + b0, s));
+            std::vector<int32_t> bad((size_t)nm);
+            HIP_TRY(hipMemcpyAsync(bad.data(), p->refine_fail.p, sizeof(int32_t) * nm,
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            std::vector<int32_t> redo;
+            for (int32_t i = 0; i < nm; i++)
+                if (bad[(size_t)i]) redo.push_back(b0 + i);
+            if (redo.empty()) continue;
+            // the flagged models, packed, through the FP64 kernel
+            const int32_t nr = (int32_t)redo.size();
+            std::vector<DevModel> rm((size_t)nr);
+            for (int32_t i = 0; i < nr; i++) rm[(size_t)i] = dm[(size_t)redo[(size_t)i]];
+            HIP_TRY(p->models_redo.ensure(sizeof(DevModel) * nr));
+            HIP_TRY(p->out_redo.ensure(sizeof(double) * nr));
+            HIP_TRY(p->status_redo.ensure(sizeof(int32_t) * nr));
+            HIP_TRY(hipMemcpyAsync(p->models_redo.p, rm.data(), sizeof(DevModel) * nr,
+                                   hipMemcpyHostToDevice, s));
+            HIP_TRY(gs::launch_zero_i32(p->status_redo.as<int32_t>(), nr, s));
+            gs::VecchiaParams vr = vp;
+            vr.n_models = nr;
+            vr.models = p->models_redo.as<DevModel>();
+            vr.status = p->status_redo.as<int32_t>();
+            vr.refine_fail = nullptr;
+            HIP_TRY(gs::launch_vecchia(p->D, NB, variant, lean, vr, n_blocks, s));
+            HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nr,
+                                              p->out_redo.as<double>(), s));
+            std::vector<double> ro((size_t)nr);
+            std::vector<int32_t> rs((size_t)nr);
+            HIP_TRY(hipMemcpyAsync(ro.data(), p->out_redo.p, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(rs.data(), p->status_redo.p, sizeof(int32_t) * nr, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            for (int32_t i = 0; i < nr; i++) {
+                HIP_TRY(hipMemcpyAsync(p->out.as<double>() + redo[(size_t)i], &ro[(size_t)i],
+                                       sizeof(double), hipMemcpyHostToDevice, s));
+                HIP_TRY(hipMemcpyAsync(p->status.as<int32_t>() + redo[(size_t)i], &rs[(size_t)i],
+                                       sizeof(int32_t), hipMemcpyHostToDevice, s));
+            }
+            HIP_TRY(hipStreamSynchronize(s));
+            continue;
+        }
         hipError_t e = gs::launch_vecchia(p->D, NB, variant, lean, vp, n_blocks, s);
         if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
         HIP_TRY(e);
@@ -2269,8 +2370,7 @@ int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs
     if (!values || !models || !loglik || !grad) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
-    if (precision != GS_PREC_FP64)
-        return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: FP32-refined Vecchia");
+    // the gradient runs in FP64 for either precision
     if (p->max_count > 32)
         return fail(GS_ERR_INVALID_PARAMETER,
                     "unsupported on GPU: gradient with conditioning sets above 32 points");
@@ -2400,6 +2500,7 @@ using Clock = std::chrono::steady_clock;
 
 struct SReq {
     int kind = 0;   // 0 kriging, 1 Vecchia likelihood
+    int32_t precision = GS_PREC_FP64;
     gs_kriging *k = nullptr;
     const double *targets = nullptr;
     int64_t T = 0;
@@ -2481,11 +2582,11 @@ struct gs_server {
             }
             int32_t st;
             if (r->kind == 0)
-                st = predict_host(r->k, slot->w, r->targets, r->T, GS_PREC_FP64, r->value,
+                st = predict_host(r->k, slot->w, r->targets, r->T, r->precision, r->value,
                                   r->variance, r->lagrange, r->status, nullptr, Extra(),
                                   slot->stream, false);
             else
-                st = vecchia_batch(r->p, r->values, r->models, r->n_models, GS_PREC_FP64, r->out,
+                st = vecchia_batch(r->p, r->values, r->models, r->n_models, r->precision, r->out,
                                    r->vstatus, slot->stream);
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -2547,9 +2648,11 @@ void gs_server_destroy(gs_server *s) {
 }
 
 int32_t gs_server_submit_krige(gs_server *s, gs_kriging *k, const double *targets, int64_t T,
-                               double *value, double *variance, double *lagrange,
-                               int32_t *status, int64_t *ticket) {
+                               int32_t precision, double *value, double *variance,
+                               double *lagrange, int32_t *status, int64_t *ticket) {
     if (!s || !k || !ticket) return fail(GS_ERR_INVALID_PARAMETER, "NULL argument");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (T <= 0 || !targets || !value || !variance)
         return fail(GS_ERR_INVALID_PARAMETER, "empty request or NULL buffer");
     if (k->ctx != s->ctx) return fail(GS_ERR_INVALID_PARAMETER, "predictor belongs to another context");
@@ -2557,6 +2660,7 @@ int32_t gs_server_submit_krige(gs_server *s, gs_kriging *k, const double *target
         return fail(GS_ERR_INVALID_PARAMETER, "unsupported in the request loop: external drift");
     auto r = std::make_shared<SReq>();
     r->kind = 0;
+    r->precision = precision;
     r->k = k;
     r->targets = targets;
     r->T = T;
@@ -2578,14 +2682,17 @@ int32_t gs_server_submit_krige(gs_server *s, gs_kriging *k, const double *target
 }
 
 int32_t gs_server_submit_vecchia(gs_server *s, gs_vplan *p, const double *values,
-                                 const gs_model *models, int32_t n_models, double *out,
-                                 int32_t *status, int64_t *ticket) {
+                                 const gs_model *models, int32_t n_models, int32_t precision,
+                                 double *out, int32_t *status, int64_t *ticket) {
     if (!s || !p || !ticket) return fail(GS_ERR_INVALID_PARAMETER, "NULL argument");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (n_models <= 0 || !values || !models || !out)
         return fail(GS_ERR_INVALID_PARAMETER, "empty request or NULL buffer");
     if (p->ctx != s->ctx) return fail(GS_ERR_INVALID_PARAMETER, "plan belongs to another context");
     auto r = std::make_shared<SReq>();
     r->kind = 1;
+    r->precision = precision;
     r->p = p;
     r->values = values;
     r->models = models;

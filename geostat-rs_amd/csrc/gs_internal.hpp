@@ -108,6 +108,12 @@ struct SolveParams {
     const double *offsets;   // [n_off][D] block discretization, or null (point kriging)
     int32_t n_off;
     const double *cbb;       // device scalar: within-block covariance
+    // FP32-refined solves: systems whose FP32 factorization is not trusted
+    // (smallest pivot below fb_ratio * N * matrix scale, N the system size)
+    // are appended to fb_list (count in *fb_len) for the FP64 kernel.
+    int32_t *fb_list;
+    int32_t *fb_len;
+    double fb_ratio;
 };
 
 // Global neighbourhood above 64 points: one shared LU, per-target
@@ -175,6 +181,10 @@ struct VecchiaParams {
     double *partial;         // [n_blocks][n_models]
     int32_t *status;         // [n_models]
     int32_t points_per_block;
+    // FP32-refined batches: refine_fail[p] = 1 when a conditional of model p
+    // failed the FP32 pivot test (the model is recomputed in FP64)
+    int32_t *refine_fail;
+    double fb_ratio;
 };
 
 hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s);
@@ -197,8 +207,19 @@ hipError_t launch_krige_fp64(int D, int NB, int M, int variant, bool lean, bool 
 // *out = within-block covariance of the n_off discretization offsets.
 hipError_t launch_block_cbb(int D, bool lean, const DevModel *model, const double *offs,
                             int32_t n_off, double *out, hipStream_t s);
+// FP32-refined kriging (lean models, simple / ordinary kriging, no extended
+// systems): hipErrorInvalidValue for other shapes.
 hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
                               hipStream_t s);
+// FP32-refined Vecchia batch (lean models).
+hipError_t launch_vecchia_fp32r(int D, int NB, const VecchiaParams &p, int64_t n_blocks,
+                                hipStream_t s);
+template <int D, int NB, int M>
+hipError_t launch_krige_fp32r_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
+template <int D, int NB>
+hipError_t launch_vecchia_fp32r_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
+// Lanes per system of the FP32-refined kernels.
+constexpr int fp32r_group(int NB) { return NB == 16 ? 4 : (NB == 32 ? 16 : 32); }
 hipError_t launch_iota_nbr(int32_t *nbr, int32_t *count, int64_t T, int32_t n, int32_t stride,
                            hipStream_t s);
 hipError_t launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);

@@ -224,9 +224,36 @@ hipError_t launch_block_cbb(int D, bool lean, const DevModel *model, const doubl
     return hipGetLastError();
 }
 
+template <int D, int M>
+static hipError_t krige32r_nb(int NB, const SolveParams &p, int64_t gg, hipStream_t s) {
+    switch (NB) {
+    case 16: return launch_krige_fp32r_inst<D, 16, M>(p, gg, s);
+    case 32: return launch_krige_fp32r_inst<D, 32, M>(p, gg, s);
+    case 64: return launch_krige_fp32r_inst<D, 64, M>(p, gg, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_krige_fp32r(int D, int NB, int M, const SolveParams &p, int64_t grid_groups,
                               hipStream_t s) {
-    (void)D; (void)NB; (void)M; (void)p; (void)grid_groups; (void)s;
+    if (D == 2 && M == 0) return krige32r_nb<2, 0>(NB, p, grid_groups, s);
+    if (D == 2 && M == 1) return krige32r_nb<2, 1>(NB, p, grid_groups, s);
+    if (D == 3 && M == 0) return krige32r_nb<3, 0>(NB, p, grid_groups, s);
+    if (D == 3 && M == 1) return krige32r_nb<3, 1>(NB, p, grid_groups, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_vecchia_fp32r(int D, int NB, const VecchiaParams &p, int64_t n_blocks,
+                                hipStream_t s) {
+    if (D == 2) {
+        if (NB == 16) return launch_vecchia_fp32r_inst<2, 16>(p, n_blocks, s);
+        if (NB == 32) return launch_vecchia_fp32r_inst<2, 32>(p, n_blocks, s);
+        if (NB == 64) return launch_vecchia_fp32r_inst<2, 64>(p, n_blocks, s);
+    } else if (D == 3) {
+        if (NB == 16) return launch_vecchia_fp32r_inst<3, 16>(p, n_blocks, s);
+        if (NB == 32) return launch_vecchia_fp32r_inst<3, 32>(p, n_blocks, s);
+        if (NB == 64) return launch_vecchia_fp32r_inst<3, 64>(p, n_blocks, s);
+    }
     return hipErrorInvalidValue;
 }
 

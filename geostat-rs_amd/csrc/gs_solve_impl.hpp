@@ -714,6 +714,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
 #pragma unroll
         for (int d = 0; d < D; d++) tg[d] = P.targets[t * D + d];
         const int32_t *nbl = P.nbr + t * (int64_t)P.stride;
+        // entries at or right of the diagonal are never read (their
+        // multipliers are selected away), so slots past the count stay as
+        // the previous system left them
         for (int j = g; j < cnt; j += G) {
             const int64_t id = nbl[j];
 #pragma unroll
@@ -870,10 +873,12 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
 #pragma unroll
         for (int d = 0; d < D; d++) tg[d] = P.coords[self * D + d];
         const double zi = P.zc[self];
-        for (int j = g; j < cnt; j += G) {
-            const int64_t id = nbl[j];
+        // slots past the count are zero: every entry a kernel evaluates,
+        // used or not, then comes from this system's own finite data
+        for (int j = g; j < NB; j += G) {
+            const int64_t id = j < cnt ? nbl[j] : 0;
 #pragma unroll
-            for (int d = 0; d < D; d++) xy[j * D + d] = P.coords[id * D + d];
+            for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt ? P.coords[id * D + d] : 0.0;
         }
         double zr[R];
 #pragma unroll
@@ -1045,10 +1050,12 @@ __global__ __launch_bounds__(kWave) void vecchia_grad_kernel(VecchiaParams P) {
 #pragma unroll
         for (int d = 0; d < D; d++) tg[d] = P.coords[self * D + d];
         const double zi = P.zc[self];
-        for (int j = g; j < cnt; j += G) {
-            const int64_t id = nbl[j];
+        // slots past the count are zero: every entry a kernel evaluates,
+        // used or not, then comes from this system's own finite data
+        for (int j = g; j < NB; j += G) {
+            const int64_t id = j < cnt ? nbl[j] : 0;
 #pragma unroll
-            for (int d = 0; d < D; d++) xy[j * D + d] = P.coords[id * D + d];
+            for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt ? P.coords[id * D + d] : 0.0;
         }
         double zr[R];
 #pragma unroll

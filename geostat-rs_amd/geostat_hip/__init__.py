@@ -862,25 +862,25 @@ class Server:
     def __del__(self):
         self.close()
 
-    def submit_krige(self, kriging: "Kriging", targets) -> Ticket:
+    def submit_krige(self, kriging: "Kriging", targets, precision="fp64") -> Ticket:
         t = _f64(targets).reshape(-1, kriging.dim)
         T = t.shape[0]
         v, var, lag = np.empty(T), np.empty(T), np.empty(T)
         st = np.empty(T, dtype=np.int32)
         tid = C.c_int64()
         _check(_abi.lib().gs_server_submit_krige(self._h, kriging._h, _abi.dptr(t), T,
-                                                 _abi.dptr(v), _abi.dptr(var), _abi.dptr(lag),
-                                                 _abi.i32ptr(st), C.byref(tid)))
+                                                 _PREC[precision], _abi.dptr(v), _abi.dptr(var),
+                                                 _abi.dptr(lag), _abi.i32ptr(st), C.byref(tid)))
         return Ticket(tid.value, "krige", (kriging, t), (v, var, lag, st))
 
-    def submit_vecchia(self, plan: "VecchiaPlan", values, models) -> Ticket:
+    def submit_vecchia(self, plan: "VecchiaPlan", values, models, precision="fp64") -> Ticket:
         v = _f64(values).reshape(-1)
         arr = (_abi.gs_model * len(models))(*[mm._c() for mm in models])
         out = np.empty(len(models))
         st = np.empty(len(models), dtype=np.int32)
         tid = C.c_int64()
         _check(_abi.lib().gs_server_submit_vecchia(self._h, plan._h, _abi.dptr(v), arr,
-                                                   len(models), _abi.dptr(out),
+                                                   len(models), _PREC[precision], _abi.dptr(out),
                                                    _abi.i32ptr(st), C.byref(tid)))
         return Ticket(tid.value, "vecchia", (plan, v, arr), (out, st))
 

@@ -102,10 +102,10 @@ SIGNATURES = {
                                 C.POINTER(gs_kconfig), _I64P, _DP, _DP]),
     "gs_server_create": (C.c_int32, [_P, C.c_int32, C.c_int64, C.POINTER(_P)]),
     "gs_server_destroy": (None, [_P]),
-    "gs_server_submit_krige": (C.c_int32, [_P, _P, _DP, C.c_int64, _DP, _DP, _DP, _I32P,
-                                           C.POINTER(C.c_int64)]),
-    "gs_server_submit_vecchia": (C.c_int32, [_P, _P, _DP, C.POINTER(gs_model), C.c_int32, _DP,
-                                             _I32P, C.POINTER(C.c_int64)]),
+    "gs_server_submit_krige": (C.c_int32, [_P, _P, _DP, C.c_int64, C.c_int32, _DP, _DP, _DP,
+                                           _I32P, C.POINTER(C.c_int64)]),
+    "gs_server_submit_vecchia": (C.c_int32, [_P, _P, _DP, C.POINTER(gs_model), C.c_int32,
+                                             C.c_int32, _DP, _I32P, C.POINTER(C.c_int64)]),
     "gs_server_wait": (C.c_int32, [_P, C.c_int64, _DP, _DP]),
     "gs_model_validate": (C.c_int32, [C.POINTER(gs_model)]),
     "gs_vecchia_mle": (C.c_int32, [_P, _DP, C.c_int32, C.c_double, C.POINTER(gs_model), _DP]),

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GS_ABI_VERSION 1
+#define GS_ABI_VERSION 2
 #define GS_MAX_STRUCTURES 8
 #define GS_NONE (-1)
 
@@ -309,14 +309,15 @@ int32_t gs_vecchia_mle(gs_vplan *p, const double *values, int32_t kind, double p
 typedef struct gs_server gs_server;
 int32_t gs_server_create(gs_ctx *ctx, int32_t n_slots, int64_t mem_budget, gs_server **out);
 void gs_server_destroy(gs_server *s);
-/* A predict_many request on a handle (not external drift), FP64. */
+/* A predict_many request on a handle (not external drift) at `precision`
+ * (gs_precision). */
 int32_t gs_server_submit_krige(gs_server *s, gs_kriging *k, const double *targets, int64_t T,
-                               double *value, double *variance, double *lagrange,
-                               int32_t *status, int64_t *ticket);
-/* A gs_vecchia_loglik_batch request on a plan, FP64. */
+                               int32_t precision, double *value, double *variance,
+                               double *lagrange, int32_t *status, int64_t *ticket);
+/* A gs_vecchia_loglik_batch request on a plan at `precision`. */
 int32_t gs_server_submit_vecchia(gs_server *s, gs_vplan *p, const double *values,
-                                 const gs_model *models, int32_t n_models, double *out,
-                                 int32_t *status, int64_t *ticket);
+                                 const gs_model *models, int32_t n_models, int32_t precision,
+                                 double *out, int32_t *status, int64_t *ticket);
 /* Waits for a ticket; returns the request's status; queue_ms / service_ms
  * (may be NULL) receive its time in the queue and on a slot. */
 int32_t gs_server_wait(gs_server *s, int64_t ticket, double *queue_ms, double *service_ms);

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_string():
     from geostat_hip import _abi
-    assert _abi.lib().gs_abi_version() == 1
+    assert _abi.lib().gs_abi_version() == 2
     assert isinstance(_abi.last_error(), str)
 
 

@@ -8,7 +8,9 @@ workload (the oracle's serial plan build is too slow at this size):
 * conditioning sets (point indices): min(k, m) predecessors, and, for a
   sample of positions, exactly the m nearest predecessors by brute force in
   ascending (d2, position) order;
-* the likelihood batch is deterministic across calls.
+* the likelihood batch is deterministic across calls;
+* the log-likelihoods of two of the 64 parameter vectors equal the oracle's
+  (the reference's serial loop, restated) on the same plan within 1e-10.
 """
 import numpy as np
 import pytest
@@ -52,3 +54,23 @@ def test_full_size_plan_properties():
     a, sa = p.loglik_batch(vals, models)
     b, sb = p.loglik_batch(vals, models)
     assert (sa == 0).all() and np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+def test_full_size_loglik_against_oracle(oracle):
+    import concurrent.futures as cf
+    c = workloads.vecchia_c3()
+    m = 30
+    p = gh.vecchia_plan(c, m)
+    o, nb, cnt = p.arrays()
+    vals = workloads.vecchia_field(c, o, nb, cnt, m)
+    models = workloads.vecchia_models(64)
+    picks = [0, 37]          # two of the 64 C3 parameter vectors
+    print(f"vecchia full size vs oracle: n={len(c)} m={m} models {picks}", flush=True)
+    ll, st = p.loglik_batch(vals, models)
+    assert (st == 0).all()
+    with cf.ThreadPoolExecutor(max_workers=len(picks)) as ex:
+        refs = list(ex.map(lambda i: oracle.vecchia_loglik_plan(c, vals, models[i], o, nb, cnt),
+                           picks))
+    for i, (rc, oll) in zip(picks, refs):
+        assert rc == 0
+        assert abs(ll[i] - oll) <= 1e-10 * abs(oll), (i, ll[i], oll)
