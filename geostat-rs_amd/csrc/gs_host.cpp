@@ -44,18 +44,35 @@ int32_t fail(int32_t code, const std::string &msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));            \
     } while (0)
 
-// Device buffer owned by a handle; grows, never shrinks.
+// A fixed device region handed out front to back in 256-byte units
+// (the no-allocation entry point's workspace).
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+// Device buffer owned by a handle; grows, never shrinks.  With an arena it
+// takes its bytes from the arena instead and never allocates or frees.
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    Arena *arena = nullptr;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() {
-        if (p) (void)hipFree(p);
+        if (p && !arena) (void)hipFree(p);
     }
     hipError_t ensure(size_t b) {
         if (b <= bytes && p) return hipSuccess;
+        if (arena) {
+            const size_t need = ((b ? b : 8) + 255) / 256 * 256;
+            if (need > arena->cap - arena->used) return hipErrorOutOfMemory;
+            p = arena->base + arena->used;
+            arena->used += need;
+            bytes = b;
+            return hipSuccess;
+        }
         if (p) {
             (void)hipFree(p);
             p = nullptr;
@@ -431,6 +448,12 @@ struct KWork {
     DBuf sel_tmp;       // selection workspace of the unflagged targets
     DBuf fb;            // [T + 1] FP32-refined fallback list and its length
     SideStream side;
+    void use_arena(Arena *a) {
+        for (DBuf *b : {&nbr, &count, &lists, &lens, &h_targets, &h_value, &h_var, &h_lag,
+                        &h_status, &keys, &keys_out, &vals, &perm, &sort_tmp, &flags, &h_tdrift,
+                        &blk, &g_w, &g_b0, &mark, &sel_tmp, &fb})
+            b->arena = a;
+    }
 };
 
 struct gs_kriging {
@@ -588,6 +611,37 @@ int32_t build_cell_grid(gs_kriging *K, const std::vector<double> &tc) {
 
 }  // namespace
 
+namespace {
+
+// ---- fixed scratch of the no-allocation entry point ------------------------
+// Every GPU the library runs on holds one copy of this array from module
+// load to process end; gs_ctx_create records its device address once per
+// device (the array's name is not a device address in host code).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch;   // per device
+
+// With `device` current.
+int32_t scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch.size() <= device) g_scratch.resize((size_t)device + 1, nullptr);
+    if (g_scratch[(size_t)device]) return GS_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(GS_ERR_DEVICE, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scratch[(size_t)device] = ptr;
+    return GS_OK;
+}
+
+void *scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < (int)g_scratch.size() ? g_scratch[(size_t)device] : nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *gs_last_error(void) { return g_err.c_str(); }
@@ -608,6 +662,12 @@ int32_t gs_ctx_create(int32_t device, gs_ctx **out) {
     if (e != hipSuccess) {
         delete c;
         return fail(GS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    int32_t sst = scratch_init(device);
+    if (sst != GS_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return sst;
     }
     gs::BesselNodes nodes;
     gauss_legendre(60, nodes.x60, nodes.w60);
@@ -1106,7 +1166,7 @@ int32_t run_global(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, 
 int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                     int32_t precision, double *d_value, double *d_var, double *d_lag,
                     int32_t *d_status, hipStream_t s, const Extra &x = Extra(),
-                    bool timed = true) {
+                    bool timed = true, bool allow_side = true) {
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
     if (T == 0) return GS_OK;
@@ -1120,7 +1180,7 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     // The kd-tree replay of flagged targets overlaps the solve of the
     // others.  Small calls keep it in line: the extra launches and events
     // would cost more than the overlap saves.
-    const bool side = K->has_tree && T >= 65536;
+    const bool side = allow_side && K->has_tree && T >= 65536;
     int32_t st = run_neighbors(K, W, d_targets, T, s, side);
     if (st != GS_OK) return st;
     if (timed && K->timing) HIP_TRY(hipEventRecord(ev[1], s));
@@ -1295,6 +1355,21 @@ int32_t solve_all(gs_kriging *K, KWork &W, gs::SolveParams p, int64_t T, hipStre
     return solve(pad_nb(K->nb_bound), p, false);
 }
 
+// Scratch bytes of one gs_kriging_predict_many_noalloc call: the workspace
+// buffers run_neighbors / run_solves request (no side stream), each rounded
+// to the arena's 256-byte units.
+size_t noalloc_bytes(const gs_kriging *K, int64_t T, bool r32) {
+    auto al = [](size_t b) { return ((b ? b : 8) + 255) / 256 * 256; };
+    const size_t t = (size_t)std::max<int64_t>(T, 1);
+    size_t n = al(4 * t * (size_t)stride_of(K)) + al(4 * t);      // lists, counts
+    if (K->has_tree)
+        n += 2 * al(8 * t) + 2 * al(4 * t) + al(gs::tile_sort_workspace(T)) + al(4 * (t + 4));
+    if (K->per_oct > 0 && K->nb_bound > 16) n += al(4 * 3 * t);  // size buckets
+    n += al(4 * 8);                                              // list lengths
+    if (r32) n += al(4 * (t + 1));                               // FP64 fallback list
+    return n;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1321,6 +1396,52 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
     }
     return run_predict(k, k->w, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status,
                        s);
+}
+
+// Same as gs_kriging_predict_many_device without any device allocation: the
+// workspace comes from the fixed per-device scratch, and the kd-tree replay
+// stays on the caller's stream.
+int32_t gs_kriging_predict_many_noalloc(gs_kriging *k, const double *d_targets, int64_t T,
+                                        int32_t precision, double *d_value, double *d_variance,
+                                        double *d_lagrange, int32_t *d_status, void *stream) {
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T < 0) return fail(GS_ERR_INVALID_PARAMETER, "T < 0");
+    if (T > 0 && (!d_targets || !d_value || !d_variance))
+        return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    if (k->global)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "unsupported without allocation: a global neighbourhood's scratch grows with "
+                    "the data; use gs_kriging_predict_many_device");
+    if (T == 0) return GS_OK;
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    void *base = scratch_of(k->ctx->device);
+    if (!base) return fail(GS_ERR_DEVICE, "scratch not set up for this device");
+    hipStream_t s = (hipStream_t)stream;
+    if (k->nv > 0) {   // external drift: every target fails (kriging.rs:494-501)
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        HIP_TRY(gs::launch_fill_f64(d_value, T, nan, s));
+        HIP_TRY(gs::launch_fill_f64(d_variance, T, nan, s));
+        if (d_lagrange) HIP_TRY(gs::launch_fill_f64(d_lagrange, T, nan, s));
+        if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INVALID_PARAMETER, s));
+        return GS_OK;
+    }
+    const bool r32 = precision == GS_PREC_FP32_REFINED && k->lean && k->M <= 1;
+    const size_t need = noalloc_bytes(k, T, r32);
+    if (need > kScratchBytes)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "too many targets for one call without allocation: " + std::to_string(T) +
+                        " targets need " + std::to_string(need) + " bytes of scratch, the fixed "
+                        "scratch holds " + std::to_string(kScratchBytes) + "; split the targets");
+    Arena a;
+    a.base = static_cast<char *>(base);
+    a.cap = kScratchBytes;
+    KWork W;
+    W.use_arena(&a);
+    return run_predict(k, W, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status, s,
+                       Extra(), false, false);
 }
 
 }  // extern "C"

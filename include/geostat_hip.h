@@ -186,6 +186,21 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
                                        int32_t precision, double *d_value, double *d_variance,
                                        double *d_lagrange, int32_t *d_status, void *stream);
 
+/* gs_kriging_predict_many_device without any device allocation (no
+ * reference counterpart: for callers that cannot allocate, such as code under
+ * HIP graph capture, tight serving loops or harnesses that forbid hipMalloc).
+ * The workspace comes from a fixed 64 MiB scratch per GPU, allocated when the
+ * library is loaded and held until the process ends; a call whose workspace
+ * would exceed it (about 300,000 targets at 32 neighbours) fails with
+ * GS_ERR_INVALID_PARAMETER before any work is enqueued, and so does a handle
+ * with a global neighbourhood.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU race (serialize them).  The kd-tree
+ * replay of flagged targets stays on `stream`.  Results are bit-identical to
+ * gs_kriging_predict_many_device's. */
+int32_t gs_kriging_predict_many_noalloc(gs_kriging *k, const double *d_targets, int64_t T,
+                                        int32_t precision, double *d_value, double *d_variance,
+                                        double *d_lagrange, int32_t *d_status, void *stream);
+
 /* Kriging::neighbors (kriging.rs:441-490) for T targets: idx is T*max_out
  * int32 (original data indices in the reference's order, -1 padded),
  * count has T entries (the neighbourhood size before any min_neighbors
