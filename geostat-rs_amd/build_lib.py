@@ -143,6 +143,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (CSRC / "gs_host.cpp", EXACT),
         (CSRC / "gs_mle.cpp", EXACT),
         (CSRC / "gs_global.hip", EXACT),
+        (CSRC / "gs_big.hip", EXACT),
         (CSRC / "gs_plan.hip", EXACT),
     ]
     units += [(p, []) for p in _gen_sources()]

@@ -447,11 +447,15 @@ struct KWork {
     DBuf mark;          // [T] bytes: 1 for targets flagged for the kd-tree replay
     DBuf sel_tmp;       // selection workspace of the unflagged targets
     DBuf fb;            // [T + 1] FP32-refined fallback list and its length
+    // neighbourhoods above 64 points: target list (+ length), lists and
+    // counts by list position, device-memory heaps, factorization scratch
+    DBuf big_list, big_nbr, big_cnt, big_heap_d, big_heap_ix, big_scratch;
     SideStream side;
     void use_arena(Arena *a) {
         for (DBuf *b : {&nbr, &count, &lists, &lens, &h_targets, &h_value, &h_var, &h_lag,
                         &h_status, &keys, &keys_out, &vals, &perm, &sort_tmp, &flags, &h_tdrift,
-                        &blk, &g_w, &g_b0, &mark, &sel_tmp, &fb})
+                        &blk, &g_w, &g_b0, &mark, &sel_tmp, &fb, &big_list, &big_nbr, &big_cnt,
+                        &big_heap_d, &big_heap_ix, &big_scratch})
             b->arena = a;
     }
 };
@@ -468,6 +472,11 @@ struct gs_kriging {
     int64_t k_heap = 0;
     int per_oct = 0;
     int64_t nb_bound = 0;   // padded system bound (<= 64)
+    // Neighbourhoods above 64 points (gs_big.hip): big_req is the list
+    // bound of the configuration (0: never above 64); big_only: octant
+    // quotas above 64 slots, every target goes through the large path.
+    int64_t big_req = 0;
+    bool big_only = false;
     int64_t stride = 0;     // neighbour-list capacity per target
     int32_t min_nb = 1;
     int depth = 0;
@@ -719,6 +728,9 @@ int32_t check_pointset(gs_ctx *ctx, int32_t dim, const double *coords, const dou
 // Largest global neighbourhood: the shared system is (n + M)^2 doubles
 // (2 GiB at the limit) and its factorization n + M dependent steps.
 constexpr int64_t kGlobalMax = 16384;
+// Largest moving neighbourhood solved on the GPU (gs_big.hip); a target
+// with more neighbours gets status GS_ERR_INVALID_PARAMETER.
+constexpr int64_t kBigCap = 4096;
 
 gs::GlobalParams global_params(gs_kriging *K) {
     gs::GlobalParams g;
@@ -834,11 +846,12 @@ int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const doub
     }
     K->has_tree = cfg.max_neighbors != GS_NONE || cfg.has_search_radius || cfg.max_per_octant != GS_NONE;
     // Neighbour-list capacity (stride) and padded system bound (nb_bound).
-    // Systems hold at most 64 neighbours on the GPU.  When the configuration
-    // does not bound the list by 64 (radius-only search, max_neighbors above
-    // 64), the search keeps 65 slots: a target that fills them exceeds the
-    // GPU bound and gets a NaN estimate with status GS_ERR_INVALID_PARAMETER,
-    // while every other target is exact (its list never saturated).
+    // The register-resident solves hold at most 64 neighbours.  When the
+    // configuration does not bound the list by 64 (radius-only search,
+    // max_neighbors above 64), the fast search keeps 65 slots: a target that
+    // fills them goes through the large path (run_big), every other target
+    // is exact (its list never saturated).  Octant quotas above 64 slots
+    // send every target through the large path.
     if (!K->has_tree) {
         if (n > kGlobalMax)
             return fail(GS_ERR_INVALID_PARAMETER,
@@ -846,15 +859,22 @@ int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const doub
         K->global = n > 64;
         K->stride = K->global ? 1 : n;
     } else if (cfg.max_per_octant != GS_NONE) {
-        if (cfg.max_per_octant > 64 || (cfg.max_per_octant << dim) > 64)
-            return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: max_per_octant * 2^dim > 64");
-        int64_t cap = cfg.max_per_octant << dim;
-        K->per_oct = (int)cfg.max_per_octant;
+        const int64_t q = std::min<int64_t>(cfg.max_per_octant, n);
+        const int64_t cap = q << dim;
+        K->per_oct = (int)q;
         K->stride = cfg.max_neighbors != GS_NONE ? std::min(cap, cfg.max_neighbors) : cap;
+        if (cap > 64) {
+            if (K->stride > kBigCap)
+                return fail(GS_ERR_INVALID_PARAMETER,
+                            "unsupported on GPU: octant neighbourhoods above 4096 points");
+            K->big_only = true;
+            K->big_req = K->stride;
+        }
     } else {
         int64_t req = cfg.max_neighbors != GS_NONE ? cfg.max_neighbors : n;
         K->k_heap = std::min<int64_t>(req, 65);
         K->stride = std::min(K->k_heap, n);
+        if (std::min(req, n) > 64) K->big_req = std::min(req, n);
     }
     K->nb_bound = std::min<int64_t>(K->stride, 64);
     K->r2 = cfg.has_search_radius ? cfg.search_radius * cfg.search_radius
@@ -986,7 +1006,10 @@ void gs_kriging_free(gs_kriging *k) {
 }
 
 int64_t gs_kriging_max_neighbors(const gs_kriging *k) {
-    return k ? (k->global ? k->n : k->stride) : -1;
+    if (!k) return -1;
+    if (k->global) return k->n;
+    if (k->big_req > 0) return std::min<int64_t>(k->big_req, kBigCap);
+    return k->stride;
 }
 
 int32_t gs_kriging_set_timing(gs_kriging *k, int32_t enable) {
@@ -1163,6 +1186,9 @@ int32_t run_global(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, 
     return GS_OK;
 }
 
+int32_t run_big(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, double *d_value,
+                double *d_var, double *d_lag, int32_t *d_status, hipStream_t s, const Extra &x);
+
 int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
                     int32_t precision, double *d_value, double *d_var, double *d_lag,
                     int32_t *d_status, hipStream_t s, const Extra &x = Extra(),
@@ -1181,10 +1207,15 @@ int32_t run_predict(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     // others.  Small calls keep it in line: the extra launches and events
     // would cost more than the overlap saves.
     const bool side = allow_side && K->has_tree && T >= 65536;
-    int32_t st = run_neighbors(K, W, d_targets, T, s, side);
+    int32_t st = GS_OK;
+    if (!K->big_only) st = run_neighbors(K, W, d_targets, T, s, side);
     if (st != GS_OK) return st;
     if (timed && K->timing) HIP_TRY(hipEventRecord(ev[1], s));
-    st = run_solves(K, W, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x, side);
+    if (!K->big_only)
+        st = run_solves(K, W, d_targets, T, precision, d_value, d_var, d_lag, d_status, s, x, side);
+    if (st != GS_OK) return st;
+    // neighbourhoods above 64 points, in FP64 whatever the precision
+    if (K->big_req > 0) st = run_big(K, W, d_targets, T, d_value, d_var, d_lag, d_status, s, x);
     if (st != GS_OK) return st;
     if (timed && K->timing) {
         HIP_TRY(hipEventRecord(ev[2], s));
@@ -1208,9 +1239,11 @@ constexpr double kFallbackRatioKrige = 6e-4;
 // error near 1e-6, averaged over the points).
 constexpr double kFallbackRatioVecchia = 6e-5;
 
-int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
-                   int32_t precision, double *d_value, double *d_var, double *d_lag,
-                   int32_t *d_status, hipStream_t s, const Extra &x, bool side) {
+// Kernel parameters of a predict call (data, model, method, outputs and the
+// extended inputs); neighbour lists from the workspace.
+gs::SolveParams solve_params(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
+                             double *d_value, double *d_var, double *d_lag, int32_t *d_status,
+                             const Extra &x) {
     gs::SolveParams p;
     std::memset(&p, 0, sizeof(p));
     p.coords = K->coords.as<double>();
@@ -1241,6 +1274,13 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     p.offsets = x.offsets;
     p.n_off = x.n_off;
     p.cbb = x.cbb;
+    return p;
+}
+
+int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
+                   int32_t precision, double *d_value, double *d_var, double *d_lag,
+                   int32_t *d_status, hipStream_t s, const Extra &x, bool side) {
+    gs::SolveParams p = solve_params(K, W, d_targets, T, d_value, d_var, d_lag, d_status, x);
     const bool ext = p.err || p.xdrift || p.offsets;
     // FP32-refined solves cover lean models with simple or ordinary
     // kriging; every other configuration is solved in FP64, which meets the
@@ -1355,6 +1395,149 @@ int32_t solve_all(gs_kriging *K, KWork &W, gs::SolveParams p, int64_t T, hipStre
     return solve(pad_nb(K->nb_bound), p, false);
 }
 
+// ---- neighbourhoods above 64 points ----------------------------------------
+// The targets that need the large path: every target of a big_only handle,
+// otherwise those whose fast search filled its 65 slots.  *F on the host.
+int32_t big_targets(gs_kriging *K, KWork &W, int64_t T, hipStream_t s, int64_t *F) {
+    HIP_TRY(W.big_list.ensure(sizeof(int32_t) * (T + 1)));
+    int32_t *list = W.big_list.as<int32_t>();
+    int32_t *len = list + T;
+    HIP_TRY(gs::launch_zero_i32(len, 1, s));
+    if (K->big_only) {
+        HIP_TRY(W.count.ensure(sizeof(int32_t) * std::max<int64_t>(T, 1)));
+        HIP_TRY(gs::launch_fill_i32(W.count.as<int32_t>(), T, 65, s));
+    }
+    HIP_TRY(gs::launch_collect_over(W.count.as<int32_t>(), T, 64, list, len, s));
+    int32_t f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, len, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *F = f;
+    return GS_OK;
+}
+
+// Heap capacity of the large search: the list bound, or for a radius
+// search whose bound exceeds the cap, the largest in-radius count among
+// the listed targets (capped at kBigCap + 1: a target with more points
+// than the cap is refused).
+int32_t big_capacity(gs_kriging *K, KWork &W, const double *d_targets, int64_t F, hipStream_t s,
+                     int64_t *kcap) {
+    if (K->big_only) {
+        *kcap = K->stride;
+        return GS_OK;
+    }
+    if (K->big_req <= kBigCap || !std::isfinite(K->r2)) {
+        *kcap = std::min<int64_t>(K->big_req, kBigCap + 1);
+        return GS_OK;
+    }
+    gs::KnnParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.list = W.big_list.as<int32_t>();
+    p.tcoords = K->tcoords.as<double>();
+    p.tidx = K->tidx.as<uint32_t>();
+    p.n = K->n;
+    p.targets = d_targets;
+    p.T = F;
+    p.r2 = K->r2;
+    p.xf = K->xf;
+    HIP_TRY(W.big_cnt.ensure(sizeof(int32_t) * (F + 1)));
+    int32_t *cnt = W.big_cnt.as<int32_t>();
+    HIP_TRY(gs::launch_radius_count(K->D, p, cnt, s));
+    HIP_TRY(gs::launch_max_i32(cnt, F, cnt + F, s));
+    int32_t mx = 0;
+    HIP_TRY(hipMemcpyAsync(&mx, cnt + F, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *kcap = std::max<int64_t>(1, std::min<int64_t>(mx, kBigCap + 1));
+    return GS_OK;
+}
+
+struct BigChunking {
+    int64_t stride, slots, chunk, heap_blocks;
+};
+
+BigChunking big_chunking(const gs_kriging *K, int64_t F, int64_t kcap) {
+    BigChunking c;
+    c.stride = K->big_only ? K->stride : kcap;
+    c.slots = K->per_oct > 0 ? ((int64_t)K->per_oct << K->D) : kcap;
+    const int64_t budget = (int64_t)256 << 20;
+    c.chunk = std::min<int64_t>(F, std::max<int64_t>(64, budget / (4 * c.stride)));
+    c.heap_blocks = std::min<int64_t>((c.chunk + 63) / 64,
+                                      std::max<int64_t>(1, budget / (c.slots * 64 * 12)));
+    return c;
+}
+
+// The exact kd-tree replay of list[0..Fc) into W.big_nbr / W.big_cnt (by
+// list position), heaps in device memory.
+int32_t big_search(gs_kriging *K, KWork &W, const double *d_targets, const int32_t *list,
+                   int64_t Fc, int64_t kcap, const BigChunking &c, hipStream_t s) {
+    HIP_TRY(W.big_nbr.ensure(sizeof(int32_t) * c.chunk * c.stride));
+    HIP_TRY(W.big_cnt.ensure(sizeof(int32_t) * (c.chunk + 1)));
+    HIP_TRY(W.big_heap_d.ensure(sizeof(double) * c.heap_blocks * c.slots * 64));
+    HIP_TRY(W.big_heap_ix.ensure(sizeof(uint32_t) * c.heap_blocks * c.slots * 64));
+    gs::KnnParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.list = list;
+    p.tcoords = K->tcoords.as<double>();
+    p.tidx = K->tidx.as<uint32_t>();
+    p.n = K->n;
+    p.targets = d_targets;
+    p.T = Fc;
+    p.k = (int32_t)kcap;
+    p.per_octant = K->per_oct;
+    p.max_nb = (int32_t)K->stride;
+    p.depth = K->depth;
+    p.r2 = K->r2;
+    p.xf = K->xf;
+    p.out_idx = W.big_nbr.as<int32_t>();
+    p.out_count = W.big_cnt.as<int32_t>();
+    p.stride = (int32_t)c.stride;
+    p.gheap_d = W.big_heap_d.as<double>();
+    p.gheap_ix = W.big_heap_ix.as<uint32_t>();
+    p.heap_blocks = c.heap_blocks;
+    p.out_by_pos = 1;
+    HIP_TRY(gs::launch_knn(K->D, p, s));
+    return GS_OK;
+}
+
+// Targets whose neighbourhood exceeds 64 points: the exact kd-tree replay
+// with device-memory heaps, then one workgroup per target solving the
+// reference's bordered system by LU (gs_big.hip), in chunks that keep the
+// lists, heaps and factorization scratch within bounded device memory.
+int32_t run_big(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, double *d_value,
+                double *d_var, double *d_lag, int32_t *d_status, hipStream_t s, const Extra &x) {
+    int64_t F = 0;
+    int32_t st = big_targets(K, W, T, s, &F);
+    if (st != GS_OK || F == 0) return st;
+    int64_t kcap = 0;
+    st = big_capacity(K, W, d_targets, F, s, &kcap);
+    if (st != GS_OK) return st;
+    const BigChunking c = big_chunking(K, F, kcap);
+    const int64_t nmax = std::min<int64_t>(c.stride, kBigCap) + K->M;
+    gs::BigParams bp;
+    std::memset(&bp, 0, sizeof(bp));
+    bp.S = solve_params(K, W, d_targets, T, d_value, d_var, d_lag, d_status, x);
+    bp.stride = (int32_t)c.stride;
+    bp.M = K->M;
+    bp.nv = K->nv;
+    bp.cap = (int32_t)kBigCap;
+    bp.slot = nmax * nmax + 2 * nmax;
+    const int64_t sblocks = std::max<int64_t>(1, ((int64_t)512 << 20) / (8 * bp.slot));
+    for (int64_t c0 = 0; c0 < F; c0 += c.chunk) {
+        const int64_t Fc = std::min<int64_t>(c.chunk, F - c0);
+        const int32_t *list = W.big_list.as<int32_t>() + c0;
+        st = big_search(K, W, d_targets, list, Fc, kcap, c, s);
+        if (st != GS_OK) return st;
+        const int64_t blocks = std::min<int64_t>(Fc, sblocks);
+        HIP_TRY(W.big_scratch.ensure(sizeof(double) * blocks * bp.slot));
+        bp.list = list;
+        bp.F = Fc;
+        bp.nbr = W.big_nbr.as<int32_t>();
+        bp.count = W.big_cnt.as<int32_t>();
+        bp.scratch = W.big_scratch.as<double>();
+        HIP_TRY(gs::launch_big_solve(K->D, K->lean, bp, blocks, s));
+    }
+    return GS_OK;
+}
+
 // Scratch bytes of one gs_kriging_predict_many_noalloc call: the workspace
 // buffers run_neighbors / run_solves request (no side stream), each rounded
 // to the arena's 256-byte units.
@@ -1410,10 +1593,11 @@ int32_t gs_kriging_predict_many_noalloc(gs_kriging *k, const double *d_targets, 
         return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
     if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
         return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
-    if (k->global)
+    if (k->global || k->big_req > 0)
         return fail(GS_ERR_INVALID_PARAMETER,
-                    "unsupported without allocation: a global neighbourhood's scratch grows with "
-                    "the data; use gs_kriging_predict_many_device");
+                    "unsupported without allocation: global neighbourhoods and neighbourhoods "
+                    "above 64 points need scratch that grows with the data; use "
+                    "gs_kriging_predict_many_device");
     if (T == 0) return GS_OK;
     std::lock_guard<std::mutex> lk(k->mu);
     HIP_TRY(hipSetDevice(k->ctx->device));
@@ -1585,16 +1769,47 @@ int32_t gs_kriging_neighbors(gs_kriging *k, const double *targets, int64_t T, in
     hipStream_t s = k->ctx->stream;
     HIP_TRY(k->w.h_targets.ensure(sizeof(double) * T * k->D));
     HIP_TRY(hipMemcpyAsync(k->w.h_targets.p, targets, sizeof(double) * T * k->D, hipMemcpyHostToDevice, s));
-    int32_t st = run_neighbors(k, k->w, k->w.h_targets.as<double>(), T, s);
+    const double *d_targets = k->w.h_targets.as<double>();
+    if (!k->big_only) {
+        int32_t st = run_neighbors(k, k->w, d_targets, T, s);
+        if (st != GS_OK) return st;
+        const int32_t stride = stride_of(k);
+        std::vector<int32_t> tmp((size_t)(T * stride));
+        HIP_TRY(hipMemcpyAsync(tmp.data(), k->w.nbr.p, sizeof(int32_t) * T * stride, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(count, k->w.count.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int64_t t = 0; t < T; t++)
+            for (int64_t j = 0; j < max_out; j++)
+                idx[t * max_out + j] = j < stride ? tmp[(size_t)(t * stride + j)] : -1;
+    }
+    if (k->big_req == 0) return GS_OK;
+    // lists above 64 points: the large search for the targets that need it
+    int64_t F = 0;
+    int32_t st = big_targets(k, k->w, T, s, &F);
+    if (st != GS_OK || F == 0) return st;
+    int64_t kcap = 0;
+    st = big_capacity(k, k->w, d_targets, F, s, &kcap);
     if (st != GS_OK) return st;
-    const int32_t stride = stride_of(k);
-    std::vector<int32_t> tmp((size_t)(T * stride));
-    HIP_TRY(hipMemcpyAsync(tmp.data(), k->w.nbr.p, sizeof(int32_t) * T * stride, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(count, k->w.count.p, sizeof(int32_t) * T, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    for (int64_t t = 0; t < T; t++)
-        for (int64_t j = 0; j < max_out; j++)
-            idx[t * max_out + j] = j < stride ? tmp[(size_t)(t * stride + j)] : -1;
+    const BigChunking c = big_chunking(k, F, kcap);
+    std::vector<int32_t> list((size_t)F), lists, cnts;
+    HIP_TRY(hipMemcpyAsync(list.data(), k->w.big_list.p, sizeof(int32_t) * F, hipMemcpyDeviceToHost, s));
+    for (int64_t c0 = 0; c0 < F; c0 += c.chunk) {
+        const int64_t Fc = std::min<int64_t>(c.chunk, F - c0);
+        st = big_search(k, k->w, d_targets, k->w.big_list.as<int32_t>() + c0, Fc, kcap, c, s);
+        if (st != GS_OK) return st;
+        lists.resize((size_t)(Fc * c.stride));
+        cnts.resize((size_t)Fc);
+        HIP_TRY(hipMemcpyAsync(lists.data(), k->w.big_nbr.p, sizeof(int32_t) * Fc * c.stride,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(cnts.data(), k->w.big_cnt.p, sizeof(int32_t) * Fc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int64_t q = 0; q < Fc; q++) {
+            const int64_t t = list[(size_t)(c0 + q)];
+            count[t] = cnts[(size_t)q];
+            for (int64_t j = 0; j < max_out; j++)
+                idx[t * max_out + j] = j < c.stride ? lists[(size_t)(q * c.stride + j)] : -1;
+        }
+    }
     return GS_OK;
 }
 

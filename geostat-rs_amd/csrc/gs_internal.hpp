@@ -77,6 +77,13 @@ struct KnnParams {
     int32_t *out_idx;        // [T][stride]
     int32_t *out_count;      // [T]
     int32_t stride;
+    // Large neighbourhoods: heaps in device memory ([block][slot][lane]
+    // instead of LDS; the grid must not exceed heap_blocks), and outputs
+    // indexed by list position instead of target id.
+    double *gheap_d;
+    uint32_t *gheap_ix;
+    int64_t heap_blocks;
+    int32_t out_by_pos;
 };
 
 struct SolveParams {
@@ -188,6 +195,33 @@ struct VecchiaParams {
 };
 
 hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s);
+// In-radius point count of each listed target (kd-tree traversal, no heap),
+// written by list position.
+hipError_t launch_radius_count(int D, const KnnParams &p, int32_t *counts, hipStream_t s);
+
+// Moving neighbourhoods above 64 points (gs_big.hip): one workgroup per
+// listed target builds the bordered system from its neighbour list and
+// solves it by LU with partial pivoting in the reference's order.
+struct BigParams {
+    SolveParams S;           // data, model, method, extended inputs, outputs (by target id)
+    const int32_t *list;     // [F] target ids
+    int64_t F;
+    const int32_t *nbr;      // [F][stride] neighbour lists by list position
+    const int32_t *count;    // [F]
+    int32_t stride;
+    int32_t M;               // drift terms
+    int32_t nv;              // external covariates
+    int32_t cap;             // largest system the scratch holds (neighbours)
+    double *scratch;         // [blocks][slot]
+    int64_t slot;            // doubles per block: (cap+M)^2 + 2 (cap+M)
+};
+hipError_t launch_big_solve(int D, bool lean, const BigParams &p, int64_t blocks, hipStream_t s);
+// Targets whose neighbour count exceeds `limit`, appended to list (length
+// in *len).
+hipError_t launch_collect_over(const int32_t *count, int64_t T, int32_t limit, int32_t *list,
+                               int32_t *len, hipStream_t s);
+// *out = max of counts[0..n) (counts >= 0).
+hipError_t launch_max_i32(const int32_t *counts, int64_t n, int32_t *out, hipStream_t s);
 size_t tile_sort_workspace(int64_t T);
 hipError_t launch_tile_order(int D, const double *targets, int64_t T, const SearchXf &xf,
                              const CellGridView &G, uint64_t *keys, uint64_t *keys_out,

@@ -122,11 +122,16 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(KnnParams P) {
     const int lane = threadIdx.x;
     const int slots = OCT ? (P.per_octant << D) : P.k;
     LHeap H;
-    H.d = reinterpret_cast<double *>(smem_raw);
-    H.ix = reinterpret_cast<uint32_t *>(H.d + (size_t)slots * kBlock);
+    if (P.gheap_d) {   // large neighbourhoods: this block's heaps in device memory
+        H.d = P.gheap_d + (size_t)blockIdx.x * slots * kBlock;
+        H.ix = P.gheap_ix + (size_t)blockIdx.x * slots * kBlock;
+    } else {
+        H.d = reinterpret_cast<double *>(smem_raw);
+        H.ix = reinterpret_cast<uint32_t *>(H.d + (size_t)slots * kBlock);
+    }
     H.lane = lane;
 
-    const int64_t nt = P.list ? (int64_t)(*P.list_len) : P.T;
+    const int64_t nt = P.list && P.list_len ? (int64_t)(*P.list_len) : P.T;
     for (int64_t it = (int64_t)blockIdx.x * kBlock + lane; it < nt;
          it += (int64_t)gridDim.x * kBlock) {
     const int64_t t = P.list ? (int64_t)P.list[it] : it;
@@ -222,12 +227,13 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(KnnParams P) {
         if (!found) break;
     }
 
-    int32_t *out = P.out_idx + t * P.stride;
+    const int64_t oi = P.out_by_pos ? it : t;
+    int32_t *out = P.out_idx + oi * P.stride;
     if (!OCT) {
         H.sort(len);
         for (int e = 0; e < len; e++) out[e] = (int32_t)H.IX(e);
         for (int e = len; e < P.stride; e++) out[e] = -1;
-        P.out_count[t] = len;
+        P.out_count[oi] = len;
         continue;
     }
     // Octant mode: sort each octant, then merge by (d2, idx) and truncate
@@ -262,17 +268,80 @@ __global__ __launch_bounds__(kBlock) void knn_kernel(KnnParams P) {
         out[e] = (int32_t)bi;
     }
     for (int e = outn; e < P.stride; e++) out[e] = -1;
-    P.out_count[t] = outn;
+    P.out_count[oi] = outn;
     }
+}
+
+// Number of data points within the search radius of each listed target:
+// the kd-tree walk of knn_kernel without a heap (every subtree whose split
+// plane is within the radius is visited).
+template <int D>
+__global__ __launch_bounds__(kBlock) void radius_count_kernel(KnnParams P, int32_t *counts) {
+    const int64_t nt = P.list && P.list_len ? (int64_t)(*P.list_len) : P.T;
+    for (int64_t it = (int64_t)blockIdx.x * kBlock + threadIdx.x; it < nt;
+         it += (int64_t)gridDim.x * kBlock) {
+        const int64_t t = P.list ? (int64_t)P.list[it] : it;
+        double tg[D], st[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) tg[d] = P.targets[t * D + d];
+        search_xf<D>(P.xf, tg, st);
+        const double r2 = P.r2;
+        const double *tc = P.tcoords;
+        StackEnt stk[kStack];
+        uint32_t stk_axis[kStack];
+        int sp = 0;
+        uint32_t lo = 0, hi = (uint32_t)P.n;
+        int axis = 0;
+        int32_t cnt = 0;
+        for (;;) {
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                double pc[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) pc[d] = tc[(size_t)mid * D + d];
+                if (dist2<D>(pc, st) <= r2) cnt++;
+                const double diff = st[axis] - pc[axis];
+                uint32_t nlo, nhi, flo, fhi;
+                if (diff <= 0.0) { nlo = lo; nhi = mid; flo = mid + 1; fhi = hi; }
+                else { nlo = mid + 1; nhi = hi; flo = lo; fhi = mid; }
+                if (flo < fhi && diff * diff <= r2 && sp < kStack) {
+                    stk[sp].lo = flo;
+                    stk[sp].hi = fhi;
+                    stk_axis[sp] = (uint32_t)axis;
+                    sp++;
+                }
+                lo = nlo;
+                hi = nhi;
+                axis = (axis + 1 == D) ? 0 : axis + 1;
+            }
+            if (sp == 0) break;
+            sp--;
+            lo = stk[sp].lo;
+            hi = stk[sp].hi;
+            axis = ((int)stk_axis[sp] + 1 == D) ? 0 : (int)stk_axis[sp] + 1;
+        }
+        counts[it] = cnt;
+    }
+}
+
+hipError_t launch_radius_count(int D, const KnnParams &p, int32_t *counts, hipStream_t s) {
+    if (p.T == 0) return hipSuccess;
+    int64_t blocks = (p.T + kBlock - 1) / kBlock;
+    if (p.list) blocks = std::min<int64_t>(blocks, 2048);
+    if (D == 2) hipLaunchKernelGGL((radius_count_kernel<2>), dim3((unsigned)blocks), dim3(kBlock), 0, s, p, counts);
+    else if (D == 3) hipLaunchKernelGGL((radius_count_kernel<3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, p, counts);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_knn(int D, const KnnParams &p, hipStream_t s) {
     if (p.T == 0) return hipSuccess;
     const bool oct = p.per_octant > 0;
     const int slots = oct ? (p.per_octant << D) : p.k;
-    const size_t lds = (size_t)slots * kBlock * (sizeof(double) + sizeof(uint32_t));
+    const size_t lds = p.gheap_d ? 0 : (size_t)slots * kBlock * (sizeof(double) + sizeof(uint32_t));
     int64_t blocks = (p.T + kBlock - 1) / kBlock;
     if (p.list) blocks = std::min<int64_t>(blocks, 2048);   // grid-stride over the list
+    if (p.gheap_d) blocks = std::min<int64_t>(blocks, p.heap_blocks);
     const dim3 grid((unsigned)blocks);
     if (D == 2) {
         if (oct) hipLaunchKernelGGL((knn_kernel<2, true>), grid, dim3(kBlock), lds, s, p);

@@ -231,30 +231,33 @@ def test_min_neighbors_error_and_build_errors():
         gh.Kriging(np.zeros((0, 2)), np.zeros(0), m)
     with pytest.raises(gh.InvalidParameter):   # Power: semivariogram form not on GPU
         gh.Kriging(c, v, gh.VariogramModel(0.0, [gh.Structure(gh.ModelKind.Power(1.2), 2.0, 1.0)]))
-    # a neighbourhood above the GPU's 64-point systems is a per-target
-    # InvalidParameter (NaN estimate), never a silently truncated system
-    big = gh.Kriging(*workloads.points(1, 200, [10.0, 10.0]), m, gh.KrigingConfig(max_neighbors=100))
+    # a neighbourhood above 64 points goes through the large path: the
+    # reference's estimate, not a truncated system
+    c1, v1 = workloads.points(1, 200, [10.0, 10.0])
+    big = gh.Kriging(c1, v1, m, gh.KrigingConfig(max_neighbors=100))
     val, _, _, st = big.predict_many([[5.0, 5.0], [1.0, 2.0]])
-    assert (st == 4).all() and np.isnan(val).all()
-    with pytest.raises(gh.InvalidParameter):
-        big.predict([5.0, 5.0])
+    rc, ov, _, _, ost, _ = oracle_mod().predict_many(c1, v1, m, gh.KrigingConfig(max_neighbors=100),
+                                                     [[5.0, 5.0], [1.0, 2.0]])
+    assert (st == 0).all() and (ost == 0).all() and rel_inf(val, ov) < TOL64
 
 
-def test_radius_neighbourhood_overflow_is_flagged(oracle):
-    # radius-only search keeps every in-radius point (kriging.rs:445-451);
-    # targets with more than 64 get status 4, all others match the oracle
+def oracle_mod():
+    from oracle import refcpu
+    return refcpu
+
+
+def test_radius_neighbourhood_above_64(oracle):
+    # radius-only search keeps every in-radius point (kriging.rs:445-451):
+    # targets with more than 64 take the large path, every target matches
+    # the oracle
     c, v = workloads.points(13, 20000, [1000.0, 1000.0])
     t = workloads.points(14, 3000, [1000.0, 1000.0])[0]
     m = cs.MODELS_2D["exp"]
     cfg = gh.KrigingConfig(search_radius=30.0)
-    k = gh.Kriging(c, v, m, cfg)
-    val, var, _, st = k.predict_many(t)
-    rc, ov, ovar, _, ost, _ = oracle.predict_many(c, v, m, cfg, t)
-    _, _, ocnt = oracle.neighbors(c, v, m, cfg, t, 200)
-    over = ocnt > 64
-    assert over.any() and (~over).any()
-    assert (st[over] == 4).all() and np.array_equal(st[~over], ost[~over])
-    assert rel_inf(val[~over], ov[~over]) < TOL64 and rel_inf(var[~over], ovar[~over], 1e-3) < TOL64
+    _, _, ocnt = oracle.neighbors(c, v, m, cfg, t, 400)
+    assert (ocnt > 64).any() and (ocnt <= 64).any()
+    check_neighbors(oracle, c, v, m, cfg, t)
+    check_predict(oracle, c, v, m, cfg, t)
 
 
 def test_singular_targets_are_nan_with_status(oracle):
