@@ -133,3 +133,20 @@ def test_device_path_rejects_bad_buffers_before_any_call():
             k.predict_many_device(t, v, var, lag, st)
     with pytest.raises(gh.InvalidParameter):
         k.predict_many_device(good_t, *outs, precision="fp16")
+
+
+def test_rust_binding_declares_every_entry_point():
+    # The geostat-hip crate's ffi.rs declares exactly the functions of the
+    # header, and its #[repr(C)] structs list the header's fields in order.
+    rs = (ROOT / "geostat-rs_amd" / "rust" / "geostat-hip" / "src" / "ffi.rs").read_text()
+    fns = sorted(set(re.findall(r"pub fn (gs_[a-z0-9_]+)\(", rs)))
+    assert fns == _declared_functions()
+    hdr = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    for c_name, rs_name in (("gs_anisotropy", "GsAnisotropy"), ("gs_structure", "GsStructure"),
+                            ("gs_model", "GsModel"), ("gs_kconfig", "GsKConfig")):
+        body = re.search(r"typedef struct \{([^{}]*)\} " + c_name + ";", hdr).group(1)
+        c_fields = re.findall(r"(\w+)(?:\[\w+\])?\s*[;,]", body)
+        rs_body = re.search(r"pub struct " + rs_name + r" \{(.*?)\}", rs, re.S).group(1)
+        rs_fields = re.findall(r"pub (\w+):", rs_body)
+        assert c_fields == rs_fields, (c_name, c_fields, rs_fields)
+    assert "GS_ABI_VERSION: i32 = 2" in rs
