@@ -102,6 +102,14 @@ def cpu_baseline(w, targets, pass_s: float = 3.0):
                       f"(median in median_value), kd-tree build excluded ({build_s:.2f} s)"}
 
 
+def traffic_of(workload: str, prec: str):
+    """HBM bytes per launch of the dominant kernel, from the rocprof
+    FETCH_SIZE / WRITE_SIZE summary committed under profiles/ (None when
+    not measured)."""
+    tf = ROOT / "profiles" / f"traffic_{workload}_{prec}.json"
+    return json.loads(tf.read_text()).get("bytes_per_launch") if tf.exists() else None
+
+
 def vecchia_flops_per_point(s: int, c: int) -> float:
     """E*c + s^3/3 + 2 s^2 + 4 s + 6 with E = s(s+1)/2 (s conditioning points)."""
     return s * (s + 1) / 2 * c + s ** 3 / 3.0 + 2 * s * s + 4 * s + 6
@@ -165,12 +173,19 @@ def bench_vecchia(args) -> dict:
     order, nb, cnt = plan.arrays()
     vals = workloads.vecchia_field(coords, order, nb, cnt, m, threads=cpu_threads())
     models = workloads.vecchia_models(64)
-    for _ in range(args.warmup):
-        plan.loglik_batch(vals, models)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ll, st = plan.loglik_batch(vals, models)
-    dt = (time.perf_counter() - t0) / args.steps
+    precs = ["fp64", "fp32r"] if args.precision == "both" else [args.precision]
+    timed, lls = {}, {}
+    for prec in precs:
+        for _ in range(args.warmup):
+            plan.loglik_batch(vals, models, precision=prec)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ll, st = plan.loglik_batch(vals, models, precision=prec)
+        timed[prec] = (time.perf_counter() - t0) / args.steps
+        lls[prec] = (ll, st)
+    best = min(precs, key=lambda q: timed[q])
+    dt = timed[best]
+    ll, st = lls[best]
     grad_ms = None
     if args.grad:
         gmodels = workloads.vecchia_models(64)
@@ -192,14 +207,22 @@ def bench_vecchia(args) -> dict:
         "metric": "Vecchia conditionals/s (points x parameter vectors)",
         "value": n * 64 / dt, "unit": "conditionals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if best == "fp64" else "f32+f64",
         "data": "synthetic (seeded, sequential Vecchia draw over the plan)",
         "config": {"workload": "vecchia_c3", "points": int(n), "m": m, "models": 64,
+                   "precision": best,
+                   **{f"{q}_ms_per_batch": timed[q] * 1e3 for q in precs},
+                   **({"fp32r_vs_fp64": timed["fp64"] / timed["fp32r"],
+                       "fp32r_rel_err_vs_fp64": float(
+                           np.abs(lls["fp32r"][0] - lls["fp64"][0]).max()
+                           / np.abs(lls["fp64"][0]).max())} if len(precs) == 2 else {}),
                    "plan_build_s": plan_s, "ok_models": int((st == 0).sum()),
                    "grad_ms_per_batch": grad_ms, "mle_fit": mle},
         "roofline": {"bound": "valu", "achieved": flops / dt / 1e12, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / dt / 1e12 / FP64_PEAK_TFLOPS,
-                     "traffic": None, "kernel": "vecchia_kernel", "flops_per_launch": flops},
+                     "traffic": traffic_of("vecchia_c3", best),
+                     "kernel": "vecchia_kernel" if best == "fp64" else "vecchia_fp32r_kernel",
+                     "flops_per_launch": flops},
         "cpu_baseline": (None if args.no_cpu_baseline else
                          cpu_baseline_vecchia(coords, vals, order, nb, cnt, models, m)),
     }

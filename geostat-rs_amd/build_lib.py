@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -125,11 +126,39 @@ def _compile(src: Path, extra: list[str], incremental: bool = False) -> Path:
     obj = OBJ / (src.stem + ".o")
     if incremental and obj.exists() and obj.stat().st_mtime > max(_deps_mtime(src), src.stat().st_mtime):
         return obj
-    cmd = [HIPCC, *COMMON, *extra, "-c", str(src), "-o", str(obj)]
+    remarks = src.suffix == ".hip"
+    cmd = [HIPCC, *COMMON, *extra, *(["-Rpass-analysis=kernel-resource-usage"] if remarks else []),
+           "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if remarks:   # the compiler's register / spill / occupancy report per kernel
+        (OBJ / (src.stem + ".rem")).write_text(r.stderr)
     return obj
+
+
+_REMARK = re.compile(r"remark:\s+(Function Name|VGPRs|AGPRs|TotalSGPRs|ScratchSize \[bytes/lane\]|"
+                     r"Occupancy \[waves/SIMD\]|SGPRs Spill|VGPRs Spill|LDS Size \[bytes/block\]):"
+                     r"\s+(\S+)")
+
+
+def resource_report() -> list[dict]:
+    """Per kernel of the last build: VGPRs, AGPRs, SGPRs, spills, scratch
+    bytes per lane, LDS bytes and occupancy, from the compiler's
+    kernel-resource-usage remarks (kernels only, not device functions)."""
+    rows, cur = [], None
+    for rem in sorted(OBJ.glob("*.rem")):
+        for line in rem.read_text().splitlines():
+            m = _REMARK.search(line)
+            if not m:
+                continue
+            key, val = m.groups()
+            if key == "Function Name":
+                cur = {"unit": rem.stem, "kernel": val}
+                rows.append(cur)
+            elif cur is not None:
+                cur[key.split(" [")[0]] = int(val) if val.lstrip("-").isdigit() else val
+    return [r for r in rows if "Occupancy" in r]
 
 
 def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> Path:
@@ -177,7 +206,20 @@ def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
 
 
+def print_report(only_spills: bool = True) -> None:
+    cols = ("VGPRs", "AGPRs", "TotalSGPRs", "SGPRs Spill", "VGPRs Spill", "ScratchSize",
+            "LDS Size", "Occupancy")
+    print("unit kernel " + " ".join(c.replace(" ", "_") for c in cols))
+    for r in resource_report():
+        if only_spills and not (r.get("SGPRs Spill") or r.get("VGPRs Spill") or r.get("ScratchSize")):
+            continue
+        print(r["unit"], r["kernel"][:60], *(r.get(c, "-") for c in cols))
+
+
 if __name__ == "__main__":
+    if "--report" in sys.argv:       # resource usage of the last build's kernels
+        print_report(only_spills="--all" not in sys.argv)
+        sys.exit(0)
     if "--clean" in sys.argv:
         clean()
     print(build(incremental="--incremental" in sys.argv))

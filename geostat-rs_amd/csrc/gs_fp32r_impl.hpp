@@ -419,7 +419,7 @@ __device__ __forceinline__ Refined fp32r_system(const DevModel &Mdl, const int c
         _Pragma("unroll") for (int r = 0; r < R; r++) {                                        \
             double dh[D];                                                                      \
             _Pragma("unroll") for (int d = 0; d < D; d++) dh[d] = rowx[r][d] - tg[d];          \
-            tb[r] += lean_term<D, AN, SP>(L, dh);                                              \
+            tb[r] += lean_term<D, AN, SP, false>(L, dh);                                              \
         }                                                                                      \
         static_for<0, (R + 1) / 2>([&](auto PP) {                                              \
             constexpr int rA = decltype(PP)::value;                                            \
@@ -434,7 +434,7 @@ __device__ __forceinline__ Refined fp32r_system(const DevModel &Mdl, const int c
                 double dh[D];                                                                  \
                 _Pragma("unroll") for (int d = 0; d < D; d++)                                 \
                     dh[d] = (inA ? rowx[rA][d] : rowx[rB][d]) - xy[c * D + d];                 \
-                double A = lean_term<D, AN, SP>(L, dh);                                        \
+                double A = lean_term<D, AN, SP, false>(L, dh);                                        \
                 A = (inA ? liveA : liveB) ? A : 0.0;                                           \
                 const float4 sc = sol[c];                                                      \
                 const double wi = inA ? w[rA] : w[rB];                                         \

@@ -17,7 +17,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -443,7 +445,7 @@ struct SideStream {
 
 struct KWork {
     DBuf nbr, count, lists, lens, h_targets, h_value, h_var, h_lag, h_status;
-    DBuf keys, keys_out, vals, perm, sort_tmp, flags, h_tdrift, blk, g_w, g_b0;
+    DBuf keys, perm, sort_tmp, flags, h_tdrift, blk, g_w, g_b0;
     DBuf mark;          // [T] bytes: 1 for targets flagged for the kd-tree replay
     DBuf sel_tmp;       // selection workspace of the unflagged targets
     DBuf fb;            // [T + 1] FP32-refined fallback list and its length
@@ -453,7 +455,7 @@ struct KWork {
     SideStream side;
     void use_arena(Arena *a) {
         for (DBuf *b : {&nbr, &count, &lists, &lens, &h_targets, &h_value, &h_var, &h_lag,
-                        &h_status, &keys, &keys_out, &vals, &perm, &sort_tmp, &flags, &h_tdrift,
+                        &h_status, &keys, &perm, &sort_tmp, &flags, &h_tdrift,
                         &blk, &g_w, &g_b0, &mark, &sel_tmp, &fb, &big_list, &big_nbr, &big_cnt,
                         &big_heap_d, &big_heap_ix, &big_scratch})
             b->arena = a;
@@ -495,7 +497,7 @@ struct gs_kriging {
     double xmean[3] = {0, 0, 0}, xscale[3] = {1, 1, 1};
     DBuf err, xdrift;
     // cell grid of the search-frame coordinates (tile search)
-    DBuf cg_start, cg_pts, cg_idx;
+    DBuf cg_start, cg_pts, cg_idx, cg_rank;   // + Hilbert rank of every cell
     gs::CellGridView cg{};
     // per-call workspace of the calls made through the handle itself
     KWork w;
@@ -615,6 +617,8 @@ int32_t build_cell_grid(gs_kriging *K, const std::vector<double> &tc) {
     G.start = K->cg_start.as<int32_t>();
     G.pts = K->cg_pts.as<double>();
     G.idx = K->cg_idx.as<uint32_t>();
+    HIP_TRY(K->cg_rank.ensure(sizeof(uint32_t) * ncells));
+    HIP_TRY(gs::build_cell_ranks(D, G, K->cg_rank.as<uint32_t>(), K->ctx->stream));
     return GS_OK;
 }
 
@@ -1046,6 +1050,9 @@ int32_t gs_kriging_timing_summary(gs_kriging *k, int64_t *calls, double *ms_sear
 namespace {
 
 int32_t stride_of(const gs_kriging *K) { return (int32_t)std::max<int64_t>(K->stride, 1); }
+int64_t ncells_of(const gs_kriging *K) {
+    return (int64_t)K->cg.nc[0] * K->cg.nc[1] * K->cg.nc[2];
+}
 
 // Neighbour lists for T device targets into the handle's nbr/count buffers.
 // side: the kd-tree replay of the flagged targets goes onto the workspace's
@@ -1062,16 +1069,14 @@ int32_t run_neighbors(gs_kriging *K, KWork &W, const double *d_targets, int64_t 
         return GS_OK;
     }
     // 1) Hilbert order of the targets' search-frame cells
-    const size_t tmp_bytes = gs::tile_sort_workspace(T);
-    HIP_TRY(W.keys.ensure(sizeof(uint64_t) * T));
-    HIP_TRY(W.keys_out.ensure(sizeof(uint64_t) * T));
-    HIP_TRY(W.vals.ensure(sizeof(int32_t) * T));
+    const size_t tmp_bytes = gs::tile_sort_workspace(T, ncells_of(K));
+    HIP_TRY(W.keys.ensure(sizeof(uint32_t) * T));
     HIP_TRY(W.perm.ensure(sizeof(int32_t) * T));
     HIP_TRY(W.sort_tmp.ensure(tmp_bytes));
     HIP_TRY(W.flags.ensure(sizeof(int32_t) * (T + 4)));
-    HIP_TRY(gs::launch_tile_order(K->D, d_targets, T, K->xf, K->cg, W.keys.as<uint64_t>(),
-                                  W.keys_out.as<uint64_t>(), W.vals.as<int32_t>(),
-                                  W.perm.as<int32_t>(), W.sort_tmp.p, tmp_bytes, s));
+    HIP_TRY(gs::launch_tile_order(K->D, d_targets, T, K->xf, K->cg, K->cg_rank.as<uint32_t>(),
+                                  W.keys.as<uint32_t>(), W.perm.as<int32_t>(), W.sort_tmp.p,
+                                  tmp_bytes, s));
     // 2) tile search: canonical lists; boundary ties and scattered
     //    wavefronts flagged
     int32_t *flag_len = W.flags.as<int32_t>() + T;
@@ -1546,11 +1551,29 @@ size_t noalloc_bytes(const gs_kriging *K, int64_t T, bool r32) {
     const size_t t = (size_t)std::max<int64_t>(T, 1);
     size_t n = al(4 * t * (size_t)stride_of(K)) + al(4 * t);      // lists, counts
     if (K->has_tree)
-        n += 2 * al(8 * t) + 2 * al(4 * t) + al(gs::tile_sort_workspace(T)) + al(4 * (t + 4));
+        n += 2 * al(4 * t) + al(gs::tile_sort_workspace(T, ncells_of(K))) + al(4 * (t + 4));
     if (K->per_oct > 0 && K->nb_bound > 16) n += al(4 * 3 * t);  // size buckets
     n += al(4 * 8);                                              // list lengths
     if (r32) n += al(4 * (t + 1));                               // FP64 fallback list
     return n;
+}
+
+// The device path of one predict call on workspace W: an external-drift
+// predictor fails every target (predict without covariates,
+// kriging.rs:494-501, mapped to NaN by predict_many); otherwise run_predict.
+int32_t predict_device(gs_kriging *k, KWork &W, const double *d_targets, int64_t T,
+                       int32_t precision, double *d_value, double *d_variance, double *d_lagrange,
+                       int32_t *d_status, hipStream_t s, bool timed, bool allow_side) {
+    if (k->nv > 0) {
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        HIP_TRY(gs::launch_fill_f64(d_value, T, nan, s));
+        HIP_TRY(gs::launch_fill_f64(d_variance, T, nan, s));
+        if (d_lagrange) HIP_TRY(gs::launch_fill_f64(d_lagrange, T, nan, s));
+        if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INVALID_PARAMETER, s));
+        return GS_OK;
+    }
+    return run_predict(k, W, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status, s,
+                       Extra(), timed, allow_side);
 }
 
 }  // namespace
@@ -1566,19 +1589,8 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
         return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
     std::lock_guard<std::mutex> lk(k->mu);
     HIP_TRY(hipSetDevice(k->ctx->device));
-    hipStream_t s = (hipStream_t)stream;
-    if (k->nv > 0) {
-        // predict on an external-drift predictor fails per target
-        // (kriging.rs:494-501) and predict_many maps that to NaN
-        const double nan = std::numeric_limits<double>::quiet_NaN();
-        HIP_TRY(gs::launch_fill_f64(d_value, T, nan, s));
-        HIP_TRY(gs::launch_fill_f64(d_variance, T, nan, s));
-        if (d_lagrange) HIP_TRY(gs::launch_fill_f64(d_lagrange, T, nan, s));
-        if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INVALID_PARAMETER, s));
-        return GS_OK;
-    }
-    return run_predict(k, k->w, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status,
-                       s);
+    return predict_device(k, k->w, d_targets, T, precision, d_value, d_variance, d_lagrange,
+                          d_status, (hipStream_t)stream, true, true);
 }
 
 // Same as gs_kriging_predict_many_device without any device allocation: the
@@ -1603,15 +1615,6 @@ int32_t gs_kriging_predict_many_noalloc(gs_kriging *k, const double *d_targets, 
     HIP_TRY(hipSetDevice(k->ctx->device));
     void *base = scratch_of(k->ctx->device);
     if (!base) return fail(GS_ERR_DEVICE, "scratch not set up for this device");
-    hipStream_t s = (hipStream_t)stream;
-    if (k->nv > 0) {   // external drift: every target fails (kriging.rs:494-501)
-        const double nan = std::numeric_limits<double>::quiet_NaN();
-        HIP_TRY(gs::launch_fill_f64(d_value, T, nan, s));
-        HIP_TRY(gs::launch_fill_f64(d_variance, T, nan, s));
-        if (d_lagrange) HIP_TRY(gs::launch_fill_f64(d_lagrange, T, nan, s));
-        if (d_status) HIP_TRY(gs::launch_fill_i32(d_status, T, GS_ERR_INVALID_PARAMETER, s));
-        return GS_OK;
-    }
     const bool r32 = precision == GS_PREC_FP32_REFINED && k->lean && k->M <= 1;
     const size_t need = noalloc_bytes(k, T, r32);
     if (need > kScratchBytes)
@@ -1624,8 +1627,92 @@ int32_t gs_kriging_predict_many_noalloc(gs_kriging *k, const double *d_targets, 
     a.cap = kScratchBytes;
     KWork W;
     W.use_arena(&a);
-    return run_predict(k, W, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status, s,
-                       Extra(), false, false);
+    return predict_device(k, W, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status,
+                          (hipStream_t)stream, false, false);
+}
+
+}  // extern "C"
+
+// A predict call captured as a HIP graph over fixed buffers, with a
+// workspace of its own (so graph launches and the predictor's other calls
+// do not share scratch).  The predictor's resident data is read, never
+// written, by the graph.
+struct gs_graph {
+    int device = 0;
+    KWork w;
+    hipGraphExec_t exec = nullptr;
+    ~gs_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+extern "C" {
+
+// Capture: one eager pass on a private stream sizes every workspace buffer,
+// then the same launches are recorded by stream capture, which therefore
+// allocates nothing.  Every launch of the path is a kernel (counters are
+// cleared by fill kernels, the target order is a counting sort of kernels),
+// and the path reads nothing back to the host, so the graph holds kernel
+// nodes only.  The kd-tree replay of flagged targets stays on the captured
+// stream.
+int32_t gs_kriging_graph_create(gs_kriging *k, const double *d_targets, int64_t T,
+                                int32_t precision, double *d_value, double *d_variance,
+                                double *d_lagrange, int32_t *d_status, gs_graph **out) {
+    if (!out) return fail(GS_ERR_INVALID_PARAMETER, "out is NULL");
+    *out = nullptr;
+    if (!k) return fail(GS_ERR_INVALID_PARAMETER, "handle is NULL");
+    if (T <= 0) return fail(GS_ERR_INVALID_PARAMETER, "a graph needs T >= 1 targets");
+    if (!d_targets || !d_value || !d_variance) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    if (precision != GS_PREC_FP64 && precision != GS_PREC_FP32_REFINED)
+        return fail(GS_ERR_INVALID_PARAMETER, "unknown precision");
+    if (k->big_req > 0)
+        return fail(GS_ERR_INVALID_PARAMETER,
+                    "unsupported in a graph: neighbourhoods above 64 points size their work from "
+                    "list lengths read back to the host; use gs_kriging_predict_many_device");
+    std::lock_guard<std::mutex> lk(k->mu);
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    std::unique_ptr<gs_graph> g(new (std::nothrow) gs_graph());
+    if (!g) return fail(GS_ERR_OUT_OF_MEMORY, "graph handle");
+    g->device = k->ctx->device;
+    struct OwnStream {
+        hipStream_t s = nullptr;
+        ~OwnStream() {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } os;
+    HIP_TRY(hipStreamCreateWithFlags(&os.s, hipStreamNonBlocking));
+    int32_t st = predict_device(k, g->w, d_targets, T, precision, d_value, d_variance, d_lagrange,
+                                d_status, os.s, false, false);
+    if (st != GS_OK) return st;
+    HIP_TRY(hipStreamSynchronize(os.s));
+    HIP_TRY(hipStreamBeginCapture(os.s, hipStreamCaptureModeThreadLocal));
+    st = predict_device(k, g->w, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status,
+                        os.s, false, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(os.s, &graph);   // ends capture on every path
+    if (st != GS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    HIP_TRY(ec);
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(ei);
+    *out = g.release();
+    return GS_OK;
+}
+
+int32_t gs_graph_launch(gs_graph *g, void *stream) {
+    if (!g) return fail(GS_ERR_INVALID_PARAMETER, "graph is NULL");
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return GS_OK;
+}
+
+void gs_graph_free(gs_graph *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    delete g;
 }
 
 }  // extern "C"
@@ -2868,7 +2955,7 @@ int64_t krige_bytes(const gs_kriging *k, int64_t T) {
     }
     const int64_t stride = std::max<int64_t>(k->stride, 1);
     return T * (k->D * 8 + 36 + stride * 4 + 4 + 16 + 8 + 4 + 16 + (k->per_oct > 0 ? 12 : 0)) +
-           (int64_t)gs::tile_sort_workspace(T);
+           (int64_t)gs::tile_sort_workspace(T, ncells_of(k));
 }
 
 int64_t vecchia_bytes(const gs_vplan *p, int32_t n_models) {

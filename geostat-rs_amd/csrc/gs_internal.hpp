@@ -222,11 +222,14 @@ hipError_t launch_collect_over(const int32_t *count, int64_t T, int32_t limit, i
                                int32_t *len, hipStream_t s);
 // *out = max of counts[0..n) (counts >= 0).
 hipError_t launch_max_i32(const int32_t *counts, int64_t n, int32_t *out, hipStream_t s);
-size_t tile_sort_workspace(int64_t T);
+// Hilbert rank of every cell of the grid, [ncells] (build time, synchronous).
+hipError_t build_cell_ranks(int D, const CellGridView &G, uint32_t *rank, hipStream_t s);
+// Target processing order: perm = targets sorted by the Hilbert rank of
+// their cell; bins is [T] scratch, tmp holds tile_sort_workspace bytes.
+size_t tile_sort_workspace(int64_t T, int64_t ncells);
 hipError_t launch_tile_order(int D, const double *targets, int64_t T, const SearchXf &xf,
-                             const CellGridView &G, uint64_t *keys, uint64_t *keys_out,
-                             int32_t *vals, int32_t *perm, void *tmp, size_t tmp_bytes,
-                             hipStream_t s);
+                             const CellGridView &G, const uint32_t *cell_rank, uint32_t *bins,
+                             int32_t *perm, void *tmp, size_t tmp_bytes, hipStream_t s);
 hipError_t launch_tile_knn(int D, const TileParams &p, hipStream_t s);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t s);
 // Returns hipErrorInvalidValue when (D, NB, M) has no instantiation.

@@ -135,6 +135,56 @@ __device__ __forceinline__ void basis(const SolveParams &P, const double *x, dou
 
 constexpr int kWave = 64;
 
+// exp(-s) for s >= 0 from a 64-entry table: s = k ln2/64 + r with
+// |r| <= ln2/128 (Cody-Waite, the fdlibm split of ln2 scaled by 1/64, so
+// k * hi is exact), k = 64 m + j, exp(-s) = 2^-m * 2^(-j/64) * e^-r, and
+// e^-r - 1 by its degree-5 Taylor polynomial (truncation below 4e-17
+// relative on that interval), added to the table entry with one fma.  About
+// 1.3 ulp at most, 6 fewer operations than the degree-12 form of exp_neg.
+// The table holds 2^(-j/64) rounded to nearest; each kernel that evaluates
+// lean covariances copies it to LDS first (load_exp_table).
+static __constant__ double kExp2Neg64[64] = {
+    0x1.0000000000000p+0, 0x1.fa7c1819e90d8p-1, 0x1.f50765b6e4540p-1, 0x1.efa1bee615a27p-1,
+    0x1.ea4afa2a490dap-1, 0x1.e502ee78b3ff6p-1, 0x1.dfc97337b9b5fp-1, 0x1.da9e603db3285p-1,
+    0x1.d5818dcfba487p-1, 0x1.d072d4a07897cp-1, 0x1.cb720dcef9069p-1, 0x1.c67f12e57d14bp-1,
+    0x1.c199bdd85529cp-1, 0x1.bcc1e904bc1d2p-1, 0x1.b7f76f2fb5e47p-1, 0x1.b33a2b84f15fbp-1,
+    0x1.ae89f995ad3adp-1, 0x1.a9e6b5579fdbfp-1, 0x1.a5503b23e255dp-1, 0x1.a0c667b5de565p-1,
+    0x1.9c49182a3f090p-1, 0x1.97d829fde4e50p-1, 0x1.93737b0cdc5e5p-1, 0x1.8f1ae99157736p-1,
+    0x1.8ace5422aa0dbp-1, 0x1.868d99b4492edp-1, 0x1.82589994cce13p-1, 0x1.7e2f336cf4e62p-1,
+    0x1.7a11473eb0187p-1, 0x1.75feb564267c9p-1, 0x1.71f75e8ec5f74p-1, 0x1.6dfb23c651a2fp-1,
+    0x1.6a09e667f3bcdp-1, 0x1.6623882552225p-1, 0x1.6247eb03a5585p-1, 0x1.5e76f15ad2148p-1,
+    0x1.5ab07dd485429p-1, 0x1.56f4736b527dap-1, 0x1.5342b569d4f82p-1, 0x1.4f9b2769d2ca7p-1,
+    0x1.4bfdad5362a27p-1, 0x1.486a2b5c13cd0p-1, 0x1.44e086061892dp-1, 0x1.4160a21f72e2ap-1,
+    0x1.3dea64c123422p-1, 0x1.3a7db34e59ff7p-1, 0x1.371a7373aa9cbp-1, 0x1.33c08b26416ffp-1,
+    0x1.306fe0a31b715p-1, 0x1.2d285a6e4030bp-1, 0x1.29e9df51fdee1p-1, 0x1.26b4565e27cddp-1,
+    0x1.2387a6e756238p-1, 0x1.2063b88628cd6p-1, 0x1.1d4873168b9aap-1, 0x1.1a35beb6fcb75p-1,
+    0x1.172b83c7d517bp-1, 0x1.1429aaea92de0p-1, 0x1.11301d0125b51p-1, 0x1.0e3ec32d3d1a2p-1,
+    0x1.0b5586cf9890fp-1, 0x1.0874518759bc8p-1, 0x1.059b0d3158574p-1, 0x1.02c9a3e778061p-1,
+};
+static __shared__ double g_exp_tab[64];
+
+__device__ __forceinline__ void load_exp_table() {
+    if (threadIdx.x < 64) g_exp_tab[threadIdx.x] = kExp2Neg64[threadIdx.x];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double exp_neg_t(double s) {
+    s = fmin(s, 800.0);
+    const double kf = rint(s * 0x1.71547652b82fep+6);      // 64 / ln2
+    double r = fma(-kf, 0x1.62e42fee00000p-7, s);           // ln2/64, high part
+    r = fma(-kf, 0x1.a39ef35793c76p-39, r);                 // ln2/64, low part
+    const double x = -r;
+    double p = 1.0 / 120.0;
+    p = fma(p, x, 1.0 / 24.0);
+    p = fma(p, x, 1.0 / 6.0);
+    p = fma(p, x, 0.5);
+    p = fma(p, x, 1.0);
+    const double q = p * x;                                 // e^x - 1
+    const int k = (int)kf;
+    const double t = g_exp_tab[k & 63];
+    return ldexp(fma(t, q, t), -(k >> 6));
+}
+
 // 1/sqrt(d): hardware estimate plus one Newton step (relative error ~1e-16
 // for normal d > 0; d <= 0 gives NaN/inf and is caught by the pivot test).
 __device__ __forceinline__ double rsqrt_nr(double d) {
@@ -212,7 +262,9 @@ __device__ __forceinline__ double sqrt_lean(double q) {
 // structures this is c0 - gamma(dh) for dh != 0, because c0 = nugget + sum
 // sill (model.rs:636-660); the exponential family gives sill p(q) e^-q'
 // directly, without the cancellation of 1 - g at long range.
-template <int D, bool ANISO, bool SPH>
+// TAB: exp by the LDS table (the kernel ran load_exp_table); otherwise by
+// exp_neg, for kernels that keep their LDS and registers for other work.
+template <int D, bool ANISO, bool SPH, bool TAB = true>
 __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *dh) {
     double h2s = 0.0;
     if constexpr (ANISO) {
@@ -235,7 +287,8 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
     } else {
         const double q = L.alpha * d;
         const double p = fma(fma(L.gam, q, L.beta), q, 1.0);
-        c = p * exp_neg(L.gau ? h2s : q);
+        const double x = L.gau ? h2s : q;
+        c = p * (TAB ? exp_neg_t(x) : exp_neg(x));
     }
     return L.sill * c;
 }
@@ -250,6 +303,9 @@ __device__ __forceinline__ void lean_slots(const LeanRegs<D> &L, const int e0, c
                                            const int iA, const double (&xA)[D],
                                            const double (&xB)[D], const double *xy,
                                            double *stage, const int lane) {
+    // one entry per iteration: unrolled, the compiler interleaves the
+    // entries' exp chains and runs out of registers
+#pragma unroll 1
     for (int e = e0; e < e1; e++) {
         const bool inA = e < iA;
         const int c = inA ? e : NB - 2 - e;
@@ -595,7 +651,27 @@ __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&
             y[1][r] -= tz * y[2 + j][r];
         }
     });
-    {
+    // Partial pivoting keeps every |u_kk| <= 2^M max|S| (the growth bound),
+    // and prod |u_kk| = det S = prod r_jj^2, so every pivot is at least
+    // det S / (2^M max|S|)^(M-1); max|S| is the largest diagonal entry (S is
+    // positive definite).  Only a system below that bound (near-singular
+    // drift) needs the exact pivots, so the elimination below is skipped
+    // for all others.
+    double det = 1.0, smax = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        det *= rr[j][j] * rr[j][j];
+        double dj = 0.0;
+#pragma unroll
+        for (int k = 0; k <= j; k++) dj += rr[k][j] * rr[k][j];
+        smax = fmax(smax, dj);
+    }
+    double growth = 1.0;
+#pragma unroll
+    for (int j = 1; j < M; j++) growth *= (double)(1 << M) * smax;
+    if (M == 1) {
+        ok = ok && !(det < thresh);
+    } else if (!(det >= 2.0 * thresh * growth)) {
         double S[M][M];
 #pragma unroll
         for (int i = 0; i < M; i++)
@@ -695,6 +771,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
     const DevModel &Mdl = *P.model;
     const double c0 = Mdl.c0;
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
+    if constexpr (LEAN) load_exp_table();
 
     for (int64_t sys = (int64_t)blockIdx.x * GPW + grp; sys < nsys;
          sys += (int64_t)gridDim.x * GPW) {
@@ -765,14 +842,18 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
         // Scale of the reference's singular test (linalg.rs:36-39): the
         // largest |entry| of the bordered matrix, max(c0 + err_i, |F_iq|)
         // over the live rows (covariances never exceed c0 in magnitude).
-        // Simple and ordinary kriging: c0, or max(c0, 1) for the constant
-        // drift column, uniform over the wavefront.
-        double amax = M == 1 ? fmax(c0, 1.0) : c0;
-        if constexpr (M > 1 || X) {
+        // Without the extended systems that is c0 for simple kriging and
+        // max(c0, 1) with a drift: the constant column is 1 and the
+        // polynomial basis of a datum stays within [-1, 1], because the
+        // basis is centred and scaled on the data's bounding box
+        // (kriging.rs:340-347).  Uniform over the wavefront, so it stays in
+        // scalar registers through the factorization.
+        double amax = M >= 1 ? fmax(c0, 1.0) : c0;
+        if constexpr (X) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 if (row_of<G>(r, g) < cnt) {
-                    if constexpr (X) amax = fmax(amax, c0 + ea[r]);
+                    amax = fmax(amax, c0 + ea[r]);
 #pragma unroll
                     for (int q = 1; q < M; q++) amax = fmax(amax, fabs(y[2 + q][r]));
                 }
@@ -859,6 +940,7 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
     double *colbuf = s_col + grp * ColStride<NB, 2, G>::value;
     const int nm = P.n_models;
     for (int p = g; p < nm; p += G) s_sum[grp * kMaxModelsPerLaunch + p] = 0.0;
+    if constexpr (LEAN) load_exp_table();
     wave_sync();
     const double ln2pi = 1.8378770664093453;  // ln(2 pi)
 
@@ -985,7 +1067,7 @@ __device__ __forceinline__ void grad_pair(const GradRegs<D> &R, const double *dh
         rho = in ? d * (1.5 - 1.5 * d * d) : 0.0;
     } else {
         const double q = R.L.alpha * d;
-        const double e = exp_neg(R.L.gau ? h2s : q);
+        const double e = exp_neg_t(R.L.gau ? h2s : q);
         c = fma(fma(R.L.gam, q, R.L.beta), q, 1.0) * e;
         rho = d * fma(fma(R.r3, d, R.r2), d, R.r1) * e;
     }
@@ -999,6 +1081,7 @@ __device__ __forceinline__ void grad_slots(const GradRegs<D> &R, const int e0, c
                                            const double (&xA)[D], const double (&xB)[D],
                                            const double *xy, double *stage, double *pcol,
                                            const int lane) {
+#pragma unroll 1
     for (int e = e0; e < e1; e++) {
         const bool inA = e < iA;
         const int c = inA ? e : NB - 2 - e;
@@ -1036,6 +1119,7 @@ __global__ __launch_bounds__(kWave) void vecchia_grad_kernel(VecchiaParams P) {
     double *sums = s_sum + grp * kMaxModelsPerLaunch * kQ;
     const int nm = P.n_models;
     for (int q = g; q < nm * kQ; q += G) sums[q] = 0.0;
+    load_exp_table();
     wave_sync();
     const double ln2pi = 1.8378770664093453;   // ln(2 pi)
 

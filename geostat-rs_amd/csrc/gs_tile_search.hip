@@ -33,6 +33,7 @@
 // ring a lane stops when the distance from its target to the unvisited part
 // of the grid (outside the visited rectangle) exceeds its k-th distance or
 // the search radius; the wavefront stops when all lanes have.
+
 #include <hipcub/hipcub.hpp>
 
 #include "gs_internal.hpp"
@@ -114,9 +115,34 @@ __device__ __forceinline__ uint64_t hilbert_index(const int *c, int bits) {
     return key;
 }
 
+// Hilbert key of every grid cell (cell id = x + nc0 * (y + nc1 * z)), with
+// the cell id as the sort payload; `shift` coarsens the cells when the
+// interleaved index would not fit in 64 bits.
 template <int D>
-__global__ void hilbert_kernel(const double *targets, int64_t T, SearchXf xf, CellGridView G,
-                               int bits, int shift, uint64_t *keys, int32_t *vals) {
+__global__ void cell_key_kernel(CellGridView G, int64_t ncells, int bits, int shift,
+                                uint64_t *keys, uint32_t *ids) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncells) return;
+    int c[D];
+    int64_t r = id;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        c[d] = (int)(r % G.nc[d]) >> shift;
+        r /= G.nc[d];
+    }
+    keys[id] = hilbert_index<D>(c, bits);
+    ids[id] = (uint32_t)id;
+}
+
+__global__ void rank_scatter_kernel(const uint32_t *sorted_ids, int64_t ncells, uint32_t *rank) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncells) rank[sorted_ids[i]] = (uint32_t)i;
+}
+
+// Per target: the Hilbert rank of its search-frame cell, >> bshift.
+template <int D>
+__global__ void target_bin_kernel(const double *targets, int64_t T, SearchXf xf, CellGridView G,
+                                  const uint32_t *rank, int bshift, uint32_t *bins) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     double p[D], st[D];
@@ -125,10 +151,13 @@ __global__ void hilbert_kernel(const double *targets, int64_t T, SearchXf xf, Ce
     xf_target<D>(xf, p, st);
     int c[D];
     cell_of<D>(G, st, c);
+    int64_t id = 0, mul = 1;
 #pragma unroll
-    for (int d = 0; d < D; d++) c[d] >>= shift;
-    keys[t] = hilbert_index<D>(c, bits);
-    vals[t] = (int32_t)t;
+    for (int d = 0; d < D; d++) {
+        id += (int64_t)c[d] * mul;
+        mul *= G.nc[d];
+    }
+    bins[t] = rank[id] >> bshift;
 }
 
 // Bounded max-heap over (d2, idx) in LDS, lane-interleaved (entry e of
@@ -677,20 +706,12 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
 
 }  // namespace
 
-size_t tile_sort_workspace(int64_t T) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t *)nullptr,
-                                             (uint64_t *)nullptr, (const int32_t *)nullptr,
-                                             (int32_t *)nullptr, (int)T);
-    return bytes;
-}
-
-hipError_t launch_tile_order(int D, const double *targets, int64_t T, const SearchXf &xf,
-                             const CellGridView &G, uint64_t *keys, uint64_t *keys_out,
-                             int32_t *vals, int32_t *perm, void *tmp, size_t tmp_bytes,
-                             hipStream_t s) {
-    if (T == 0) return hipSuccess;
+// Hilbert rank of every cell of the grid (build time; allocates its own
+// temporaries).  Consecutive ranks are adjacent cells, so targets taken in
+// rank order share cells and neighbourhoods.
+hipError_t build_cell_ranks(int D, const CellGridView &G, uint32_t *rank, hipStream_t s) {
     if (D != 2 && D != 3) return hipErrorInvalidValue;
+    const int64_t ncells = (int64_t)G.nc[0] * G.nc[1] * G.nc[2];
     // bits per axis covering the largest cell index; coarser cells when the
     // interleaved index would not fit in 64 bits
     int need = 1;
@@ -698,17 +719,201 @@ hipError_t launch_tile_order(int D, const double *targets, int64_t T, const Sear
         while (need < 31 && (int64_t)(G.nc[d] - 1) >> need) need++;
     const int bits = std::min(need, 64 / D);
     const int shift = need - bits;
+    uint64_t *keys = nullptr, *keys_out = nullptr;
+    uint32_t *ids = nullptr, *ids_out = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    hipError_t e = hipMalloc(&keys, sizeof(uint64_t) * ncells);
+    if (e == hipSuccess) e = hipMalloc(&keys_out, sizeof(uint64_t) * ncells);
+    if (e == hipSuccess) e = hipMalloc(&ids, sizeof(uint32_t) * ncells);
+    if (e == hipSuccess) e = hipMalloc(&ids_out, sizeof(uint32_t) * ncells);
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_out, ids, ids_out,
+                                               (int)ncells, 0, D * bits, s);
+    if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 8);
+    const dim3 grid((unsigned)((ncells + 255) / 256));
+    if (e == hipSuccess) {
+        if (D == 2)
+            hipLaunchKernelGGL(cell_key_kernel<2>, grid, dim3(256), 0, s, G, ncells, bits, shift,
+                               keys, ids);
+        else
+            hipLaunchKernelGGL(cell_key_kernel<3>, grid, dim3(256), 0, s, G, ncells, bits, shift,
+                               keys, ids);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_out, ids, ids_out,
+                                               (int)ncells, 0, D * bits, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rank_scatter_kernel, grid, dim3(256), 0, s, ids_out, ncells, rank);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *p : {(void *)keys, (void *)keys_out, (void *)ids, (void *)ids_out, tmp})
+        if (p) (void)hipFree(p);
+    return e;
+}
+
+// Target order: a counting sort of the targets by the Hilbert rank of their
+// cell -- histogram, exclusive scan, scatter.  Targets of one bin (one cell,
+// or a few consecutive cells for calls with far fewer targets than cells)
+// keep no particular order among themselves, which no result depends on:
+// every neighbour list is canonical and every system is solved on its own.
+// Only kernels, with no memset and no host synchronisation, so a predict
+// call can be captured into a HIP graph.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__global__ void bin_hist_kernel(const uint32_t *bins, int64_t T, int32_t *hist) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < T) atomicAdd(&hist[bins[t]], 1);
+}
+
+// Exclusive scan of a block's kScanThreads values in LDS (returned per
+// thread) and their total.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t *lds, int32_t &total) {
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {   // Hillis-Steele inclusive scan
+        const int32_t u = tid >= off ? lds[tid - off] : 0;
+        __syncthreads();
+        lds[tid] += u;
+        __syncthreads();
+    }
+    total = lds[kScanThreads - 1];
+    const int32_t r = lds[tid] - v;
+    __syncthreads();
+    return r;
+}
+
+// Pass 1: the sum of each chunk of kScanChunk counts.
+__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(const int32_t *hist, int64_t nb,
+                                                                  int32_t *sums) {
+    __shared__ int32_t lds[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    int32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const int64_t j = base + i * kScanThreads + threadIdx.x;   // coalesced
+        v += j < nb ? hist[j] : 0;
+    }
+    int32_t total;
+    (void)block_exclusive_scan(v, lds, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Pass 2 (one workgroup): exclusive scan of the chunk sums in place.
+__global__ __launch_bounds__(kScanThreads) void scan_chunks_kernel(int32_t *sums, int64_t n) {
+    __shared__ int32_t lds[kScanThreads];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < n; base += kScanThreads) {
+        const int64_t j = base + threadIdx.x;
+        const int32_t v = j < n ? sums[j] : 0;
+        int32_t total;
+        const int32_t ex = block_exclusive_scan(v, lds, total);
+        if (j < n) sums[j] = carry + ex;
+        carry += total;
+    }
+}
+
+// Pass 3: exclusive prefix of every count, from its chunk's offset.
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(int32_t *hist, int64_t nb,
+                                                                   const int32_t *sums) {
+    __shared__ int32_t chunk[kScanChunk];
+    __shared__ int32_t lds[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const int64_t j = base + i * kScanThreads + threadIdx.x;
+        chunk[i * kScanThreads + threadIdx.x] = j < nb ? hist[j] : 0;
+    }
+    __syncthreads();
+    int32_t v[kScanItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {   // thread's own consecutive run
+        v[i] = chunk[threadIdx.x * kScanItems + i];
+        sum += v[i];
+    }
+    int32_t total;
+    int32_t run = sums[blockIdx.x] + block_exclusive_scan(sum, lds, total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        chunk[threadIdx.x * kScanItems + i] = run;
+        run += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const int64_t j = base + i * kScanThreads + threadIdx.x;
+        if (j < nb) hist[j] = chunk[i * kScanThreads + threadIdx.x];
+    }
+}
+
+__global__ void bin_scatter_kernel(const uint32_t *bins, int64_t T, int32_t *offs, int32_t *perm) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < T) perm[atomicAdd(&offs[bins[t]], 1)] = (int32_t)t;
+}
+
+namespace {
+
+// Bins of a call: cells, merged in runs of 2^shift consecutive ranks while
+// the grid has more than max(T, 65536) cells (a histogram far larger than
+// the call would cost more to clear and scan than it saves).
+int order_shift(int64_t ncells, int64_t T) {
+    int shift = 0;
+    while ((ncells - 1) >> shift >= std::max<int64_t>(T, 65536)) shift++;
+    return shift;
+}
+
+int64_t order_bins(int64_t ncells, int64_t T) {
+    return ((ncells - 1) >> order_shift(ncells, T)) + 1;
+}
+
+}  // namespace
+
+size_t tile_sort_workspace(int64_t T, int64_t ncells) {
+    const int64_t nb = order_bins(std::max<int64_t>(ncells, 1), T);
+    const int64_t chunks = (nb + kScanChunk - 1) / kScanChunk;
+    return sizeof(int32_t) * ((nb + 63) / 64 * 64 + chunks);
+}
+
+hipError_t launch_tile_order(int D, const double *targets, int64_t T, const SearchXf &xf,
+                             const CellGridView &G, const uint32_t *cell_rank, uint32_t *bins,
+                             int32_t *perm, void *tmp, size_t tmp_bytes, hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    if (D != 2 && D != 3) return hipErrorInvalidValue;
+    const int64_t ncells = (int64_t)G.nc[0] * G.nc[1] * G.nc[2];
+    if (tmp_bytes < tile_sort_workspace(T, ncells)) return hipErrorInvalidValue;
+    const int bshift = order_shift(ncells, T);
+    const int64_t nb = ((ncells - 1) >> bshift) + 1;
+    const int64_t chunks = (nb + kScanChunk - 1) / kScanChunk;
+    int32_t *hist = static_cast<int32_t *>(tmp);
+    int32_t *sums = hist + (nb + 63) / 64 * 64;
     const dim3 grid((unsigned)((T + 255) / 256));
     if (D == 2)
-        hipLaunchKernelGGL(hilbert_kernel<2>, grid, dim3(256), 0, s, targets, T, xf, G, bits, shift,
-                           keys, vals);
+        hipLaunchKernelGGL(target_bin_kernel<2>, grid, dim3(256), 0, s, targets, T, xf, G,
+                           cell_rank, bshift, bins);
     else
-        hipLaunchKernelGGL(hilbert_kernel<3>, grid, dim3(256), 0, s, targets, T, xf, G, bits, shift,
-                           keys, vals);
+        hipLaunchKernelGGL(target_bin_kernel<3>, grid, dim3(256), 0, s, targets, T, xf, G,
+                           cell_rank, bshift, bins);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_out, vals, perm, (int)T, 0,
-                                              D * bits, s);
+    if ((e = launch_fill_i32(hist, nb, 0, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(bin_hist_kernel, grid, dim3(256), 0, s, bins, T, hist);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)chunks), dim3(kScanThreads), 0, s, hist, nb,
+                       sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads), 0, s, sums, chunks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)chunks), dim3(kScanThreads), 0, s, hist, nb,
+                       sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(bin_scatter_kernel, grid, dim3(256), 0, s, bins, T, hist, perm);
+    return hipGetLastError();
 }
 
 static size_t tile_lds_bytes(int D, int k, int per_octant) {

@@ -562,6 +562,24 @@ class Kriging:
             self._h, ptr(targets), T, _PREC[precision], ptr(value), ptr(variance),
             ptr(lagrange), ptr(status), stream))
 
+    def capture_graph(self, targets, value, variance, lagrange=None, status=None,
+                      precision="fp64") -> "PredictGraph":
+        """predict_many_device captured as a HIP graph over these tensors
+        (gs_kriging_graph_create): runs the call once, then every
+        PredictGraph.launch repeats it on whatever `targets` holds, as one
+        graph launch.  The tensors must stay alive (the graph keeps
+        references) and keep their shapes; launches of one graph must not
+        overlap."""
+        if precision not in _PREC:
+            raise InvalidParameter(f"unknown precision {precision!r}")
+        T = self._check_device_args(targets, value, variance, lagrange, status)
+        ptr = (lambda x: None if x is None else x.data_ptr())
+        h = C.c_void_p()
+        _check(_abi.lib().gs_kriging_graph_create(
+            self._h, ptr(targets), T, _PREC[precision], ptr(value), ptr(variance),
+            ptr(lagrange), ptr(status), C.byref(h)))
+        return PredictGraph(h, self, (targets, value, variance, lagrange, status))
+
     def set_timing(self, enable: bool) -> None:
         """Records HIP events around the search and solve phases of every
         subsequent predict call (measurement hook)."""
@@ -587,6 +605,30 @@ class Kriging:
         _check(_abi.lib().gs_kriging_neighbors(self._h, _abi.dptr(t), T, k, _abi.i32ptr(idx),
                                                _abi.i32ptr(cnt)))
         return idx, cnt
+
+
+class PredictGraph:
+    """A captured predict call (Kriging.capture_graph)."""
+
+    def __init__(self, h, owner, buffers):
+        self._h = h
+        self._owner = owner          # the predictor and buffers outlive the graph
+        self._buffers = buffers
+
+    def launch(self, stream=None) -> None:
+        """Enqueues the captured call on `stream` (a hipStream_t as an int;
+        None or 0 is the null stream); returns immediately."""
+        if not self._h:
+            raise InvalidParameter("graph is closed")
+        _check(_abi.lib().gs_graph_launch(self._h, stream))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _abi.lib().gs_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def block_offsets(size, n) -> np.ndarray:

@@ -65,6 +65,10 @@ SIGNATURES = {
     "gs_kriging_predict_many": (C.c_int32, [_P, _DP, C.c_int64, C.c_int32, _DP, _DP, _DP, _I32P]),
     "gs_kriging_predict_many_device": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, _P]),
     "gs_kriging_predict_many_noalloc": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, _P]),
+    "gs_kriging_graph_create": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P,
+                                            C.POINTER(_P)]),
+    "gs_graph_launch": (C.c_int32, [_P, _P]),
+    "gs_graph_free": (None, [_P]),
     "gs_kriging_neighbors": (C.c_int32, [_P, _DP, C.c_int64, C.c_int64, _I32P, _I32P]),
     "gs_kriging_max_neighbors": (C.c_int64, [_P]),
     "gs_kriging_set_timing": (C.c_int32, [_P, C.c_int32]),

@@ -4,7 +4,9 @@ For every compiled shape of the lean solve kernels (kriging: dim, padded
 system size NB, drift terms M; Vecchia: dim, NB) it times the two group-size
 variants on a synthetic workload of that shape, and for the uneven octant
 workload it times the persistent grid of the size-bucketed launches at
-several wavefronts per CU.  The winners are written as the C table
+several wavefronts per CU; the shapes of the benchmark workloads are then
+timed again at their full sizes, which decide those rows.  The winners are
+written as the C table
 csrc/gs_tune_table.inc, which the library compiles in: the schedule is then a
 fixed function of (device architecture, shape), identical in every process.
 
@@ -144,6 +146,63 @@ def tune_vecchia(quick: bool):
     return rows
 
 
+def tune_bench_shapes(rows):
+    """The shapes of the benchmark workloads, timed at their full sizes (the
+    small generic workloads above can rank close variants the other way):
+    2-D ordinary kriging at k = 32 (4*10^6 targets), the uneven 3-D
+    universal-kriging octant search (4*10^6 targets, all three size buckets,
+    group-size variant and persistent grid together), and the 64-model
+    Vecchia batch at 10^6 points."""
+    import torch
+
+    import geostat_hip as gh
+    from geostat_hip import workloads
+
+    def set_row(family, D, NB, M, variant=None, wpc=None):
+        for r in rows:
+            if r[0] == family and r[1] == D and r[2] == NB and r[3] == M:
+                if variant is not None:
+                    r[6] = variant
+                if wpc is not None:
+                    r[7] = wpc
+
+    for name, shapes in (("ok2d_large", [(2, 32, 1)]),
+                         ("uk3d_octant", [(3, 16, 4), (3, 32, 4), (3, 64, 4)])):
+        w = getattr(workloads, name)()
+        k = gh.Kriging(w.coords, w.values, w.model, w.config)
+        t = w.targets()
+        d_t = torch.from_numpy(t).to("cuda:0")
+        outs = _outs(len(t))
+        grid = (16,) if name == "ok2d_large" else (8, 16, 32, 128)
+        best, best_ms = None, None
+        for var in (0, 1):
+            for wpc in grid:
+                ms = _with_env("GS_SOLVE_VARIANT", var, lambda: _with_env(
+                    "GS_WAVES_PER_CU", wpc, lambda: _time_solve(k, d_t, outs)))
+                print(f"{name}: variant {var}, {wpc} waves/CU -> {ms:.3f} ms", flush=True)
+                if best_ms is None or ms < best_ms:
+                    best, best_ms = (var, wpc), ms
+        for D, NB, M in shapes:
+            set_row("krige", D, NB, M, *best)
+        del k
+    c = workloads.vecchia_c3()
+    p = gh.vecchia_plan(c, 30)
+    o, nb, cnt = p.arrays()
+    vals = workloads.vecchia_field(c, o, nb, cnt, 30)
+    models = workloads.vecchia_models(64)
+    res = []
+    for var in (0, 1):
+        def run():
+            p.loglik_batch(vals, models)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                p.loglik_batch(vals, models)
+            return (time.perf_counter() - t0) / 3 * 1e3
+        res.append(_with_env("GS_SOLVE_VARIANT", var, run))
+        print(f"vecchia_c3: variant {var} -> {res[-1]:.3f} ms", flush=True)
+    set_row("vecchia", 2, 32, 0, int(np.argmin(res)))
+
+
 def write_table(rows, out: Path):
     lines = ["// Launch schedules measured on an MI355X (gfx950) by geostat_hip/tune.py:",
              "// {family, D, NB, M, lean, ext, group-size variant, waves per CU}."]
@@ -160,6 +219,7 @@ def main() -> int:
     rows = tune_krige(args.quick)
     tune_octant_grid(rows, args.quick)
     rows += tune_vecchia(args.quick)
+    tune_bench_shapes(rows)
     write_table(rows, Path(args.out))
     print(f"wrote {args.out} ({len(rows)} entries)", flush=True)
     return 0

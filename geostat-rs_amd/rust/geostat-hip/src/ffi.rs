@@ -31,6 +31,7 @@ pub type gs_ctx = c_void;
 pub type gs_kriging = c_void;
 pub type gs_vplan = c_void;
 pub type gs_server = c_void;
+pub type gs_graph = c_void;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
@@ -120,6 +121,12 @@ extern "C" {
                                            precision: i32, d_value: *mut f64,
                                            d_variance: *mut f64, d_lagrange: *mut f64,
                                            d_status: *mut i32, stream: *mut c_void) -> i32;
+    pub fn gs_kriging_graph_create(k: *mut gs_kriging, d_targets: *const f64, t: i64,
+                                   precision: i32, d_value: *mut f64, d_variance: *mut f64,
+                                   d_lagrange: *mut f64, d_status: *mut i32,
+                                   out: *mut *mut gs_graph) -> i32;
+    pub fn gs_graph_launch(g: *mut gs_graph, stream: *mut c_void) -> i32;
+    pub fn gs_graph_free(g: *mut gs_graph);
     pub fn gs_kriging_neighbors(k: *mut gs_kriging, targets: *const f64, t: i64, max_out: i64,
                                 idx: *mut i32, count: *mut i32) -> i32;
     pub fn gs_kriging_max_neighbors(k: *const gs_kriging) -> i64;

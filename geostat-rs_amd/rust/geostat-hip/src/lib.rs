@@ -437,6 +437,45 @@ impl<const D: usize> Kriging<D> {
                                                              lagrange, status, stream),
                          [0, 0])
     }
+
+    /// predict_many_device captured as a HIP graph over these buffers: the
+    /// call runs once, then every `PredictGraph::launch` repeats it on
+    /// whatever `targets` holds.
+    ///
+    /// # Safety
+    /// As `predict_many_device`; the buffers must outlive the graph.
+    pub unsafe fn capture_graph(&self, targets: *const f64, t: usize, precision: Precision,
+                                value: *mut f64, variance: *mut f64, lagrange: *mut f64,
+                                status: *mut i32) -> Result<PredictGraph<'_, D>> {
+        let mut g = std::ptr::null_mut();
+        status_to_result(ffi::gs_kriging_graph_create(self.h, targets, t as i64, precision.code(),
+                                                      value, variance, lagrange, status, &mut g),
+                         [0, 0])?;
+        Ok(PredictGraph { g, _k: self })
+    }
+}
+
+/// A captured predict call; borrows its predictor so that it cannot outlive it.
+pub struct PredictGraph<'a, const D: usize> {
+    g: *mut ffi::gs_graph,
+    _k: &'a Kriging<D>,
+}
+
+impl<const D: usize> PredictGraph<'_, D> {
+    /// Enqueues the captured call on `stream` (null: the null stream).
+    ///
+    /// # Safety
+    /// `stream` must be a stream of the predictor's GPU; launches of one
+    /// graph must not overlap.
+    pub unsafe fn launch(&self, stream: *mut c_void) -> Result<()> {
+        status_to_result(ffi::gs_graph_launch(self.g, stream), [0, 0])
+    }
+}
+
+impl<const D: usize> Drop for PredictGraph<'_, D> {
+    fn drop(&mut self) {
+        unsafe { ffi::gs_graph_free(self.g) }
+    }
 }
 
 impl Kriging<2> {

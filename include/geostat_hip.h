@@ -193,13 +193,35 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
  * library is loaded and held until the process ends; a call whose workspace
  * would exceed it (about 300,000 targets at 32 neighbours) fails with
  * GS_ERR_INVALID_PARAMETER before any work is enqueued, and so does a handle
- * with a global neighbourhood.  Every call on one GPU shares that GPU's
+ * with a global neighbourhood or neighbourhoods above 64 points.  Every call
+ * on one GPU shares that GPU's
  * scratch: calls that overlap on one GPU race (serialize them).  The kd-tree
  * replay of flagged targets stays on `stream`.  Results are bit-identical to
  * gs_kriging_predict_many_device's. */
 int32_t gs_kriging_predict_many_noalloc(gs_kriging *k, const double *d_targets, int64_t T,
                                         int32_t precision, double *d_value, double *d_variance,
                                         double *d_lagrange, int32_t *d_status, void *stream);
+
+/* A predict call captured as a HIP graph (no reference counterpart: the
+ * rayon par_map of predict_many, parallel.rs:9-17, replayed as one graph
+ * launch for callers that predict the same number of targets over and over,
+ * such as a serving loop or a simulation refreshing a field).
+ * gs_kriging_graph_create runs the call of gs_kriging_predict_many_device
+ * once (its results land in the output buffers) and records its launches
+ * over these buffers and a workspace owned by the graph; each
+ * gs_graph_launch then enqueues the whole call on `stream` as one graph
+ * launch, reading whatever the target buffer holds at that point.  Results
+ * are bit-identical to gs_kriging_predict_many_device's on the same targets.
+ * The predictor and the buffers must outlive the graph; launches of one graph
+ * must not overlap (they share its workspace); the predictor's own calls may
+ * run beside them.  Neighbourhoods above 64 points are refused
+ * (GS_ERR_INVALID_PARAMETER), since their work is sized on the host. */
+typedef struct gs_graph gs_graph;
+int32_t gs_kriging_graph_create(gs_kriging *k, const double *d_targets, int64_t T,
+                                int32_t precision, double *d_value, double *d_variance,
+                                double *d_lagrange, int32_t *d_status, gs_graph **out);
+int32_t gs_graph_launch(gs_graph *g, void *stream);
+void gs_graph_free(gs_graph *g);
 
 /* Kriging::neighbors (kriging.rs:441-490) for T targets: idx is T*max_out
  * int32 (original data indices in the reference's order, -1 padded),
