@@ -183,7 +183,9 @@ def bench_vecchia(args) -> dict:
             ll, st = plan.loglik_batch(vals, models, precision=prec)
         timed[prec] = (time.perf_counter() - t0) / args.steps
         lls[prec] = (ll, st)
-    best = min(precs, key=lambda q: timed[q])
+    # FP32-refined is reported only when it beats FP64 beyond run-to-run noise
+    best = "fp32r" if "fp32r" in precs and ("fp64" not in precs or
+                                           timed["fp32r"] < 0.98 * timed["fp64"]) else "fp64"
     dt = timed[best]
     ll, st = lls[best]
     grad_ms = None
@@ -444,7 +446,10 @@ def main() -> int:
     for prec in precs:
         runs[prec] = measure(prec)
         outs[prec] = (d_v.cpu().numpy(), d_var.cpu().numpy(), d_lag.cpu().numpy())
-    best = min(precs, key=lambda q: runs[q][0])
+    # FP32-refined is reported only when it beats FP64 beyond run-to-run noise
+    # (configurations outside its scope run the FP64 kernels under either name)
+    best = "fp32r" if "fp32r" in precs and ("fp64" not in precs or
+                                           runs["fp32r"][0] < 0.98 * runs["fp64"][0]) else "fp64"
     dt, calls, ms_search, ms_solve, st = runs[best]
     ok = int((st == 0).sum())
 

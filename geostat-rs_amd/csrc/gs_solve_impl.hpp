@@ -481,7 +481,8 @@ __device__ __forceinline__ bool factor_forward(double (&a)[NB / G][NB], double (
 // With X: ea[r] is added to the diagonal of the owned row of block r
 // (measurement error), and when n_off > 0 the target column is the block
 // average (1/U) sum_u C(x_i - (tg + off_u)) instead (kriging.rs:709-725).
-template <int D, int NB, int G, int Q, bool LEAN, bool X = false, bool GE = true>
+template <int D, int NB, int G, int Q, bool LEAN, bool X = false, bool GE = true,
+          int HS = stage_slots<NB>()>
 __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const int cnt,
                                                    const double (&rowx)[NB / G][D],
                                                    const double *xy, const double *tg,
@@ -497,7 +498,6 @@ __device__ __forceinline__ bool build_factor_solve(const DevModel &Mdl, const in
     const double nug = Mdl.nugget;
     double a[R][NB];
     constexpr int NE = NB - 1;              // entries per lane and block pair
-    constexpr int HS = stage_slots<NB>();   // slots per staging pass
     constexpr int NPS = (NE + HS - 1) / HS; // staging passes
     static_for<0, (R + 1) / 2>([&](auto PP) {
         constexpr int rA = decltype(PP)::value;
@@ -746,17 +746,36 @@ __device__ __forceinline__ double drift_solve(double (&y)[Q][R], const double (&
     return value;
 }
 
-// Occupancy target handed to the register allocator: three wavefronts per
-// SIMD (168 VGPRs per lane) for 32-point systems on 16-lane groups with at
-// most one drift term, where that costs only a few spills outside the
-// factorization; the other shapes keep the allocator's choice.
-template <int NB, int G, int M>
-constexpr int krige_waves() { return (NB == 32 && G == 16 && M <= 1) ? 3 : 1; }
+// Wavefronts per SIMD the lean kernels are compiled for (the occupancy
+// target handed to the register allocator).  At NB = 32,
+// G = 16, M <= 1: three (168 VGPRs, no spills; registers and LDS both allow
+// three).  At NB = 64, G = 32: two, which costs 30-58 VGPRs spilled to
+// scratch but doubles the wavefronts that hide the factorization's LDS
+// round trips (the 3-D, M = 4 solve of
+// 4 * 10^6 octant systems: 108 -> 77 ms).  The general-class and extended
+// instantiations keep the compiler's choice: they would spill far more.
+template <int NB, int G, int M, bool LEAN, bool X>
+constexpr int krige_waves() {
+    if (NB == 32 && G == 16 && M <= 1) return 3;
+    if (NB == 64 && G == 32 && LEAN && !X) return 2;
+    return 1;
+}
+
+// The 2-D, 32-point Vecchia kernel at 16 lanes per point fits three
+// wavefronts per SIMD (<= 168 VGPRs, LDS below a twelfth of the CU's).
+template <int D, int NB, int G, bool LEAN>
+constexpr int vecchia_waves() { return (D == 2 && NB == 32 && G == 16 && LEAN) ? 3 : 1; }
+// Its staging passes are 10 slots wide (four passes over the 31 entries of
+// a lane) so that the LDS of three wavefronts per SIMD fits.
+template <int D, int NB, int G, bool LEAN>
+constexpr int vecchia_stage() {
+    return vecchia_waves<D, NB, G, LEAN>() == 3 ? 10 : stage_slots<NB>();
+}
 
 // X = true adds the extended systems: measurement error (kriging.rs:181-218,
 // 544), external drift (223-260, 431-437) and block kriging (678-777).
 template <int D, int NB, int G, int M, bool LEAN, bool X>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_waves<NB, G, M>()))) void krige_fp64_kernel(SolveParams P) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_waves<NB, G, M, LEAN, X>()))) void krige_fp64_kernel(SolveParams P) {
     constexpr int R = NB / G;
     constexpr int Q = 2 + M;
     constexpr int GPW = kWave / G;
@@ -926,17 +945,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(krige_wav
 // depend on scheduling.
 
 template <int D, int NB, int G, bool LEAN>
-__global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(vecchia_waves<D, NB, G, LEAN>()))) void vecchia_kernel(VecchiaParams P) {
     constexpr int R = NB / G;
     constexpr int GPW = kWave / G;
+    constexpr int HS = vecchia_stage<D, NB, G, LEAN>();
     __shared__ double s_xy[GPW * NB * D];
-    __shared__ double s_stage[stage_slots<NB>() * kWave];
+    __shared__ double s_z[GPW * NB];
+    __shared__ double s_stage[HS * kWave];
     __shared__ double s_sum[GPW * kMaxModelsPerLaunch];
     __shared__ __attribute__((aligned(16))) double s_col[GPW * ColStride<NB, 2, G>::value];
     const int lane = threadIdx.x;
     const int g = lane % G;
     const int grp = lane / G;
     double *xy = s_xy + grp * NB * D;
+    double *zs = s_z + grp * NB;
     double *colbuf = s_col + grp * ColStride<NB, 2, G>::value;
     const int nm = P.n_models;
     for (int p = g; p < nm; p += G) s_sum[grp * kMaxModelsPerLaunch + p] = 0.0;
@@ -961,31 +983,26 @@ __global__ __launch_bounds__(kWave) void vecchia_kernel(VecchiaParams P) {
             const int64_t id = j < cnt ? nbl[j] : 0;
 #pragma unroll
             for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt ? P.coords[id * D + d] : 0.0;
-        }
-        double zr[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int i = row_of<G>(r, g);
-            zr[r] = i < cnt ? P.zc[nbl[i]] : 0.0;
+            zs[j] = j < cnt ? P.zc[id] : 0.0;
         }
         wave_sync();
-        double rowx[R][D];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int i = row_of<G>(r, g);
-#pragma unroll
-            for (int d = 0; d < D; d++) rowx[r][d] = i < cnt ? xy[i * D + d] : 0.0;
-        }
         for (int p = 0; p < nm; p++) {
             const DevModel &Mdl = P.models[p];
             const double sill = Mdl.c0;
             double mu = 0.0, var = sill;
             bool ok = true;
             if (cnt > 0) {
-                double y[2][R];
+                // the owned rows' coordinates and values, re-read from LDS
+                // for every model rather than held across the model loop
+                double y[2][R], rowx[R][D];
 #pragma unroll
-                for (int r = 0; r < R; r++) y[1][r] = zr[r];
-                ok = build_factor_solve<D, NB, G, 2, LEAN, false, false>(
+                for (int r = 0; r < R; r++) {
+                    const int i = row_of<G>(r, g);
+                    y[1][r] = zs[i];
+#pragma unroll
+                    for (int d = 0; d < D; d++) rowx[r][d] = xy[i * D + d];
+                }
+                ok = build_factor_solve<D, NB, G, 2, LEAN, false, false, HS>(
                     Mdl, cnt, rowx, xy, tg, s_stage, colbuf, lane, g, 1e-12 * sill, y);
                 double s01 = 0.0, s00 = 0.0;
 #pragma unroll

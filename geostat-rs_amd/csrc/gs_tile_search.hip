@@ -765,9 +765,35 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanChunk = kScanThreads * kScanItems;
 
-__global__ void bin_hist_kernel(const uint32_t *bins, int64_t T, int32_t *hist) {
+// Consecutive targets often share a bin (a grid of targets walks through a
+// cell several targets at a time), so the lanes of a wavefront are grouped
+// into runs of equal bins and the first lane of a run adds the run's length
+// with one atomic.  heads: bit l set when lane l starts a run; lanes past T
+// carry a bin no target has, so they start a run of their own.
+struct BinRun {
+    uint32_t bin;
+    int lane, len;
+    uint64_t heads;
+    bool head;
+};
+
+__device__ __forceinline__ BinRun bin_run(const uint32_t *bins, int64_t t, int64_t T) {
+    BinRun r;
+    r.bin = t < T ? bins[t] : 0xffffffffu;
+    r.lane = (int)(threadIdx.x & 63);
+    const uint32_t prev = __shfl_up(r.bin, 1);   // lane 0 keeps its own value
+    r.head = r.lane == 0 || prev != r.bin;
+    r.heads = __ballot(r.head);
+    const uint64_t after = r.lane == 63 ? 0 : r.heads >> (r.lane + 1);
+    r.len = after ? __ffsll((long long)after) : 64 - r.lane;   // to the next head
+    return r;
+}
+
+__global__ __launch_bounds__(256) void bin_hist_kernel(const uint32_t *bins, int64_t T,
+                                                       int32_t *hist) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < T) atomicAdd(&hist[bins[t]], 1);
+    const BinRun r = bin_run(bins, t, T);
+    if (t < T && r.head) atomicAdd(&hist[r.bin], r.len);
 }
 
 // Exclusive scan of a block's kScanThreads values in LDS (returned per
@@ -852,9 +878,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(int32_t *hist,
     }
 }
 
-__global__ void bin_scatter_kernel(const uint32_t *bins, int64_t T, int32_t *offs, int32_t *perm) {
+// One atomic per run; the run's lanes take consecutive slots from the
+// base its first lane received.
+__global__ __launch_bounds__(256) void bin_scatter_kernel(const uint32_t *bins, int64_t T,
+                                                          int32_t *offs, int32_t *perm) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < T) perm[atomicAdd(&offs[bins[t]], 1)] = (int32_t)t;
+    const BinRun r = bin_run(bins, t, T);
+    int32_t base = 0;
+    if (t < T && r.head) base = atomicAdd(&offs[r.bin], r.len);
+    // first lane of this lane's run: the highest head at or below it
+    const uint64_t upto = r.lane == 63 ? r.heads : r.heads & ((2ull << r.lane) - 1);
+    const int start = 63 - __clzll((long long)upto);
+    base = __shfl(base, start);
+    if (t < T) perm[base + (r.lane - start)] = (int32_t)t;
 }
 
 namespace {
