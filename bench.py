@@ -506,6 +506,23 @@ def main() -> int:
         good = (runs["fp64"][4] == 0) & (runs["fp32r"][4] == 0)
         relerr = lambda x, y: float(np.abs(x[good] - y[good]).max() / np.abs(x[good]).max())
         out["config"]["fp32r_rel_err_vs_fp64"] = {"value": relerr(a0, b0), "variance": relerr(a1, b1)}
+    if "fp64" in precs:
+        # the same call captured once as a HIP graph and replayed
+        g = k.capture_graph(d_t, d_v, d_var, d_lag, d_st, precision="fp64")
+        for _ in range(args.warmup):
+            g.launch(sh)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.launch(sh)
+        torch.cuda.synchronize()
+        gdt = time.perf_counter() - t0
+        same = all(np.array_equal(x, y, equal_nan=True) for x, y in
+                   zip((d_v.cpu().numpy(), d_var.cpu().numpy(), d_lag.cpu().numpy()), outs["fp64"]))
+        g.close()
+        out["config"]["graph_fp64_targets_per_s"] = T * args.steps / gdt
+        out["config"]["graph_fp64_ms_per_step"] = gdt / args.steps * 1e3
+        out["config"]["graph_equals_eager"] = bool(same)
     if args.host_io:
         k.predict_many(targets)                      # warm the host path
         t0 = time.perf_counter()

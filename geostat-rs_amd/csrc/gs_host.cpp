@@ -1648,13 +1648,15 @@ struct gs_graph {
 
 extern "C" {
 
-// Capture: one eager pass on a private stream sizes every workspace buffer,
-// then the same launches are recorded by stream capture, which therefore
-// allocates nothing.  Every launch of the path is a kernel (counters are
-// cleared by fill kernels, the target order is a counting sort of kernels),
-// and the path reads nothing back to the host, so the graph holds kernel
-// nodes only.  The kd-tree replay of flagged targets stays on the captured
-// stream.
+// Capture: one eager pass on a private stream sizes every workspace buffer
+// and creates the workspace's side stream and events, then the same
+// launches are recorded by stream capture, which therefore allocates
+// nothing.  Every launch of the path is a kernel (counters are cleared by
+// fill kernels, the target order and the selection of unflagged targets are
+// scans made of kernels), and the path reads nothing back to the host, so
+// the graph holds kernel nodes only.  For large calls the kd-tree replay of
+// flagged targets forks onto the side stream and joins back through the
+// workspace's events, which capture records as a parallel branch.
 int32_t gs_kriging_graph_create(gs_kriging *k, const double *d_targets, int64_t T,
                                 int32_t precision, double *d_value, double *d_variance,
                                 double *d_lagrange, int32_t *d_status, gs_graph **out) {
@@ -1682,12 +1684,12 @@ int32_t gs_kriging_graph_create(gs_kriging *k, const double *d_targets, int64_t 
     } os;
     HIP_TRY(hipStreamCreateWithFlags(&os.s, hipStreamNonBlocking));
     int32_t st = predict_device(k, g->w, d_targets, T, precision, d_value, d_variance, d_lagrange,
-                                d_status, os.s, false, false);
+                                d_status, os.s, false, true);
     if (st != GS_OK) return st;
     HIP_TRY(hipStreamSynchronize(os.s));
     HIP_TRY(hipStreamBeginCapture(os.s, hipStreamCaptureModeThreadLocal));
     st = predict_device(k, g->w, d_targets, T, precision, d_value, d_variance, d_lagrange, d_status,
-                        os.s, false, false);
+                        os.s, false, true);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(os.s, &graph);   // ends capture on every path
     if (st != GS_OK) {

@@ -2,7 +2,6 @@
 // (size bucketing, deterministic Vecchia reduction).  The template bodies
 // are in gs_solve_impl.hpp; each (D, NB, M) instantiation is compiled in
 // its own generated translation unit (see build_lib.py).
-#include <hipcub/hipcub.hpp>
 
 #include "gs_internal.hpp"
 
@@ -39,12 +38,6 @@ __global__ void bucket_kernel(const int32_t *count, const int32_t *perm, int64_t
             lists[(int64_t)q * cap + base + __popcll(mask & below)] = t;
     }
 }
-
-// Selection predicate: a target whose kd-tree replay is still pending.
-struct Unmarked {
-    const uint8_t *mark;
-    __device__ bool operator()(const int32_t &t) const { return mark[t] == 0; }
-};
 
 // Per-model sum of the per-block partials in a fixed order: lane-strided
 // partial sums, then a fixed pairwise tree.
@@ -264,20 +257,6 @@ hipError_t launch_bucket(const int32_t *count, const int32_t *perm, int64_t T, i
     hipLaunchKernelGGL(bucket_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, count,
                        perm, T, n_dev, skip, lists, lens, list_cap);
     return hipGetLastError();
-}
-
-size_t select_workspace(int64_t T) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceSelect::If(nullptr, bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
-                                   (int32_t *)nullptr, (int)T, Unmarked{nullptr});
-    return bytes;
-}
-
-hipError_t launch_select_unmarked(const int32_t *perm, const uint8_t *mark, int64_t T,
-                                  int32_t *out, int32_t *out_len, void *tmp, size_t tmp_bytes,
-                                  hipStream_t s) {
-    if (T == 0) return hipSuccess;
-    return hipcub::DeviceSelect::If(tmp, tmp_bytes, perm, out, out_len, (int)T, Unmarked{mark}, s);
 }
 
 template <int D, int V, bool LEAN>

@@ -893,6 +893,52 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const uint32_t *bins, 
     if (t < T) perm[base + (r.lane - start)] = (int32_t)t;
 }
 
+// Order-preserving selection of the targets of `perm` whose kd-tree
+// replay is not pending (mark[t] == 0): keep flags, their exclusive scan
+// (the same three passes), scatter.  Kernels only, like the order above.
+__global__ void keep_kernel(const int32_t *perm, const uint8_t *mark, int64_t T, int32_t *pos) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < T) pos[i] = mark[perm[i]] == 0 ? 1 : 0;
+}
+
+__global__ void compact_kernel(const int32_t *perm, const uint8_t *mark, int64_t T,
+                               const int32_t *pos, int32_t *out, int32_t *out_len) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const bool keep = mark[perm[i]] == 0;
+    if (keep) out[pos[i]] = perm[i];
+    if (i == T - 1) *out_len = pos[i] + (keep ? 1 : 0);
+}
+
+size_t select_workspace(int64_t T) {
+    const int64_t chunks = (T + kScanChunk - 1) / kScanChunk;
+    return sizeof(int32_t) * ((T + 63) / 64 * 64 + chunks);
+}
+
+hipError_t launch_select_unmarked(const int32_t *perm, const uint8_t *mark, int64_t T,
+                                  int32_t *out, int32_t *out_len, void *tmp, size_t tmp_bytes,
+                                  hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    if (tmp_bytes < select_workspace(T)) return hipErrorInvalidValue;
+    const int64_t chunks = (T + kScanChunk - 1) / kScanChunk;
+    int32_t *pos = static_cast<int32_t *>(tmp);
+    int32_t *sums = pos + (T + 63) / 64 * 64;
+    const dim3 grid((unsigned)((T + 255) / 256));
+    hipLaunchKernelGGL(keep_kernel, grid, dim3(256), 0, s, perm, mark, T, pos);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)chunks), dim3(kScanThreads), 0, s, pos, T,
+                       sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads), 0, s, sums, chunks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)chunks), dim3(kScanThreads), 0, s, pos, T,
+                       sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(compact_kernel, grid, dim3(256), 0, s, perm, mark, T, pos, out, out_len);
+    return hipGetLastError();
+}
+
 namespace {
 
 // Bins of a call: cells, merged in runs of 2^shift consecutive ranks while

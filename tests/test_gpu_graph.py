@@ -90,7 +90,7 @@ def test_graph_octant_3d_buckets():
                            search_radius=90.0, min_neighbors=4)
     k = gh.Kriging(c, v, m, cfg)
     print("graph octant 3-D", flush=True)
-    _replays(k, _target_sets(212, 30_000, [1000.0, 1000.0, 200.0], recipe=1), "fp64")
+    _replays(k, _target_sets(212, 70_000, [1000.0, 1000.0, 200.0], recipe=1), "fp64")
 
 
 def test_graph_small_searchless_global_and_drift():
