@@ -191,7 +191,7 @@ int32_t gs_kriging_predict_many_device(gs_kriging *k, const double *d_targets, i
  * HIP graph capture, tight serving loops or harnesses that forbid hipMalloc).
  * The workspace comes from a fixed 64 MiB scratch per GPU, allocated when the
  * library is loaded and held until the process ends; a call whose workspace
- * would exceed it (about 300,000 targets at 32 neighbours) fails with
+ * would exceed it (about 440,000 targets at 32 neighbours) fails with
  * GS_ERR_INVALID_PARAMETER before any work is enqueued, and so does a handle
  * with a global neighbourhood or neighbourhoods above 64 points.  Every call
  * on one GPU shares that GPU's
