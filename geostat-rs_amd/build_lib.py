@@ -174,6 +174,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (CSRC / "gs_global.hip", EXACT),
         (CSRC / "gs_big.hip", EXACT),
         (CSRC / "gs_plan.hip", EXACT),
+        (CSRC / "gs_maxmin.hip", EXACT),
     ]
     units += [(p, []) for p in _gen_sources()]
     jobs = jobs or max(1, min(16, os.cpu_count() or 4))

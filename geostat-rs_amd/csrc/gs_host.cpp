@@ -26,6 +26,7 @@
 
 #include "../../include/geostat_hip.h"
 #include "gs_internal.hpp"
+#include "gs_maxmin.hpp"
 
 using gs::DevModel;
 
@@ -2340,9 +2341,23 @@ void grid_knn(const BGrid &G, const double *target, int64_t k, int64_t limit, KH
 // 64 bucket-ordered slots: a lowered key only costs a block rescan when it
 // was its block's maximum.  The order is the same as the lazy heap's; the
 // work is ~n log n without the lazy refreshes' 1-NN queries.
-void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &order) {
-    order.clear();
-    if (n == 0) return;
+//
+// The set-up (first point, bucket grid, bucket-ordered coordinates) is
+// shared with the GPU traversal, which hands its keys to the host traversal
+// when its batches stay small.
+struct MaxminSetup {
+    int D = 2;
+    int64_t n = 0;
+    int64_t first = 0;
+    uint32_t qf = 0;                 // slot of the first point
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    BGrid G;                         // ~1 point per bucket over all points
+    std::vector<double> cc;          // [slot][D]
+};
+
+void maxmin_setup(int D, const double *coords, int64_t n, MaxminSetup &S) {
+    S.D = D;
+    S.n = n;
     double c[3] = {0, 0, 0};
     for (int64_t i = 0; i < n; i++)
         for (int d = 0; d < D; d++) c[d] += coords[i * D + d] / (double)n;
@@ -2352,35 +2367,36 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
         double dd = dist2(coords + i * D, c, D);
         if (total_key(dd) < total_key(bd)) { bd = dd; first = i; }
     }
-    order.reserve((size_t)n);
-    order.push_back(first);
-    if (n == 1) return;
-    double mn[3], mx[3];
-    for (int d = 0; d < D; d++) { mn[d] = coords[d]; mx[d] = coords[d]; }
+    S.first = first;
+    for (int d = 0; d < D; d++) { S.mn[d] = coords[d]; S.mx[d] = coords[d]; }
     for (int64_t i = 0; i < n; i++)
         for (int d = 0; d < D; d++) {
-            mn[d] = std::fmin(mn[d], coords[i * D + d]);
-            mx[d] = std::fmax(mx[d], coords[i * D + d]);
+            S.mn[d] = std::fmin(S.mn[d], coords[i * D + d]);
+            S.mx[d] = std::fmax(S.mx[d], coords[i * D + d]);
         }
-    BGrid G;                               // ~1 point per bucket
-    G.init(D, mn, mx, n);
-    G.build_all(coords, n);
+    S.G.init(D, S.mn, S.mx, n);
+    S.G.build_all(coords, n);
     // Everything below lives in bucket order (slot q holds point items[q]),
-    // so a range query reads contiguous coordinates and keys.  Keys are sums
-    // of squares (never -0.0 or NaN), for which total_cmp is the plain
-    // numeric order; a selected slot's key is -1.
-    const std::vector<uint32_t> &id = G.items;
-    std::vector<double> cc((size_t)(n * D));
-    std::vector<uint32_t> slot((size_t)n);
+    // so a range query reads contiguous coordinates and keys.
+    const std::vector<uint32_t> &id = S.G.items;
+    S.cc.resize((size_t)(n * D));
     for (int64_t q = 0; q < n; q++) {
-        slot[id[(size_t)q]] = (uint32_t)q;
-        for (int d = 0; d < D; d++) cc[(size_t)(q * D + d)] = coords[(int64_t)id[(size_t)q] * D + d];
+        if ((int64_t)id[(size_t)q] == first) S.qf = (uint32_t)q;
+        for (int d = 0; d < D; d++) S.cc[(size_t)(q * D + d)] = coords[(int64_t)id[(size_t)q] * D + d];
     }
-    std::vector<double> key((size_t)n);    // min squared distance to the selected set
-    const uint32_t qf = slot[(size_t)first];
-    for (int64_t q = 0; q < n; q++)
-        key[(size_t)q] = dist2(&cc[(size_t)(q * D)], coords + first * D, D);
-    key[qf] = -1.0;
+}
+
+// Continues the traversal: `order` holds the points ordered so far and
+// key[q] each other slot's squared distance to them (-1 for ordered slots).
+// Keys are sums of squares (never -0.0 or NaN), for which total_cmp is the
+// plain numeric order.
+void maxmin_traverse(MaxminSetup &S, std::vector<double> &key, std::vector<int64_t> &order) {
+    const int D = S.D;
+    const int64_t n = S.n;
+    const BGrid &G = S.G;
+    const std::vector<uint32_t> &id = G.items;
+    const std::vector<double> &cc = S.cc;
+    if ((int64_t)order.size() >= n) return;
     // per block: the slot of its (key, index) maximum, as MaxminKey
     // (vecchia.rs:114-125), with that key and index
     constexpr int64_t kB = 64;
@@ -2427,8 +2443,8 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
             win[(size_t)i] = w;
         }
     };
-    lowered(qf / kB);
-    for (int64_t sel = 1; sel < n; sel++) {
+    order.reserve((size_t)n);
+    for (int64_t sel = (int64_t)order.size(); sel < n; sel++) {
         const int32_t wb = win[1];
         const uint32_t p = bmax[(size_t)wb];
         const double kp = key[p];
@@ -2478,6 +2494,159 @@ void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &
     }
 }
 
+void maxmin_order(int D, const double *coords, int64_t n, std::vector<int64_t> &order) {
+    order.clear();
+    if (n == 0) return;
+    MaxminSetup S;
+    maxmin_setup(D, coords, n, S);
+    order.push_back(S.first);
+    if (n == 1) return;
+    std::vector<double> key((size_t)n);    // min squared distance to the selected set
+    for (int64_t q = 0; q < n; q++)
+        key[(size_t)q] = dist2(&S.cc[(size_t)(q * D)], coords + S.first * D, D);
+    key[S.qf] = -1.0;
+    maxmin_traverse(S, key, order);
+}
+
+// The same order on the GPU (gs_maxmin.hip): rounds of batched appends,
+// driven in chunks; the host finishes the traversal from the device's keys
+// when the recent rounds appended few points each (ties of one key value
+// make them retry, and lattices make batches small).  info (may be NULL):
+// rounds run, rounds that raised the threshold, points ordered on the GPU.
+int32_t maxmin_order_gpu(gs_ctx *ctx, int D, const double *coords, int64_t n,
+                         std::vector<int64_t> &order, int64_t *info, hipStream_t s) {
+    order.clear();
+    if (info) { info[0] = 0; info[1] = 0; info[2] = 0; }
+    if (n == 0) return GS_OK;
+    if (n >= (int64_t)1 << 31) return fail(GS_ERR_INVALID_PARAMETER, "more than 2^31 - 1 points");
+    MaxminSetup S;
+    maxmin_setup(D, coords, n, S);
+    if (n == 1) {
+        order.push_back(S.first);
+        if (info) info[2] = 1;
+        return GS_OK;
+    }
+    const BGrid &G = S.G;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int cap = gs::kMaxminCap;
+    DBuf d_cc, d_id, d_start, d_enc, d_cenc, d_cslot, d_bslot, d_brow, d_bbox, d_hist, d_order, d_st;
+    HIP_TRY(d_cc.ensure(sizeof(double) * n * D));
+    HIP_TRY(d_id.ensure(sizeof(uint32_t) * n));
+    HIP_TRY(d_start.ensure(sizeof(int64_t) * G.start.size()));
+    HIP_TRY(d_enc.ensure(sizeof(unsigned long long) * n));
+    HIP_TRY(d_cenc.ensure(sizeof(unsigned long long) * cap));
+    HIP_TRY(d_cslot.ensure(sizeof(uint32_t) * cap));
+    HIP_TRY(d_bslot.ensure(sizeof(uint32_t) * cap));
+    HIP_TRY(d_brow.ensure(sizeof(int64_t) * (cap + 1)));
+    HIP_TRY(d_bbox.ensure(sizeof(int32_t) * 6 * cap));
+    HIP_TRY(d_hist.ensure(sizeof(int32_t) * gs::kMaxminBins));
+    HIP_TRY(d_order.ensure(sizeof(int64_t) * n));
+    HIP_TRY(d_st.ensure(sizeof(gs::MaxminState)));
+    HIP_TRY(hipMemcpyAsync(d_cc.p, S.cc.data(), sizeof(double) * n * D, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_id.p, G.items.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_start.p, G.start.data(), sizeof(int64_t) * G.start.size(),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(gs::launch_zero_i32(d_hist.as<int32_t>(), gs::kMaxminBins, s));
+    // initial threshold: a power of two above every squared distance
+    double diag2 = 0.0;
+    for (int d = 0; d < D; d++) diag2 += (S.mx[d] - S.mn[d]) * (S.mx[d] - S.mn[d]);
+    uint64_t tb;
+    {
+        const double t = diag2 * 4.0 + 1e-300;
+        std::memcpy(&tb, &t, sizeof(tb));
+    }
+    gs::MaxminState st0;
+    std::memset(&st0, 0, sizeof(st0));
+    st0.tau_key = ((tb >> 52) + 1) << 52;
+    st0.s = 1;
+    HIP_TRY(hipMemcpyAsync(d_st.p, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    gs::MaxminParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.n = n;
+    P.cc = d_cc.as<double>();
+    P.id = d_id.as<uint32_t>();
+    P.enc = d_enc.as<unsigned long long>();
+    for (int d = 0; d < 3; d++) {
+        P.first[d] = d < D ? coords[S.first * D + d] : 0.0;
+        P.origin[d] = d < D ? G.origin[d] : 0.0;
+        P.nn[d] = d < D ? G.nn[d] : 1;
+        P.strides[d] = d < D ? G.strides[d] : 0;
+    }
+    P.first_slot = S.qf;
+    P.cell = G.cell;
+    P.start = d_start.as<int64_t>();
+    P.cap = cap;
+    P.cand_enc = d_cenc.as<unsigned long long>();
+    P.cand_slot = d_cslot.as<uint32_t>();
+    P.bslot = d_bslot.as<uint32_t>();
+    P.brow = d_brow.as<int64_t>();
+    P.bbox = d_bbox.as<int32_t>();
+    P.hist = d_hist.as<int32_t>();
+    P.order = d_order.as<int64_t>();
+    P.st = d_st.as<gs::MaxminState>();
+    HIP_TRY(gs::launch_maxmin_init(D, P, s));
+    HIP_TRY(gs::launch_maxmin_batch(D, P, s));
+    // Chunks of rounds; between chunks the state comes back.  The host
+    // takes over when the last chunk appended fewer than kMinRate points per
+    // round on average (each round costs a few kernel launches, the host
+    // appends a point in about a microsecond).
+    const char *ev = std::getenv("GS_MAXMIN_HOST_AFTER");   // test hook: hand over after this many rounds
+    const int64_t host_after = ev ? std::atoll(ev) : -1;
+    constexpr double kMinRate = 8.0;
+    gs::MaxminState st;
+    std::memset(&st, 0, sizeof(st));
+    int64_t chunk = 64, prev_s = 1, prev_rounds = 0, enqueued = 1;
+    for (;;) {
+        int64_t run = chunk;
+        if (host_after >= 0) run = std::min(run, std::max<int64_t>(0, host_after - enqueued));
+        if (run == 0) {
+            HIP_TRY(hipMemcpyAsync(&st, d_st.p, sizeof(st), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            break;
+        }
+        for (int64_t r = 0; r < run; r++) HIP_TRY(gs::launch_maxmin_round(D, P, s));
+        enqueued += run;
+        HIP_TRY(hipMemcpyAsync(&st, d_st.p, sizeof(st), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (std::getenv("GS_MAXMIN_TRACE"))
+            std::fprintf(stderr, "maxmin rounds %lld retries %lld sub %lld s %lld batch %d\n",
+                         (long long)st.rounds, (long long)st.retries, (long long)st.subrounds,
+                         (long long)st.s, st.batch);
+        if (st.done) break;
+        const double rate = (double)(st.s - prev_s) / (double)std::max<int64_t>(1, st.rounds - prev_rounds);
+        // early rounds append few points whatever the input: judge from 256 rounds on
+        if (st.rounds >= 256 && rate < kMinRate) break;
+        prev_s = st.s;
+        prev_rounds = st.rounds;
+        chunk = std::min<int64_t>(chunk * 2, 1024);
+    }
+    if (info) { info[0] = st.rounds; info[1] = st.retries; info[2] = st.s; }
+    order.resize((size_t)st.s);
+    HIP_TRY(hipMemcpyAsync(order.data(), d_order.p, sizeof(int64_t) * st.s, hipMemcpyDeviceToHost, s));
+    if (st.done) {
+        HIP_TRY(hipStreamSynchronize(s));
+        return GS_OK;
+    }
+    // keys around the last batch are lowered by the next round's first
+    // step: run it alone so that every key is exact
+    HIP_TRY(gs::launch_maxmin_update(D, P, s));
+    std::vector<unsigned long long> enc((size_t)n);
+    HIP_TRY(hipMemcpyAsync(enc.data(), d_enc.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> key((size_t)n);
+    for (int64_t q = 0; q < n; q++) {
+        const unsigned long long e = enc[(size_t)q];
+        if (e == 0) {
+            key[(size_t)q] = -1.0;
+        } else {
+            const unsigned long long kb = e - 1;
+            std::memcpy(&key[(size_t)q], &kb, sizeof(double));
+        }
+    }
+    maxmin_traverse(S, key, order);
+    return GS_OK;
+}
+
 int host_threads() {
     unsigned h = std::thread::hardware_concurrency();
     return (int)std::max(1u, std::min(h, 64u));
@@ -2496,6 +2665,23 @@ int32_t gs_maxmin_order(int32_t dim, const double *coords, int64_t n, int64_t *o
         if (!std::isfinite(coords[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite coordinate");
     std::vector<int64_t> o;
     maxmin_order(dim, coords, n, o);
+    std::copy(o.begin(), o.end(), order);
+    return GS_OK;
+}
+
+int32_t gs_maxmin_order_gpu(gs_ctx *ctx, int32_t dim, const double *coords, int64_t n,
+                            int64_t *order, int64_t *info) {
+    if (!ctx) return fail(GS_ERR_INVALID_PARAMETER, "ctx is NULL");
+    if (dim != 2 && dim != 3) return fail(GS_ERR_DIMENSION_MISMATCH, "dim must be 2 or 3");
+    if (n < 0) return fail(GS_ERR_INVALID_PARAMETER, "n < 0");
+    if (info) { info[0] = 0; info[1] = 0; info[2] = 0; }
+    if (n == 0) return GS_OK;
+    if (!coords || !order) return fail(GS_ERR_INVALID_PARAMETER, "NULL buffer");
+    for (int64_t i = 0; i < n * dim; i++)
+        if (!std::isfinite(coords[i])) return fail(GS_ERR_INVALID_PARAMETER, "non-finite coordinate");
+    std::vector<int64_t> o;
+    int32_t rc = maxmin_order_gpu(ctx, dim, coords, n, o, info, ctx->stream);
+    if (rc != GS_OK) return rc;
     std::copy(o.begin(), o.end(), order);
     return GS_OK;
 }
@@ -2530,7 +2716,9 @@ int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int6
         }
         P->order.assign(order, order + n);
     } else {
-        maxmin_order(dim, coords, n, P->order);
+        HIP_TRY(hipSetDevice(ctx->device));
+        int32_t rc = maxmin_order_gpu(ctx, dim, coords, n, P->order, nullptr, ctx->stream);
+        if (rc != GS_OK) return rc;
     }
     // Nearest predecessors (vecchia.rs:262-281): all points inserted in
     // ordered position; query k only sees positions < k.

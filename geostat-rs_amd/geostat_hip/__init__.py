@@ -848,6 +848,21 @@ def maxmin_order(coords) -> np.ndarray:
     return out
 
 
+def maxmin_order_gpu(coords, ctx: Optional[Context] = None, info: bool = False):
+    """maxmin_order (vecchia.rs:144-215) computed on the GPU; the same order
+    as maxmin_order.  With info=True also returns (rounds, threshold
+    retries, points ordered on the GPU before the host finished)."""
+    ctx = ctx or Context.default()
+    c = _f64(coords)
+    if c.ndim != 2 or c.shape[1] not in (2, 3):
+        raise DimensionMismatch(f"coords must be (n, 2|3), got {c.shape}")
+    out = np.empty(c.shape[0], dtype=np.int64)
+    inf = np.zeros(3, dtype=np.int64)
+    _check(_abi.lib().gs_maxmin_order_gpu(ctx._h, c.shape[1], _abi.dptr(c), c.shape[0],
+                                           _abi.i64ptr(out), _abi.i64ptr(inf)))
+    return (out, tuple(int(v) for v in inf)) if info else out
+
+
 def vecchia_plan(coords, m: int, order=None, ctx: Optional[Context] = None) -> VecchiaPlan:
     """vecchia_plan (vecchia.rs:232-282)."""
     return VecchiaPlan(coords, m, order, ctx)

@@ -74,6 +74,7 @@ SIGNATURES = {
     "gs_kriging_set_timing": (C.c_int32, [_P, C.c_int32]),
     "gs_kriging_timing_summary": (C.c_int32, [_P, _I64P, _DP, _DP]),
     "gs_maxmin_order": (C.c_int32, [C.c_int32, _DP, C.c_int64, _I64P]),
+    "gs_maxmin_order_gpu": (C.c_int32, [C.c_void_p, C.c_int32, _DP, C.c_int64, _I64P, _I64P]),
     "gs_vecchia_plan_new": (C.c_int32, [_P, C.c_int32, _DP, C.c_int64, C.c_int64, _I64P,
                                         C.POINTER(_P)]),
     "gs_vecchia_plan_free": (None, [_P]),

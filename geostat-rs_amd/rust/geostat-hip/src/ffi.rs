@@ -154,6 +154,8 @@ extern "C" {
 
     // Vecchia
     pub fn gs_maxmin_order(dim: i32, coords: *const f64, n: i64, order: *mut i64) -> i32;
+    pub fn gs_maxmin_order_gpu(ctx: *mut gs_ctx, dim: i32, coords: *const f64, n: i64,
+                               order: *mut i64, info: *mut i64) -> i32;
     pub fn gs_vecchia_plan_new(ctx: *mut gs_ctx, dim: i32, coords: *const f64, n: i64, m: i64,
                                order: *const i64, out: *mut *mut gs_vplan) -> i32;
     pub fn gs_vecchia_plan_free(p: *mut gs_vplan);

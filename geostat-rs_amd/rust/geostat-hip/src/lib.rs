@@ -582,6 +582,17 @@ pub struct VecchiaPlan<const D: usize> {
 unsafe impl<const D: usize> Send for VecchiaPlan<D> {}
 unsafe impl<const D: usize> Sync for VecchiaPlan<D> {}
 
+/// maxmin_order (vecchia.rs:144-215), computed on the context's GPU.
+pub fn maxmin_order<const D: usize>(ctx: &Context, coords: &[[f64; D]]) -> Result<Vec<usize>> {
+    let mut o = vec![0i64; coords.len()];
+    let st = unsafe {
+        ffi::gs_maxmin_order_gpu(ctx.h, D as i32, flat(coords).as_ptr(), coords.len() as i64,
+                                 o.as_mut_ptr(), ptr::null_mut())
+    };
+    status_to_result(st, [0, 0])?;
+    Ok(o.into_iter().map(|x| x as usize).collect())
+}
+
 /// vecchia_plan (vecchia.rs:232-282): maxmin ordering (order = None) or the
 /// given permutation, then each point's m nearest predecessors.
 pub fn vecchia_plan<const D: usize>(ctx: &Context, coords: &[[f64; D]], m: usize,

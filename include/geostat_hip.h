@@ -280,6 +280,15 @@ int32_t gs_cv_block(gs_ctx *ctx, int32_t dim, const double *coords, const double
  * is the one farthest from those already ordered, ties to the larger
  * index. */
 int32_t gs_maxmin_order(int32_t dim, const double *coords, int64_t n, int64_t *order);
+/* The same order computed on the device of ctx (maxmin_order,
+ * vecchia.rs:144-215): rounds that each append the longest run of the
+ * largest-key candidates that cannot lower each other's keys; the host
+ * finishes the traversal when rounds append few points (tie-heavy inputs such
+ * as lattices).  info (3 entries, may be NULL): rounds run on the device,
+ * rounds that raised the candidate threshold instead of appending, points
+ * ordered on the device.  Used by gs_vecchia_plan_new when order is NULL. */
+int32_t gs_maxmin_order_gpu(gs_ctx *ctx, int32_t dim, const double *coords, int64_t n,
+                            int64_t *order, int64_t *info);
 
 /* vecchia_plan (vecchia.rs:232-282): maxmin ordering (order == NULL) or the
  * given permutation, then each ordered point's m nearest predecessors. */
