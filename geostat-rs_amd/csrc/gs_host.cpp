@@ -2608,10 +2608,6 @@ int32_t maxmin_order_gpu(gs_ctx *ctx, int D, const double *coords, int64_t n,
         enqueued += run;
         HIP_TRY(hipMemcpyAsync(&st, d_st.p, sizeof(st), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if (std::getenv("GS_MAXMIN_TRACE"))
-            std::fprintf(stderr, "maxmin rounds %lld retries %lld sub %lld s %lld batch %d\n",
-                         (long long)st.rounds, (long long)st.retries, (long long)st.subrounds,
-                         (long long)st.s, st.batch);
         if (st.done) break;
         const double rate = (double)(st.s - prev_s) / (double)std::max<int64_t>(1, st.rounds - prev_rounds);
         // early rounds append few points whatever the input: judge from 256 rounds on

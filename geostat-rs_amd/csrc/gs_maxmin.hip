@@ -155,21 +155,6 @@ __global__ __launch_bounds__(256) void maxmin_collect_kernel(MaxminParams P) {
     flush_hist(lh, P.hist);
 }
 
-// Inclusive scan over kBatchThreads int64 values in LDS (one per thread).
-__device__ int64_t block_scan_incl(int64_t v, int64_t *buf) {
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kBatchThreads; off <<= 1) {
-        const int64_t add = threadIdx.x >= (unsigned)off ? buf[threadIdx.x - off] : 0;
-        __syncthreads();
-        buf[threadIdx.x] += add;
-        __syncthreads();
-    }
-    const int64_t r = buf[threadIdx.x];
-    __syncthreads();
-    return r;
-}
-
 // Candidate id a ranks above id b: (key, index) descending; the id
 // kNoId (padding) ranks below everything.
 __device__ __forceinline__ bool ranks_above(const unsigned long long *ck, const uint32_t *ci,
@@ -197,34 +182,107 @@ __device__ void sort_ids(uint16_t *ord, const unsigned long long *ck, const uint
     }
 }
 
-// Exclusive scan of one int per thread over the workgroup (wavefront
+// Exclusive scan of one value per thread over the workgroup (wavefront
 // shuffles, then the wavefront totals); *total gets the sum.  ws holds
-// kBatchThreads / 64 ints of LDS.
-__device__ int block_excl_scan(int v, int *ws, int *total) {
+// kBatchThreads / 64 values of LDS.
+template <typename T>
+__device__ T block_excl_scan(T v, T *ws, T *total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
+    T x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
+        const T y = __shfl_up(x, o);
         if (lane >= o) x += y;
     }
     if (lane == 63) ws[w] = x;
     __syncthreads();
     if (w == 0) {
         constexpr int nw = kBatchThreads / 64;
-        int t = lane < nw ? ws[lane] : 0;
+        T t = lane < nw ? ws[lane] : 0;
 #pragma unroll
         for (int o = 1; o < nw; o <<= 1) {
-            const int y = __shfl_up(t, o);
+            const T y = __shfl_up(t, o);
             if (lane >= o) t += y;
         }
         if (lane < nw) ws[lane] = t;
     }
     __syncthreads();
-    const int r = x - v + (w > 0 ? ws[w - 1] : 0);
+    const T r = x - v + (w > 0 ? ws[w - 1] : 0);
     *total = ws[kBatchThreads / 64 - 1];
     __syncthreads();
     return r;
+}
+
+__device__ __forceinline__ bool el_gt(unsigned long long ka, uint32_t ia, unsigned long long kb, uint32_t ib) {
+    return ka > kb || (ka == kb && ia > ib);
+}
+
+// Descending bitonic sort of 2 * kBatchThreads (key, index, id) elements
+// held two per thread (positions 2 tid, 2 tid + 1): stages whose partner is
+// in the same wavefront exchange through shuffles, the others through LDS
+// (xk, xi, xid: 2 * kBatchThreads entries each).  Equal elements (padding)
+// may be exchanged freely.
+__device__ void sort_regs(unsigned long long (&k)[2], uint32_t (&i)[2], uint32_t (&id)[2],
+                          unsigned long long *xk, uint32_t *xi, uint32_t *xid) {
+    const int tid = threadIdx.x;
+    constexpr int N = 2 * kBatchThreads;
+    for (int kk = 2; kk <= N; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            if (j == 1) {
+                const bool desc = ((2 * tid) & kk) == 0;
+                if (desc == el_gt(k[1], i[1], k[0], i[0])) {
+                    const unsigned long long tk = k[0];
+                    k[0] = k[1];
+                    k[1] = tk;
+                    const uint32_t t1 = i[0];
+                    i[0] = i[1];
+                    i[1] = t1;
+                    const uint32_t t2 = id[0];
+                    id[0] = id[1];
+                    id[1] = t2;
+                }
+                continue;
+            }
+            unsigned long long ok[2];
+            uint32_t oi[2], oid[2];
+            if (j < 128) {
+                const int m = j >> 1;
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    ok[b] = __shfl_xor(k[b], m);
+                    oi[b] = __shfl_xor(i[b], m);
+                    oid[b] = __shfl_xor(id[b], m);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    xk[2 * tid + b] = k[b];
+                    xi[2 * tid + b] = i[b];
+                    xid[2 * tid + b] = id[b];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const int q = (2 * tid + b) ^ j;
+                    ok[b] = xk[q];
+                    oi[b] = xi[q];
+                    oid[b] = xid[q];
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const int pos = 2 * tid + b;
+                const bool keep_greater = ((pos & j) == 0) == ((pos & kk) == 0);
+                const bool take = keep_greater ? el_gt(ok[b], oi[b], k[b], i[b]) : el_gt(k[b], i[b], ok[b], oi[b]);
+                if (take) {
+                    k[b] = ok[b];
+                    i[b] = oi[b];
+                    id[b] = oid[b];
+                }
+            }
+        }
+    }
 }
 
 // Spatial hash of the candidates: cells of side h, more than the square root
@@ -259,10 +317,10 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
     __shared__ uint16_t bsel[kCap];           // appended ids in order
     __shared__ uint16_t tmpu[kCap], tmpg[kCap];
     __shared__ int hcnt[kHash];               // bucket ends, then starts
-    __shared__ uint16_t hlist[kCap];
     __shared__ int lhist[kBins];
-    __shared__ int64_t sbuf[kBatchThreads];
     __shared__ int wsum[kBatchThreads / 64];
+    __shared__ int64_t wsum64[kBatchThreads / 64];
+    __shared__ uint8_t chg[kCap];              // pending key lowered this sub-round
     __shared__ int sh_b, sh_k1, sh_k2, sh_kf;
     MaxminState *st = P.st;
     const int tid = threadIdx.x;
@@ -273,18 +331,39 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
     const int64_t s0 = st->s;
     const int cap = P.cap;
     const int cl = C < (uint32_t)cap ? (int)C : cap;
-    int n2 = 1;
-    while (n2 < cl) n2 <<= 1;
-    for (int j = tid; j < cl; j += kBatchThreads) {
-        const uint32_t slot = P.cand_slot[j];
-        ck[j] = P.cand_enc[j];
-        cs[j] = slot;
-        ci[j] = P.id[slot];
+    {
+        // candidates in registers, two per thread, sorted; the id of a
+        // candidate is its collection position until the hash reorders them
+        unsigned long long rk[2];
+        uint32_t ri[2], rid[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int j = 2 * tid + e;
+            if (j < cl) {
+                const uint32_t slot = P.cand_slot[j];
+                rk[e] = P.cand_enc[j];
+                ri[e] = P.id[slot];
+                rid[e] = (uint32_t)j;
+                ck[j] = rk[e];
+                cs[j] = slot;
+                ci[j] = ri[e];
+            } else {
+                rk[e] = 0;   // below every candidate (their enc is >= 1)
+                ri[e] = 0;
+                rid[e] = kNoId;
+            }
+        }
+        for (int i = tid; i < kBins; i += kBatchThreads) lhist[i] = P.hist[i];
+        if (cl > 1) {
+            unsigned long long *xk = reinterpret_cast<unsigned long long *>(cx);
+            uint32_t *xi = reinterpret_cast<uint32_t *>(xk + kCap);
+            uint32_t *xid = xi + kCap;
+            sort_regs(rk, ri, rid, xk, xi, xid);
+        }
+#pragma unroll
+        for (int e = 0; e < 2; e++) ord[2 * tid + e] = (uint16_t)rid[e];
+        __syncthreads();
     }
-    for (int j = tid; j < n2; j += kBatchThreads) ord[j] = j < cl ? (uint16_t)j : kNoId;
-    for (int i = tid; i < kBins; i += kBatchThreads) lhist[i] = P.hist[i];
-    __syncthreads();
-    if (cl > 1) sort_ids(ord, ck, ci, n2);
     if (C > (uint32_t)cap) {
         // too many candidates: nothing is appended; the median of the kept
         // ones becomes the threshold (strictly above the old one, since the
@@ -323,10 +402,8 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
             const int64_t c = sat_i64((x - P.origin[d]) * inv_h);
             cc3[j * D + d] = (int)(c < 0 ? 0 : (c > 0x7FFFFFF0 ? 0x7FFFFFF0 : c));
         }
-        stamp[j] = -1;
     }
     __syncthreads();
-    for (int p = tid; p < cl; p += kBatchThreads) rnk[ord[p]] = (int16_t)p;
     for (int j = tid; j < cl; j += kBatchThreads)
         atomicAdd(&hcnt[wrap_bucket<D>(cc3[j * D], cc3[j * D + 1], D == 3 ? cc3[j * D + 2] : 0)], 1);
     __syncthreads();
@@ -342,11 +419,47 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
         for (int e = 0; e < per; e++) { run += v[e]; hcnt[tid * per + e] = run; }
     }
     __syncthreads();
-    for (int j = tid; j < cl; j += kBatchThreads) {
-        const int b = wrap_bucket<D>(cc3[j * D], cc3[j * D + 1], D == 3 ? cc3[j * D + 2] : 0);
-        hlist[atomicSub(&hcnt[b], 1) - 1] = (uint16_t)j;
+    // candidates move to bucket order: the id of a candidate is from here on
+    // its position in the hash, so a bucket is a contiguous id range
+    {
+        int pos[2] = {0, 0};
+        unsigned long long k2[2];
+        uint32_t i2[2], s2[2];
+        double x2[2][D];
+        int c2[2][D];
+        for (int e = 0; e < 2; e++) {
+            const int j = tid * 2 + e;
+            if (j >= cl) break;
+            const int b = wrap_bucket<D>(cc3[j * D], cc3[j * D + 1], D == 3 ? cc3[j * D + 2] : 0);
+            pos[e] = atomicSub(&hcnt[b], 1) - 1;
+            k2[e] = ck[j];
+            i2[e] = ci[j];
+            s2[e] = cs[j];
+#pragma unroll
+            for (int d = 0; d < D; d++) { x2[e][d] = cx[j * D + d]; c2[e][d] = cc3[j * D + d]; }
+        }
+        __syncthreads();
+        for (int e = 0; e < 2; e++) {
+            const int j = tid * 2 + e;
+            if (j >= cl) break;
+            const int q = pos[e];
+            ck[q] = k2[e];
+            ci[q] = i2[e];
+            cs[q] = s2[e];
+#pragma unroll
+            for (int d = 0; d < D; d++) { cx[q * D + d] = x2[e][d]; cc3[q * D + d] = c2[e][d]; }
+            tmpu[j] = (uint16_t)q;     // old id -> new id
+        }
+        __syncthreads();
+        for (int p = tid; p < cl; p += kBatchThreads) {
+            const uint16_t q = tmpu[ord[p]];
+            ord[p] = q;
+            rnk[q] = (int16_t)p;
+            stamp[q] = -1;
+            chg[q] = 0;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // visits the ids hashed near candidate j (its 3^D neighbour cells)
     auto near = [&](int j, auto &&fn) {
         const int c0 = cc3[j * D], c1 = cc3[j * D + 1], c2 = D == 3 ? cc3[j * D + 2] : 0;
@@ -355,25 +468,65 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
                 for (int dx = -1; dx <= 1; dx++) {
                     const int b = wrap_bucket<D>(c0 + dx, c1 + dy, c2 + dz);
                     const int e = b + 1 < kHash ? hcnt[b + 1] : cl;
-                    for (int q = hcnt[b]; q < e; q++) fn((int)hlist[q]);
+                    for (int q = hcnt[b]; q < e; q++) fn(q);
                 }
     };
+    // dense when the candidates crowd few cells (large keys against the
+    // spread of the candidates): the neighbour lists are then most of the
+    // candidates, and direct scans over the ranks are cheaper
+    bool dense;
+    {
+        int nearcnt = 0;
+        for (int e = 0; e < 2; e++) {
+            const int j = 2 * tid + e;
+            if (j >= cl) break;
+            near(j, [&](int) { nearcnt++; });
+        }
+        int all;
+        (void)block_excl_scan(nearcnt, wsum, &all);
+        dense = all > 48 * cl;
+    }
     int np = cl, nb = 0, nsub = 0;
     for (int t = 0; np > 0; t++) {
         nsub = t + 1;
         if (tid == 0) sh_b = np;
         __syncthreads();
-        // first rank with a closer pending candidate ranked before it
-        for (int p = tid + 1; p < np; p += kBatchThreads) {
-            const int j = ord[p];
-            const double kj = bits_key(ck[j] - 1);
-            bool conf = false;
-            near(j, [&](int i) {
-                if (!conf && stamp[i] == -1 && rnk[i] < p && sqd<D>(cx + j * D, cx + i * D) < kj) conf = true;
-            });
-            if (conf) atomicMin(&sh_b, p);
+        // first rank with a closer pending candidate ranked before it, a
+        // chunk of ranks at a time (dense: 64 ranks, 16 threads per rank
+        // scanning the ranks before it)
+        for (int p0 = 1; dense && p0 < np; p0 += kBatchThreads / 16) {
+            const int p = p0 + (tid >> 4);
+            if (p < np) {
+                const int j = ord[p];
+                const double kj = bits_key(ck[j] - 1);
+                for (int r = tid & 15; r < p; r += 16) {
+                    if (sqd<D>(cx + j * D, cx + ord[r] * D) < kj) {
+                        atomicMin(&sh_b, p);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            const int bnow = sh_b;
+            __syncthreads();
+            if (bnow < np) break;
         }
-        __syncthreads();
+        for (int p0 = 1; !dense && p0 < np; p0 += kBatchThreads) {
+            const int p = p0 + tid;
+            if (p < np) {
+                const int j = ord[p];
+                const double kj = bits_key(ck[j] - 1);
+                bool conf = false;
+                near(j, [&](int i) {
+                    if (!conf && stamp[i] == -1 && rnk[i] < p && sqd<D>(cx + j * D, cx + i * D) < kj) conf = true;
+                });
+                if (conf) atomicMin(&sh_b, p);
+            }
+            __syncthreads();
+            const int bnow = sh_b;
+            __syncthreads();
+            if (bnow < np) break;
+        }
         const int bb = sh_b;
         for (int p = tid; p < bb; p += kBatchThreads) {
             const int j = ord[p];
@@ -382,9 +535,26 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
         }
         nb += bb;
         __syncthreads();
-        // lower the other pending keys by the appended points and sort them
-        // into unchanged (still in rank order), lowered and dropped (below
-        // the threshold); positions bb + 2 tid, bb + 2 tid + 1 per thread
+        // the appended points lower the keys of the pending candidates near
+        // them: pushed through the hash, or (few appended points, or dense)
+        // pulled by each pending candidate from the appended list below
+        const bool pull = dense || bb <= 8;
+        for (int p = tid; !pull && p < bb; p += kBatchThreads) {
+            const int a = ord[p];
+            near(a, [&](int q) {
+                if (stamp[q] == -1) {
+                    const unsigned long long ne = key_bits(sqd<D>(cx + q * D, cx + a * D)) + 1;
+                    if (ne < ck[q]) {
+                        atomicMin(&ck[q], ne);
+                        chg[q] = 1;
+                    }
+                }
+            });
+        }
+        __syncthreads();
+        // the other pending candidates: unchanged (still in rank order),
+        // lowered, or dropped (below the threshold); positions bb + 2 tid,
+        // bb + 2 tid + 1 per thread
         uint16_t keep_u[2], keep_g[2];
         int nu = 0, ng = 0;
 #pragma unroll
@@ -392,24 +562,25 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
             const int p = bb + 2 * tid + e;
             if (p >= np) break;
             const int j = ord[p];
-            const unsigned long long e0 = ck[j];
-            unsigned long long en = e0;
-            near(j, [&](int i) {
-                if (stamp[i] == t) {
-                    const unsigned long long ne = key_bits(sqd<D>(cx + j * D, cx + i * D)) + 1;
+            if (pull) {
+                unsigned long long en = ck[j];
+                for (int r = 0; r < bb; r++) {
+                    const unsigned long long ne = key_bits(sqd<D>(cx + j * D, cx + ord[r] * D)) + 1;
                     en = ne < en ? ne : en;
                 }
-            });
-            const unsigned long long kb = en - 1;
-            if (!(kb > tk || (kb == tk && ci[j] >= ti))) {
-                stamp[j] = -2;
-                ck[j] = en;
-            } else if (en == e0) {
-                keep_u[nu++] = (uint16_t)j;
-            } else {
-                ck[j] = en;
-                keep_g[ng++] = (uint16_t)j;
+                if (en < ck[j]) {
+                    ck[j] = en;
+                    chg[j] = 1;
+                }
             }
+            if (!chg[j]) {
+                keep_u[nu++] = (uint16_t)j;
+                continue;
+            }
+            chg[j] = 0;
+            const unsigned long long kb = ck[j] - 1;
+            if (!(kb > tk || (kb == tk && ci[j] >= ti))) stamp[j] = -2;
+            else keep_g[ng++] = (uint16_t)j;
         }
         int mu, mg;
         const int pu = block_excl_scan(nu, wsum, &mu);
@@ -483,17 +654,10 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
     // exclusive scan of the rows over the batch (two entries per thread)
     int64_t total_rows;
     {
-        const int64_t incl = block_scan_incl(rows2[0] + rows2[1], sbuf);
-        const int64_t excl = incl - rows2[0] - rows2[1];
+        const int64_t excl = block_excl_scan<int64_t>(rows2[0] + rows2[1], wsum64, &total_rows);
         if (2 * tid < b) P.brow[2 * tid] = excl;
         if (2 * tid + 1 < b) P.brow[2 * tid + 1] = excl + rows2[0];
-        if (tid == kBatchThreads - 1) {
-            P.brow[b] = incl;   // total
-            sbuf[0] = incl;
-        }
-        __syncthreads();
-        total_rows = sbuf[0];
-        __syncthreads();
+        if (tid == 0) P.brow[b] = total_rows;
     }
     // next threshold: histogram bins from the top down until the capacity
     // is covered (the counts bound the next round's candidates from above)
@@ -504,7 +668,8 @@ __global__ __launch_bounds__(kBatchThreads) void maxmin_batch_kernel(MaxminParam
         sh_k2 = kBins;
         sh_kf = kBins;
     }
-    const int64_t cum = block_scan_incl(hv, sbuf);   // bins 0..tid
+    int64_t hall;
+    const int64_t cum = block_excl_scan<int64_t>(hv, wsum64, &hall) + hv;   // bins 0..tid
     if (tid < kBins) {
         if (cum >= target) atomicMin(&sh_k1, tid);
         if (cum > cap) atomicMin(&sh_k2, tid);
@@ -585,8 +750,9 @@ hipError_t launch_maxmin_init(int D, const MaxminParams &p, hipStream_t s) {
 hipError_t launch_maxmin_round(int D, const MaxminParams &p, hipStream_t s) {
     if (D != 2 && D != 3) return hipErrorInvalidValue;
     if (p.cap < 2 || p.cap > kCap || (p.cap & (p.cap - 1)) != 0) return hipErrorInvalidValue;
-    const int64_t want = (p.n + 255) / 256;
-    const unsigned cblocks = (unsigned)(want < 2048 ? (want > 0 ? want : 1) : 2048);
+    // a few points per thread: fewer blocks flush fewer histogram atomics
+    const int64_t want = (p.n + 1023) / 1024;
+    const unsigned cblocks = (unsigned)(want < 512 ? (want > 0 ? want : 1) : 512);
     if (D == 2) hipLaunchKernelGGL((maxmin_update_kernel<2>), dim3(1024), dim3(256), 0, s, p);
     else hipLaunchKernelGGL((maxmin_update_kernel<3>), dim3(1024), dim3(256), 0, s, p);
     hipLaunchKernelGGL(maxmin_collect_kernel, dim3(cblocks), dim3(256), 0, s, p);
