@@ -2614,7 +2614,11 @@ int32_t maxmin_order_gpu(gs_ctx *ctx, int D, const double *coords, int64_t n,
         if (st.rounds >= 256 && rate < kMinRate) break;
         prev_s = st.s;
         prev_rounds = st.rounds;
-        chunk = std::min<int64_t>(chunk * 2, 1024);
+        // enough rounds for the rest at the last chunk's rate (rates grow
+        // along the traversal), so that few rounds are launched after the
+        // end (they return at once, but each still costs its launches)
+        const double need = 1.25 * (double)(n - st.s) / std::max(rate, 1.0) + 4.0;
+        chunk = std::min<int64_t>(512, std::max<int64_t>(16, (int64_t)need));
     }
     if (info) { info[0] = st.rounds; info[1] = st.retries; info[2] = st.s; }
     order.resize((size_t)st.s);
