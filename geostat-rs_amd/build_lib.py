@@ -203,6 +203,61 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
     return LIB
 
 
+# Host-sanitized variant (build/asan/libgeostat_hip.so): the host runtime
+# (gs_host.cpp, gs_mle.cpp: validation, search indexes, maxmin, plans,
+# staging, request loop) built with AddressSanitizer and UBSan for the host
+# side only; the kernels are the regular objects.  Loaded with the clang ASan
+# runtime preloaded (tests/test_host_sanitized.py; GEOSTAT_HIP_LIB selects it).
+ASAN = BUILD / "asan"
+ASAN_LIB = ASAN / "libgeostat_hip.so"
+HOST_SAN = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=all"]
+HOST_UNITS = ("gs_host.cpp", "gs_mle.cpp")
+
+
+def asan_runtime() -> Path:
+    """The clang ASan runtime that must be preloaded with the sanitized library."""
+    out = subprocess.run([str(Path(HIPCC).parent.parent / "lib" / "llvm" / "bin" / "clang"),
+                          "-print-file-name=libclang_rt.asan-x86_64.so"],
+                         capture_output=True, text=True).stdout.strip()
+    p = Path(out)
+    if not p.is_absolute() or not p.exists():
+        hits = sorted(Path("/opt/rocm/lib/llvm/lib/clang").glob("*/lib/linux/libclang_rt.asan-x86_64.so"))
+        if not hits:
+            raise RuntimeError("clang ASan runtime not found")
+        p = hits[-1]
+    return p
+
+
+def build_sanitized(verbose: bool = False) -> Path:
+    """Links build/asan/libgeostat_hip.so from the regular kernel objects and
+    host-sanitized gs_host / gs_mle objects (build() first)."""
+    (ASAN / "obj").mkdir(parents=True, exist_ok=True)
+    objs = []
+    for o in sorted(OBJ.glob("*.o")):
+        if o.stem + ".cpp" in HOST_UNITS:
+            continue
+        objs.append(str(o))
+    for unit in HOST_UNITS:
+        src = CSRC / unit
+        obj = ASAN / "obj" / (src.stem + ".o")
+        cmd = [HIPCC, *COMMON, *EXACT, *HOST_SAN, "-g", "-c", str(src), "-o", str(obj)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"  built {unit} (host sanitizers)", flush=True)
+        objs.append(str(obj))
+    tmp = ASAN_LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *HOST_SAN, "-shared-libsan",
+           "-o", str(tmp), *objs, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ASAN_LIB)
+    return ASAN_LIB
+
+
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
 
@@ -224,3 +279,5 @@ if __name__ == "__main__":
     if "--clean" in sys.argv:
         clean()
     print(build(incremental="--incremental" in sys.argv))
+    if "--sanitized" in sys.argv:
+        print(build_sanitized(verbose=True))

@@ -10,7 +10,9 @@ import os
 from pathlib import Path
 
 PKG_ROOT = Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_ROOT / "build" / "libgeostat_hip.so"
+# GEOSTAT_HIP_LIB selects another build of the same library (the
+# host-sanitized one in build/asan/, used by the sanitizer tests)
+LIB_PATH = Path(os.environ.get("GEOSTAT_HIP_LIB") or PKG_ROOT / "build" / "libgeostat_hip.so")
 SYNTH_PATH = PKG_ROOT / "build" / "libgs_synth.so"
 
 GS_NONE = -1
