@@ -55,6 +55,9 @@ VECCHIA = [(d, nb, g, lean) for d in (2, 3) for nb in (16, 32, 64) for lean in (
 # (ordinary); Vecchia conditionals.
 KRIGE32R = [(d, nb, m) for d in (2, 3) for nb in (16, 32, 64) for m in (0, 1)]
 VECCHIA32R = [(d, nb) for d in (2, 3) for nb in (16, 32, 64)]
+# Vecchia likelihood on the matrix cores: single lean structure, by
+# (dimension, anisotropy, kind 0..4)
+VMFMA = [(d, an, k) for d in (2, 3) for an in (0, 1) for k in range(5)]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
 # two row blocks per lane
 VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]
@@ -78,6 +81,15 @@ def _gen_sources() -> list[Path]:
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
                f"namespace gs {{\ntemplate hipError_t launch_vecchia_inst<{d}, {nb}, {g}, "
                f"{'true' if lean else 'false'}>"
+               f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
+    for d, an, k in VMFMA:
+        p = GEN / f"vmfma_d{d}_a{an}_k{k}.hip"
+        txt = (f'#include "{CSRC}/gs_vecchia_mfma_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_vecchia_mfma_inst<{d}, "
+               f"{'true' if an else 'false'}, {k}>"
                f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
@@ -114,7 +126,8 @@ def _deps_mtime(src: Path) -> float:
     only the kernel templates; every other unit may include any header."""
     kernel = ["gs_solve_impl.hpp", "gs_internal.hpp", "gs_device.hpp"]
     if src.parent == GEN:
-        names = kernel + (["gs_fp32r_impl.hpp"] if "32r" in src.stem else [])
+        names = kernel + (["gs_fp32r_impl.hpp"] if "32r" in src.stem else []) + \
+            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith("vmfma") else [])
         hdrs = [CSRC / n for n in names]
     else:
         hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".inc")]
@@ -175,6 +188,7 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
         (CSRC / "gs_big.hip", EXACT),
         (CSRC / "gs_plan.hip", EXACT),
         (CSRC / "gs_maxmin.hip", EXACT),
+        (CSRC / "gs_vecchia_mfma.hip", []),
     ]
     units += [(p, []) for p in _gen_sources()]
     jobs = jobs or max(1, min(16, os.cpu_count() or 4))

@@ -27,6 +27,7 @@
 #include "../../include/geostat_hip.h"
 #include "gs_internal.hpp"
 #include "gs_maxmin.hpp"
+#include "gs_vecchia_mfma.hpp"
 
 using gs::DevModel;
 
@@ -2949,7 +2950,16 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
             HIP_TRY(hipStreamSynchronize(s));
             continue;
         }
-        hipError_t e = gs::launch_vecchia(p->D, NB, variant, lean, vp, n_blocks, s);
+        // 17-32-point conditioning sets with one lean structure of one kind
+        // and anisotropy flag across the batch: factorization on the matrix
+        // cores (GS_VECCHIA_MFMA=0 keeps the vector-unit kernel, for comparison)
+        bool mf_aniso = false;
+        const char *mf_env = std::getenv("GS_VECCHIA_MFMA");
+        const int mf_kind = (NB == 32 && !(mf_env && mf_env[0] == '0'))
+                                ? gs::vecchia_mfma_kind(dm.data() + b0, nm, p->max_count, &mf_aniso)
+                                : -1;
+        hipError_t e = mf_kind >= 0 ? gs::launch_vecchia_mfma(p->D, mf_aniso, mf_kind, vp, n_blocks, s)
+                                    : gs::launch_vecchia(p->D, NB, variant, lean, vp, n_blocks, s);
         if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
         HIP_TRY(e);
         HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm, p->out.as<double>() + b0, s));

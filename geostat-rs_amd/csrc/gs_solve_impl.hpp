@@ -230,6 +230,7 @@ struct LeanRegs {
     double ir2;         // 1 / range^2 (isotropic structures)
     double sill, alpha, beta, gam;
     bool aniso, sph, gau;
+    int kind;
 };
 
 template <int D>
@@ -241,6 +242,7 @@ __device__ __forceinline__ LeanRegs<D> load_lean(const DevStructure &S) {
     L.ir2 = ir * ir;
     L.sill = S.sill;
     L.aniso = S.aniso != 0;
+    L.kind = S.kind;
     L.sph = S.kind == K_SPH;
     L.gau = S.kind == K_GAU;
     L.alpha = S.kind == K_MAT15 ? 1.7320508075688772 : (S.kind == K_MAT25 ? 2.23606797749979 : 1.0);
@@ -264,7 +266,9 @@ __device__ __forceinline__ double sqrt_lean(double q) {
 // directly, without the cancellation of 1 - g at long range.
 // TAB: exp by the LDS table (the kernel ran load_exp_table); otherwise by
 // exp_neg, for kernels that keep their LDS and registers for other work.
-template <int D, bool ANISO, bool SPH, bool TAB = true>
+// FAM compiles the kind in (1 Matern 3/2, 2 Matern 5/2, 3 exponential,
+// 4 Gaussian; 0 reads it from L): the same values, fewer instructions.
+template <int D, bool ANISO, bool SPH, bool TAB = true, int FAM = 0>
 __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *dh) {
     double h2s = 0.0;
     if constexpr (ANISO) {
@@ -285,9 +289,24 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
     if constexpr (SPH) {
         c = d >= 1.0 ? 0.0 : 1.0 - (1.5 * d - 0.5 * d * d * d);
     } else {
-        const double q = L.alpha * d;
-        const double p = fma(fma(L.gam, q, L.beta), q, 1.0);
-        const double x = L.gau ? h2s : q;
+        double p, x;
+        if constexpr (FAM == 1) {
+            x = 1.7320508075688772 * d;
+            p = 1.0 + x;
+        } else if constexpr (FAM == 2) {
+            x = 2.23606797749979 * d;
+            p = fma(fma(1.0 / 3.0, x, 1.0), x, 1.0);
+        } else if constexpr (FAM == 3) {
+            x = d;
+            p = 1.0;
+        } else if constexpr (FAM == 4) {
+            x = h2s;
+            p = 1.0;
+        } else {
+            const double q = L.alpha * d;
+            p = fma(fma(L.gam, q, L.beta), q, 1.0);
+            x = L.gau ? h2s : q;
+        }
         c = p * (TAB ? exp_neg_t(x) : exp_neg(x));
     }
     return L.sill * c;
@@ -298,21 +317,21 @@ __device__ __forceinline__ double lean_term(const LeanRegs<D> &L, const double *
 // entry e >= iA is (row NB-1-iA, column NB-2-e), so every register of the
 // pair has a compile-time entry number (with one block, entries past iA
 // are unused).  FIRST writes, otherwise the slot accumulates.
-template <int D, int NB, bool ANISO, bool SPH, bool FIRST>
+template <int D, int NB, bool ANISO, bool SPH, bool FIRST, int FAM = 0>
 __device__ __forceinline__ void lean_slots(const LeanRegs<D> &L, const int e0, const int e1,
                                            const int iA, const double (&xA)[D],
                                            const double (&xB)[D], const double *xy,
                                            double *stage, const int lane) {
     // one entry per iteration: unrolled, the compiler interleaves the
     // entries' exp chains and runs out of registers
-#pragma unroll 1
+#pragma unroll 2
     for (int e = e0; e < e1; e++) {
         const bool inA = e < iA;
         const int c = inA ? e : NB - 2 - e;
         double dh[D];
 #pragma unroll
         for (int d = 0; d < D; d++) dh[d] = (inA ? xA[d] : xB[d]) - xy[c * D + d];
-        const double v = lean_term<D, ANISO, SPH>(L, dh);
+        const double v = lean_term<D, ANISO, SPH, true, FAM>(L, dh);
         double *slot = stage + (e - e0) * kWave + lane;
         *slot = FIRST ? v : *slot + v;
     }
@@ -339,22 +358,30 @@ __device__ __forceinline__ void staged_entries(const DevModel &Mdl, const int e0
     for (int s = 0; s < ns; s++) {
         if constexpr (LEAN) {
             const LeanRegs<D> L = load_lean<D>(Mdl.s[s]);
-#define GS_LEAN_CASE(AN, SP)                                                                    \
+#define GS_LEAN_CASE(AN, SP, FM)                                                                \
     do {                                                                                        \
-        if (s == 0) lean_slots<D, NB, AN, SP, true>(L, e0, e1, iA, xA, xB, xy, stage, lane);    \
-        else lean_slots<D, NB, AN, SP, false>(L, e0, e1, iA, xA, xB, xy, stage, lane);          \
+        if (s == 0) lean_slots<D, NB, AN, SP, true, FM>(L, e0, e1, iA, xA, xB, xy, stage, lane); \
+        else lean_slots<D, NB, AN, SP, false, FM>(L, e0, e1, iA, xA, xB, xy, stage, lane);      \
         if constexpr (TGT) {                                                                    \
-            tA += lean_term<D, AN, SP>(L, dA);                                                  \
-            tB += lean_term<D, AN, SP>(L, dB);                                                  \
+            tA += lean_term<D, AN, SP, true, FM>(L, dA);                                        \
+            tB += lean_term<D, AN, SP, true, FM>(L, dB);                                        \
         }                                                                                       \
     } while (0)
+#define GS_LEAN_KIND(AN)                                                                        \
+    do {                                                                                        \
+        if (L.kind == K_MAT15) GS_LEAN_CASE(AN, false, 1);                                      \
+        else if (L.kind == K_MAT25) GS_LEAN_CASE(AN, false, 2);                                 \
+        else if (L.kind == K_EXP) GS_LEAN_CASE(AN, false, 3);                                   \
+        else GS_LEAN_CASE(AN, false, 4);                                                        \
+    } while (0)
             if (L.aniso) {
-                if (L.sph) GS_LEAN_CASE(true, true);
-                else GS_LEAN_CASE(true, false);
+                if (L.sph) GS_LEAN_CASE(true, true, 0);
+                else GS_LEAN_KIND(true);
             } else {
-                if (L.sph) GS_LEAN_CASE(false, true);
-                else GS_LEAN_CASE(false, false);
+                if (L.sph) GS_LEAN_CASE(false, true, 0);
+                else GS_LEAN_KIND(false);
             }
+#undef GS_LEAN_KIND
 #undef GS_LEAN_CASE
         } else {
             const DevStructure *S = &Mdl.s[s];

@@ -1,0 +1,58 @@
+// Dispatch of the matrix-core Vecchia kernels (instantiated one per
+// generated unit from gs_vecchia_mfma_impl.hpp).
+#include "gs_vecchia_mfma.hpp"
+
+namespace gs {
+
+template <int D, bool ANISO, int KIND>
+hipError_t launch_vecchia_mfma_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
+
+int vecchia_mfma_kind(const DevModel *models, int n_models, int max_count, bool *aniso) {
+    if (n_models <= 0 || max_count > 32) return -1;
+    int kind = -1;
+    bool an = false;
+    for (int i = 0; i < n_models; i++) {
+        const DevModel &m = models[i];
+        if (m.ns != 1) return -1;
+        int k;
+        switch (m.s[0].kind) {
+        case K_SPH: k = 0; break;
+        case K_MAT15: k = 1; break;
+        case K_MAT25: k = 2; break;
+        case K_EXP: k = 3; break;
+        case K_GAU: k = 4; break;
+        default: return -1;
+        }
+        const bool a = m.s[0].aniso != 0;
+        if (i == 0) {
+            kind = k;
+            an = a;
+        } else if (k != kind || a != an) {
+            return -1;
+        }
+    }
+    *aniso = an;
+    return kind;
+}
+
+template <int D, bool AN>
+static hipError_t by_kind(int kind, const VecchiaParams &p, int64_t nb, hipStream_t s) {
+    switch (kind) {
+    case 0: return launch_vecchia_mfma_inst<D, AN, 0>(p, nb, s);
+    case 1: return launch_vecchia_mfma_inst<D, AN, 1>(p, nb, s);
+    case 2: return launch_vecchia_mfma_inst<D, AN, 2>(p, nb, s);
+    case 3: return launch_vecchia_mfma_inst<D, AN, 3>(p, nb, s);
+    case 4: return launch_vecchia_mfma_inst<D, AN, 4>(p, nb, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_vecchia_mfma(int D, bool aniso, int kind, const VecchiaParams &p,
+                               int64_t n_blocks, hipStream_t s) {
+    if (p.points_per_block % 4 != 0) return hipErrorInvalidValue;
+    if (D == 2) return aniso ? by_kind<2, true>(kind, p, n_blocks, s) : by_kind<2, false>(kind, p, n_blocks, s);
+    if (D == 3) return aniso ? by_kind<3, true>(kind, p, n_blocks, s) : by_kind<3, false>(kind, p, n_blocks, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gs
