@@ -58,6 +58,8 @@ VECCHIA32R = [(d, nb) for d in (2, 3) for nb in (16, 32, 64)]
 # Vecchia likelihood on the matrix cores: single lean structure, by
 # (dimension, anisotropy, kind 0..4)
 VMFMA = [(d, an, k) for d in (2, 3) for an in (0, 1) for k in range(5)]
+# simple / ordinary kriging on the matrix cores: (dimension, anisotropy, kind, M)
+KMFMA = [(d, an, k, m) for d in (2, 3) for an in (0, 1) for k in range(5) for m in (0, 1)]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
 # two row blocks per lane
 VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]
@@ -94,6 +96,15 @@ def _gen_sources() -> list[Path]:
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
         out.append(p)
+    for d, an, k, m in KMFMA:
+        p = GEN / f"kmfma_d{d}_a{an}_k{k}_m{m}.hip"
+        txt = (f'#include "{CSRC}/gs_vecchia_mfma_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_krige_mfma_inst<{d}, "
+               f"{'true' if an else 'false'}, {k}, {m}>"
+               f"(const SolveParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
     for d, nb, g in VGRAD:
         p = GEN / f"vgrad_d{d}_nb{nb}_g{g}.hip"
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
@@ -127,7 +138,7 @@ def _deps_mtime(src: Path) -> float:
     kernel = ["gs_solve_impl.hpp", "gs_internal.hpp", "gs_device.hpp"]
     if src.parent == GEN:
         names = kernel + (["gs_fp32r_impl.hpp"] if "32r" in src.stem else []) + \
-            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith("vmfma") else [])
+            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith(("vmfma", "kmfma")) else [])
         hdrs = [CSRC / n for n in names]
     else:
         hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".inc")]

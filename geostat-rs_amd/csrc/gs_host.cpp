@@ -34,6 +34,8 @@ using gs::DevModel;
 namespace {
 
 thread_local std::string g_err;
+// kernel family of the last solve launched on this thread (gs_last_solve_kernel)
+thread_local const char *g_solve_kernel = "";
 
 int32_t fail(int32_t code, const std::string &msg) {
     g_err = msg;
@@ -472,6 +474,8 @@ struct gs_kriging {
     gs_model model{};
     int M = 1;
     bool lean = true;        // every structure in the fast closed-form class
+    int mfma_kind = -1;      // one lean structure: its kind for the matrix-core solve (-1: none)
+    bool mfma_aniso = false;
     bool has_tree = false;
     int64_t k_heap = 0;
     int per_oct = 0;
@@ -660,6 +664,7 @@ void *scratch_of(int device) {
 extern "C" {
 
 const char *gs_last_error(void) { return g_err.c_str(); }
+const char *gs_last_solve_kernel(void) { return g_solve_kernel; }
 int32_t gs_abi_version(void) { return GS_ABI_VERSION; }
 
 int32_t gs_model_validate(const gs_model *model) { return validate_model(model, 2, true); }
@@ -903,6 +908,7 @@ int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const doub
     }
     DevModel dm = resolve_model(*model, dim, ctx->nodes.as<gs::BesselNodes>());
     K->lean = gs::model_is_lean(dm);
+    K->mfma_kind = gs::vecchia_mfma_kind(&dm, 1, 32, &K->mfma_aniso);
     HIP_TRY(K->dmodel.ensure(sizeof(DevModel)));
     HIP_TRY(hipMemcpy(K->dmodel.p, &dm, sizeof(DevModel), hipMemcpyHostToDevice));
     if (K->has_tree) {
@@ -1303,8 +1309,20 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     const int cus = device_cus(K->ctx->device);
     // persistent = false: one group per system; otherwise a persistent grid
     // of the table's wavefronts per CU, grid-strided over the list
+    // 17-32-neighbour simple and ordinary kriging with one lean structure:
+    // the factorization on the matrix cores (GS_KRIGE_MFMA=0 keeps the
+    // vector-unit kernel, for comparison)
+    const char *mf_env = std::getenv("GS_KRIGE_MFMA");
+    const bool mfma = K->mfma_kind >= 0 && K->M <= 1 && !ext && !(mf_env && mf_env[0] == '0');
     auto solve64 = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
         const Schedule sc = schedule_for("krige", K->D, NB, K->M, K->lean, ext);
+        if (mfma && NB == 32) {
+            int64_t groups = pp.T;
+            if (persistent) groups = std::min<int64_t>(pp.T, (int64_t)cus * sc.waves_per_cu * 4);
+            HIP_TRY(gs::launch_krige_mfma(K->D, K->mfma_aniso, K->mfma_kind, K->M, pp, groups, s));
+            g_solve_kernel = "krige_mfma_kernel";
+            return GS_OK;
+        }
         const int gpw = 64 / gs::group_size(NB, sc.variant);
         int64_t groups = pp.T;
         if (persistent) groups = std::min<int64_t>(pp.T, (int64_t)cus * sc.waves_per_cu * gpw);
@@ -1312,6 +1330,7 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         if (e == hipErrorInvalidValue)
             return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size");
         HIP_TRY(e);
+        g_solve_kernel = "krige_fp64_kernel";
         return GS_OK;
     };
     auto solve = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
@@ -1324,6 +1343,7 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
         if (e == hipErrorInvalidValue)
             return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: no kernel for this system size");
         HIP_TRY(e);
+        g_solve_kernel = "krige_fp32r_kernel";
         return GS_OK;
     };
     int32_t st = solve_all(K, W, p, T, s, side, solve);
@@ -1333,7 +1353,10 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     gs::SolveParams pf = p;
     pf.list = p.fb_list;
     pf.list_len = p.fb_len;
-    return solve64(pad_nb(K->nb_bound), pf, true);
+    const char *main_kernel = g_solve_kernel;
+    st = solve64(pad_nb(K->nb_bound), pf, true);
+    g_solve_kernel = main_kernel;
+    return st;
 }
 
 // The launch sequence of run_solves for one precision: size buckets for
@@ -1541,6 +1564,7 @@ int32_t run_big(gs_kriging *K, KWork &W, const double *d_targets, int64_t T, dou
         bp.count = W.big_cnt.as<int32_t>();
         bp.scratch = W.big_scratch.as<double>();
         HIP_TRY(gs::launch_big_solve(K->D, K->lean, bp, blocks, s));
+        g_solve_kernel = "big_solve_kernel";
     }
     return GS_OK;
 }
@@ -2909,6 +2933,7 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
             hipError_t e = gs::launch_vecchia_fp32r(p->D, NB, vp, n_blocks, s);
             if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
             HIP_TRY(e);
+            g_solve_kernel = "vecchia_fp32r_kernel";
             HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm, p->out.as<double>() + b0, s));
             std::vector<int32_t> bad((size_t)nm);
             HIP_TRY(hipMemcpyAsync(bad.data(), p->refine_fail.p, sizeof(int32_t) * nm,
@@ -2962,6 +2987,7 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
                                     : gs::launch_vecchia(p->D, NB, variant, lean, vp, n_blocks, s);
         if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
         HIP_TRY(e);
+        g_solve_kernel = mf_kind >= 0 ? "vecchia_mfma_kernel" : "vecchia_kernel";
         HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm, p->out.as<double>() + b0, s));
     }
     std::vector<int32_t> st((size_t)n_models);
@@ -3058,6 +3084,7 @@ int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs
             hipError_t e = gs::launch_vecchia_grad(p->D, NB, vp, n_blocks, s);
             if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
             HIP_TRY(e);
+            g_solve_kernel = "vecchia_grad_kernel";
             HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm * 4,
                                               p->out.as<double>() + (int64_t)b0 * 4, s));
         }

@@ -6,6 +6,8 @@ namespace gs {
 
 template <int D, bool ANISO, int KIND>
 hipError_t launch_vecchia_mfma_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
+template <int D, bool ANISO, int KIND, int M>
+hipError_t launch_krige_mfma_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
 
 int vecchia_mfma_kind(const DevModel *models, int n_models, int max_count, bool *aniso) {
     if (n_models <= 0 || max_count > 32) return -1;
@@ -52,6 +54,32 @@ hipError_t launch_vecchia_mfma(int D, bool aniso, int kind, const VecchiaParams 
     if (p.points_per_block % 4 != 0) return hipErrorInvalidValue;
     if (D == 2) return aniso ? by_kind<2, true>(kind, p, n_blocks, s) : by_kind<2, false>(kind, p, n_blocks, s);
     if (D == 3) return aniso ? by_kind<3, true>(kind, p, n_blocks, s) : by_kind<3, false>(kind, p, n_blocks, s);
+    return hipErrorInvalidValue;
+}
+
+template <int D, bool AN, int M>
+static hipError_t krige_by_kind(int kind, const SolveParams &p, int64_t gg, hipStream_t s) {
+    switch (kind) {
+    case 0: return launch_krige_mfma_inst<D, AN, 0, M>(p, gg, s);
+    case 1: return launch_krige_mfma_inst<D, AN, 1, M>(p, gg, s);
+    case 2: return launch_krige_mfma_inst<D, AN, 2, M>(p, gg, s);
+    case 3: return launch_krige_mfma_inst<D, AN, 3, M>(p, gg, s);
+    case 4: return launch_krige_mfma_inst<D, AN, 4, M>(p, gg, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int D, int M>
+static hipError_t krige_by_aniso(bool aniso, int kind, const SolveParams &p, int64_t gg, hipStream_t s) {
+    return aniso ? krige_by_kind<D, true, M>(kind, p, gg, s) : krige_by_kind<D, false, M>(kind, p, gg, s);
+}
+
+hipError_t launch_krige_mfma(int D, bool aniso, int kind, int M, const SolveParams &p,
+                             int64_t grid_groups, hipStream_t s) {
+    if (D == 2) return M == 0 ? krige_by_aniso<2, 0>(aniso, kind, p, grid_groups, s)
+                              : (M == 1 ? krige_by_aniso<2, 1>(aniso, kind, p, grid_groups, s) : hipErrorInvalidValue);
+    if (D == 3) return M == 0 ? krige_by_aniso<3, 0>(aniso, kind, p, grid_groups, s)
+                              : (M == 1 ? krige_by_aniso<3, 1>(aniso, kind, p, grid_groups, s) : hipErrorInvalidValue);
     return hipErrorInvalidValue;
 }
 

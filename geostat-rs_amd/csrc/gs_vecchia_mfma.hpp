@@ -1,6 +1,6 @@
-// The Vecchia likelihood batch on the matrix cores (gs_vecchia_mfma_impl.hpp),
-// for conditioning sets of at most 32 points and batches whose models are
-// all one lean structure of the same kind and anisotropy flag.
+// The Vecchia likelihood batch and the 17-32-neighbour simple / ordinary
+// kriging solve on the matrix cores (gs_vecchia_mfma_impl.hpp), for models
+// of one lean structure (the same kind and anisotropy flag across a batch).
 #pragma once
 
 #include "gs_internal.hpp"
@@ -13,5 +13,11 @@ int vecchia_mfma_kind(const DevModel *models, int n_models, int max_count, bool 
 
 hipError_t launch_vecchia_mfma(int D, bool aniso, int kind, const VecchiaParams &p,
                                int64_t n_blocks, hipStream_t s);
+
+// Simple (M = 0) or ordinary (M = 1) kriging with 17-32 neighbours and one
+// lean structure on the matrix cores (kind as vecchia_mfma_kind);
+// grid_groups systems in flight, four per wavefront.
+hipError_t launch_krige_mfma(int D, bool aniso, int kind, int M, const SolveParams &p,
+                             int64_t grid_groups, hipStream_t s);
 
 }  // namespace gs

@@ -258,6 +258,198 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     }
 }
 
+// Kriging::predict_inner (kriging.rs:569-644) for simple (M = 0) and
+// ordinary (M = 1) kriging with 17-32 neighbours and a single lean
+// structure, on the matrix cores: the system of vecchia_mfma_kernel with
+// the last tile row holding f (the constant drift column), k (covariances
+// to the target) and z (the values; minus the mean for simple kriging).
+// The elimination leaves the Gram matrix of a = L^-1 f, b = L^-1 k and
+// c = L^-1 z in the bottom-right tile, and the bordered solve of the
+// reference follows from it: for ordinary kriging (one drift term, f(t) =
+// 1) the Lagrange multiplier is (a.b - 1) / a.a, the value b.c - (a.c) mu
+// and the variance c0 - b.b + (a.b - 1)^2 / a.a; for simple kriging the
+// value is mean + b.c and the variance c0 - b.b.  Singular systems as in
+// krige_fp64_kernel: Cholesky pivots below 1e-12 of the bordered matrix's
+// scale, or a drift Schur complement a.a below it (linalg.rs:36-57).
+template <int D, bool ANISO, int KIND, int M>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void krige_mfma_kernel(SolveParams P) {
+    using namespace vm;
+    static_assert(M == 0 || M == 1, "simple or ordinary kriging");
+    constexpr int NP = 32;
+    __shared__ double s_xy[4 * NP * D];
+    __shared__ double s_v[4 * NP];
+    __shared__ double s_t[4 * 16];
+    __shared__ double s_dg[4 * 48];
+    __shared__ double s_k[4 * NP];
+    const int lane = threadIdx.x;
+    const int blk = (lane >> 2) & 3;
+    const int x_ = lane >> 4, y_ = lane & 3;
+    const int g = x_ * 4 + y_;
+    double *xy = s_xy + blk * NP * D;
+    double *vs = s_v + blk * NP;
+    double *tt = s_t + blk * 16;
+    double *dgs = s_dg + blk * 48;
+    double *ks = s_k + blk * NP;
+    const DevModel &Mdl = *P.model;
+    const double c0 = Mdl.c0;
+    const LeanRegs<D> L = load_lean<D>(Mdl.s[0]);
+    const double thresh = 1e-12 * (M >= 1 ? fmax(c0, 1.0) : c0);
+    load_exp_table();
+    wave_sync();
+    const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
+    for (int64_t sb = (int64_t)blockIdx.x * 4; sb < nsys; sb += (int64_t)gridDim.x * 4) {
+        const int64_t sys = sb + blk;
+        const bool live = sys < nsys;
+        const int64_t t = P.list ? (int64_t)P.list[live ? sys : sb] : (live ? sys : sb);
+        const int craw = P.count[t];
+        const bool bad = craw < P.min_nb || craw < M || craw > NP;
+        const int cnt_ = live && !bad ? craw : 0;
+        double tg[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) tg[d] = P.targets[t * D + d];
+        const int32_t *nbl = P.nbr + t * (int64_t)P.stride;
+        for (int j = g; j < NP; j += 16) {
+            const int64_t id = j < cnt_ ? nbl[j] : 0;
+#pragma unroll
+            for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt_ ? P.coords[id * D + d] : 0.0;
+            vs[j] = j < cnt_ ? (M == 0 ? P.values[id] - P.sk_mean : P.values[id]) : 0.0;
+        }
+        wave_sync();
+        const int y = opaque(y_), x = opaque(x_), cnt = opaque(cnt_);
+        double X[kTiles];
+#pragma unroll
+        for (int e3 = 0; e3 < 3; e3++) {
+            const int u = 3 * g + e3;
+            const int tt6 = u / 6, e = u % 6;
+            const int r = e >= 3 ? 3 : (e >= 1 ? 2 : 1);
+            const int c = e - r * (r - 1) / 2;
+            const int a = 4 * tt6 + r, b = 4 * tt6 + c;
+            const double v = pin(cov<D, ANISO, KIND>(L, c0, xy + a * D, xy + b * D));
+            dgs[u] = (a < cnt && b < cnt) ? v : 0.0;
+        }
+#pragma unroll
+        for (int e2 = 0; e2 < 2; e2++) {
+            const int b = 2 * g + e2;
+            const double v = pin(cov<D, ANISO, KIND>(L, c0, tg, xy + b * D));
+            ks[b] = b < cnt ? v : 0.0;
+        }
+        static_for<1, 8>([&](auto II) {
+            constexpr int i = decltype(II)::value;
+            const int a = 4 * i + y;
+            double pa[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) pa[d] = xy[a * D + d];
+            static_for<0, i>([&](auto JJ) {
+                constexpr int j = decltype(JJ)::value;
+                const int b = 4 * j + x;
+                double pb[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) pb[d] = xy[b * D + d];
+                const double v = pin(cov<D, ANISO, KIND>(L, c0, pa, pb));
+                X[tix(i, j)] = (a < cnt && b < cnt) ? v : 0.0;
+                if constexpr ((tix(i, j) % 2) == 1) __builtin_amdgcn_sched_barrier(0);
+            });
+        });
+        wave_sync();
+        {
+            const int hi = y > x ? y : x, lo = y > x ? x : y;
+            const int e = hi * (hi - 1) / 2 + lo;
+            static_for<0, 8>([&](auto TT) {
+                constexpr int q = decltype(TT)::value;
+                X[tix(q, q)] = y == x ? c0 : dgs[6 * q + e];
+            });
+        }
+        static_for<0, 8>([&](auto JJ) {
+            constexpr int j = decltype(JJ)::value;
+            const int b = 4 * j + x;
+            const double fb = (M == 1 && b < cnt) ? 1.0 : 0.0;
+            X[tix(8, j)] = y == 0 ? fb : (y == 1 ? ks[b] : (y == 2 ? vs[b] : 0.0));
+        });
+        X[tix(8, 8)] = 0.0;
+        wave_sync();
+        bool ok = true;
+        static_for<0, 8>([&](auto PP) {
+            constexpr int q = decltype(PP)::value;
+            tt[4 * y + x] = X[tix(q, q)];
+            wave_sync();
+            const double m00 = tt[0];
+            const double m10 = tt[4], m11 = tt[5];
+            const double m20 = tt[8], m21 = tt[9], m22 = tt[10];
+            const double m30 = tt[12], m31 = tt[13], m32 = tt[14], m33 = tt[15];
+            wave_sync();
+            const double d0 = m00;
+            const double i0 = rsqrt_nr(d0);
+            const double l10 = m10 * i0, l20 = m20 * i0, l30 = m30 * i0;
+            const double d1 = m11 - l10 * l10;
+            const double i1 = rsqrt_nr(d1);
+            const double l21 = (m21 - l20 * l10) * i1, l31 = (m31 - l30 * l10) * i1;
+            const double d2 = (m22 - l20 * l20) - l21 * l21;
+            const double i2 = rsqrt_nr(d2);
+            const double l32 = ((m32 - l30 * l20) - l31 * l21) * i2;
+            const double d3 = ((m33 - l30 * l30) - l31 * l31) - l32 * l32;
+            const double i3 = rsqrt_nr(d3);
+            ok = ok && d0 >= thresh && d1 >= thresh && d2 >= thresh && d3 >= thresh;
+            const double v10 = -l10 * i0 * i1;
+            const double v21 = -l21 * i1 * i2;
+            const double v20 = -(l20 * i0 + l21 * v10) * i2;
+            const double v32 = -l32 * i2 * i3;
+            const double v31 = -(l31 * i1 + l32 * v21) * i3;
+            const double v30 = -((l30 * i0 + l31 * v10) + l32 * v20) * i3;
+            double linv = 0.0;
+            if (y == x) linv = y == 0 ? i0 : (y == 1 ? i1 : (y == 2 ? i2 : i3));
+            else if (y == 1 && x == 0) linv = v10;
+            else if (y == 2) linv = x == 0 ? v20 : (x == 1 ? v21 : 0.0);
+            else if (y == 3) linv = x == 0 ? v30 : (x == 1 ? v31 : (x == 2 ? v32 : 0.0));
+            static_for<q + 1, 9>([&](auto II) {
+                constexpr int i = decltype(II)::value;
+                X[tix(i, q)] = mfma(linv, X[tix(i, q)], 0.0);
+            });
+            static_for<q + 1, 9>([&](auto II) {
+                constexpr int i = decltype(II)::value;
+                static_for<q + 1, i + 1>([&](auto JJ) {
+                    constexpr int j = decltype(JJ)::value;
+                    X[tix(i, j)] = mfma(-X[tix(j, q)], X[tix(i, q)], X[tix(i, j)]);
+                });
+            });
+        });
+        // Gram entries (negated) of a, b, c: element [r][s] at lane x = s, y = r
+        const double gaa = -__shfl(X[tix(8, 8)], 4 * blk);
+        const double gab = -__shfl(X[tix(8, 8)], 16 + 4 * blk);
+        const double gac = -__shfl(X[tix(8, 8)], 32 + 4 * blk);
+        const double gbb = -__shfl(X[tix(8, 8)], 16 + 4 * blk + 1);
+        const double gbc = -__shfl(X[tix(8, 8)], 32 + 4 * blk + 1);
+        if (live && g == 0) {
+            double value, var, lag;
+            if constexpr (M == 0) {
+                value = P.sk_mean + gbc;
+                var = c0 - gbb;
+                lag = __builtin_nan("");
+            } else {
+                ok = ok && gaa > 0.0 && !(gaa < thresh);
+                lag = (gab - 1.0) / gaa;
+                value = gbc - gac * lag;
+                var = (c0 - gbb) + (gab - 1.0) * lag;
+            }
+            const bool good = !bad && ok;
+            P.value[t] = good ? value : __builtin_nan("");
+            P.variance[t] = good ? fmax(var, 0.0) : __builtin_nan("");
+            if (P.lagrange) P.lagrange[t] = good ? lag : __builtin_nan("");
+            if (P.status) P.status[t] = bad ? (craw > NP ? 4 : (craw < P.min_nb ? 5 : 1)) : (ok ? 0 : 3);
+        }
+        wave_sync();
+    }
+}
+
+template <int D, bool ANISO, int KIND, int M>
+hipError_t launch_krige_mfma_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s) {
+    int64_t blocks = (grid_groups + 3) / 4;
+    if (blocks < 1) blocks = 1;
+    if (blocks > (1 << 30)) blocks = 1 << 30;
+    hipLaunchKernelGGL((krige_mfma_kernel<D, ANISO, KIND, M>), dim3((unsigned)blocks), dim3(kWave), 0,
+                       s, p);
+    return hipGetLastError();
+}
+
 template <int D, bool ANISO, int KIND>
 hipError_t launch_vecchia_mfma_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s) {
     hipLaunchKernelGGL((vecchia_mfma_kernel<D, ANISO, KIND>), dim3((unsigned)n_blocks), dim3(kWave), 0,

@@ -60,6 +60,7 @@ SIGNATURES = {
     "gs_ctx_destroy": (None, [_P]),
     "gs_ctx_synchronize": (C.c_int32, [_P]),
     "gs_last_error": (C.c_char_p, []),
+    "gs_last_solve_kernel": (C.c_char_p, []),
     "gs_abi_version": (C.c_int32, []),
     "gs_kriging_new": (C.c_int32, [_P, C.c_int32, _DP, _DP, C.c_int64, C.POINTER(gs_model),
                                    C.POINTER(gs_kconfig), C.POINTER(_P), _I64P]),
@@ -159,6 +160,11 @@ def synth() -> C.CDLL:
 
 def last_error() -> str:
     return lib().gs_last_error().decode(errors="replace")
+
+
+def last_solve_kernel() -> str:
+    """Kernel family of the last solve launched on this thread."""
+    return lib().gs_last_solve_kernel().decode(errors="replace")
 
 
 def dptr(a):

@@ -84,6 +84,7 @@ extern "C" {
     pub fn gs_ctx_destroy(ctx: *mut gs_ctx);
     pub fn gs_ctx_synchronize(ctx: *mut gs_ctx) -> i32;
     pub fn gs_last_error() -> *const c_char;
+    pub fn gs_last_solve_kernel() -> *const c_char;
     pub fn gs_abi_version() -> i32;
     pub fn gs_model_validate(model: *const GsModel) -> i32;
 

@@ -109,6 +109,11 @@ void gs_ctx_destroy(gs_ctx *ctx);
 int32_t gs_ctx_synchronize(gs_ctx *ctx);
 /* Thread-local message of the last failing call on this thread. */
 const char *gs_last_error(void);
+/* Kernel family of the last solve this thread launched ("krige_mfma_kernel",
+ * "krige_fp64_kernel", "krige_fp32r_kernel", "vecchia_mfma_kernel",
+ * "vecchia_kernel", ...; "" before any).  No reference counterpart: a
+ * measurement hook naming the kernel a roofline figure belongs to. */
+const char *gs_last_solve_kernel(void);
 int32_t gs_abi_version(void);
 /* VariogramModel::new's checks (model.rs:554-618): nugget, sills, ranges,
  * kind parameters, anisotropy, positive total sill.  GS_OK or
