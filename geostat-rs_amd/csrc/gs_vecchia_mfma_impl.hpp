@@ -64,6 +64,69 @@ __device__ __forceinline__ double cov(const LeanRegs<D> &L, double c0, const dou
     return zero ? c0 : v;   // gamma(0) = 0
 }
 
+// The same covariance for two distinct points: the handles reject
+// duplicate coordinates (kriging.rs:312, vecchia.rs:248), and distinct
+// doubles have a nonzero difference, so a separation between two data
+// points is never the zero vector.
+template <int D, bool ANISO, int KIND>
+__device__ __forceinline__ double cov_nz(const LeanRegs<D> &L, const double *a, const double *b) {
+    double dh[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) dh[d] = a[d] - b[d];
+    return lean_term<D, ANISO, KIND == 0, true, KIND == 0 ? 0 : KIND>(L, dh);
+}
+
+// Element [y][x] of L^-1 for the diagonal tile in tt (row-major 4 x 4,
+// written by the block): every lane of the block factors the tile
+// redundantly, lane 0 of the block stores the ten entries of the inverse
+// factor in lt (whose upper triangle is kept at zero), and each lane reads
+// its own element back, so the choice is an LDS address rather than a
+// chain of lane-dependent selects.  ok is cleared when a pivot fails the
+// singular test (GE: pivot >= thresh, else pivot > thresh).
+template <bool GE>
+__device__ __forceinline__ double tile_linv(const double *tt, double *lt, int g, int y, int x,
+                                            double thresh, bool &ok) {
+    const double m00 = tt[0];
+    const double m10 = tt[4], m11 = tt[5];
+    const double m20 = tt[8], m21 = tt[9], m22 = tt[10];
+    const double m30 = tt[12], m31 = tt[13], m32 = tt[14], m33 = tt[15];
+    const double d0 = m00;
+    const double i0 = rsqrt_nr(d0);
+    const double l10 = m10 * i0, l20 = m20 * i0, l30 = m30 * i0;
+    const double d1 = m11 - l10 * l10;
+    const double i1 = rsqrt_nr(d1);
+    const double l21 = (m21 - l20 * l10) * i1, l31 = (m31 - l30 * l10) * i1;
+    const double d2 = (m22 - l20 * l20) - l21 * l21;
+    const double i2 = rsqrt_nr(d2);
+    const double l32 = ((m32 - l30 * l20) - l31 * l21) * i2;
+    const double d3 = ((m33 - l30 * l30) - l31 * l31) - l32 * l32;
+    const double i3 = rsqrt_nr(d3);
+    if constexpr (GE) ok = ok && d0 >= thresh && d1 >= thresh && d2 >= thresh && d3 >= thresh;
+    else ok = ok && d0 > thresh && d1 > thresh && d2 > thresh && d3 > thresh;
+    const double v10 = -l10 * i0 * i1;
+    const double v21 = -l21 * i1 * i2;
+    const double v20 = -(l20 * i0 + l21 * v10) * i2;
+    const double v32 = -l32 * i2 * i3;
+    const double v31 = -(l31 * i1 + l32 * v21) * i3;
+    const double v30 = -((l30 * i0 + l31 * v10) + l32 * v20) * i3;
+    if (g == 0) {
+        lt[0] = i0;
+        lt[4] = v10;
+        lt[5] = i1;
+        lt[8] = v20;
+        lt[9] = v21;
+        lt[10] = i2;
+        lt[12] = v30;
+        lt[13] = v31;
+        lt[14] = v32;
+        lt[15] = i3;
+    }
+    wave_sync();
+    const double linv = lt[4 * y + x];
+    wave_sync();
+    return linv;
+}
+
 }  // namespace vm
 
 template <int D, bool ANISO, int KIND>
@@ -71,22 +134,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     using namespace vm;
     constexpr int NP = 32;                    // conditioning slots per point
     __shared__ double s_xy[4 * NP * D];
-    __shared__ double s_z[4 * NP];
+    __shared__ double s_rhs[4 * 4 * NP];      // per block: the last tile row's rows k, z, 0, 0
     __shared__ double s_t[4 * 16];
+    __shared__ double s_li[4 * 16];           // per block: L^-1 of the current diagonal tile
     __shared__ double s_dg[4 * 48];           // per block: the diagonal tiles' entries below the diagonal
-    __shared__ double s_k[4 * NP];            // per block: covariances to the point
     __shared__ double s_sum[4 * kMaxModelsPerLaunch];
     const int lane = threadIdx.x;
     const int blk = (lane >> 2) & 3;          // the point of this lane
     const int x_ = lane >> 4, y_ = lane & 3;  // element [y][x] of every tile
     double *xy = s_xy + blk * NP * D;
-    double *zs = s_z + blk * NP;
+    double *rhs = s_rhs + blk * 4 * NP;
+    double *ks = rhs;                         // row 0: covariances to the point
+    double *zs = rhs + NP;                    // row 1: centred values of the conditioning set
     double *tt = s_t + blk * 16;
+    double *lt = s_li + blk * 16;
     double *dgs = s_dg + blk * 48;
-    double *ks = s_k + blk * NP;
     const int g = x_ * 4 + y_;                // rank of the lane inside its block
     const int nm = P.n_models;
     for (int p = g; p < nm; p += 16) s_sum[blk * kMaxModelsPerLaunch + p] = 0.0;
+    for (int j = g; j < 2 * NP; j += 16) rhs[2 * NP + j] = 0.0;
+    lt[g] = 0.0;
     load_exp_table();
     wave_sync();
     const double ln2pi = 1.8378770664093453;  // ln(2 pi)
@@ -139,13 +206,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 const int r = e >= 3 ? 3 : (e >= 1 ? 2 : 1);
                 const int c = e - r * (r - 1) / 2;
                 const int a = 4 * t + r, b = 4 * t + c;
-                const double v = pin(cov<D, ANISO, KIND>(L, c0, xy + a * D, xy + b * D));
+                const double v = pin(cov_nz<D, ANISO, KIND>(L, xy + a * D, xy + b * D));
                 dgs[u] = (a < cnt && b < cnt) ? v : 0.0;
             }
+            // (the point is not in its own conditioning set)
 #pragma unroll
             for (int e2 = 0; e2 < 2; e2++) {
                 const int b = 2 * g + e2;
-                const double v = pin(cov<D, ANISO, KIND>(L, c0, tg, xy + b * D));
+                const double v = pin(cov_nz<D, ANISO, KIND>(L, tg, xy + b * D));
                 ks[b] = b < cnt ? v : 0.0;
             }
             static_for<1, 8>([&](auto II) {
@@ -160,7 +228,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                     double pb[D];
 #pragma unroll
                     for (int d = 0; d < D; d++) pb[d] = xy[b * D + d];
-                    const double v = pin(cov<D, ANISO, KIND>(L, c0, pa, pb));
+                    const double v = pin(cov_nz<D, ANISO, KIND>(L, pa, pb));
                     X[tix(i, j)] = (a < cnt && b < cnt) ? v : 0.0;
                     if constexpr ((tix(i, j) % 2) == 1) __builtin_amdgcn_sched_barrier(0);
                 });
@@ -176,8 +244,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             }
             static_for<0, 8>([&](auto JJ) {
                 constexpr int j = decltype(JJ)::value;
-                const int b = 4 * j + x;
-                X[tix(8, j)] = y == 0 ? ks[b] : (y == 1 && b < cnt ? zs[b] : 0.0);
+                X[tix(8, j)] = rhs[y * NP + 4 * j + x];
             });
             X[tix(8, 8)] = 0.0;
             wave_sync();
@@ -188,35 +255,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 // the diagonal tile, factored by every lane of the block
                 tt[4 * y + x] = X[tix(q, q)];
                 wave_sync();
-                const double m00 = tt[0];
-                const double m10 = tt[4], m11 = tt[5];
-                const double m20 = tt[8], m21 = tt[9], m22 = tt[10];
-                const double m30 = tt[12], m31 = tt[13], m32 = tt[14], m33 = tt[15];
-                wave_sync();
-                const double d0 = m00;
-                const double i0 = rsqrt_nr(d0);
-                const double l10 = m10 * i0, l20 = m20 * i0, l30 = m30 * i0;
-                const double d1 = m11 - l10 * l10;
-                const double i1 = rsqrt_nr(d1);
-                const double l21 = (m21 - l20 * l10) * i1, l31 = (m31 - l30 * l10) * i1;
-                const double d2 = (m22 - l20 * l20) - l21 * l21;
-                const double i2 = rsqrt_nr(d2);
-                const double l32 = ((m32 - l30 * l20) - l31 * l21) * i2;
-                const double d3 = ((m33 - l30 * l30) - l31 * l31) - l32 * l32;
-                const double i3 = rsqrt_nr(d3);
-                ok = ok && d0 > thresh && d1 > thresh && d2 > thresh && d3 > thresh;
-                // L^-1 (lower): element [y][x] of this lane
-                const double v10 = -l10 * i0 * i1;
-                const double v21 = -l21 * i1 * i2;
-                const double v20 = -(l20 * i0 + l21 * v10) * i2;
-                const double v32 = -l32 * i2 * i3;
-                const double v31 = -(l31 * i1 + l32 * v21) * i3;
-                const double v30 = -((l30 * i0 + l31 * v10) + l32 * v20) * i3;
-                double linv = 0.0;
-                if (y == x) linv = y == 0 ? i0 : (y == 1 ? i1 : (y == 2 ? i2 : i3));
-                else if (y == 1 && x == 0) linv = v10;
-                else if (y == 2) linv = x == 0 ? v20 : (x == 1 ? v21 : 0.0);
-                else if (y == 3) linv = x == 0 ? v30 : (x == 1 ? v31 : (x == 2 ? v32 : 0.0));
+                const double linv = tile_linv<false>(tt, lt, g, y, x, thresh, ok);
                 // tiles below: L_iq = X_iq L_qq^-T
                 static_for<q + 1, 9>([&](auto II) {
                     constexpr int i = decltype(II)::value;
@@ -277,23 +316,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     static_assert(M == 0 || M == 1, "simple or ordinary kriging");
     constexpr int NP = 32;
     __shared__ double s_xy[4 * NP * D];
-    __shared__ double s_v[4 * NP];
+    __shared__ double s_rhs[4 * 4 * NP];      // per block: the last tile row's rows f, k, z, 0
     __shared__ double s_t[4 * 16];
+    __shared__ double s_li[4 * 16];
     __shared__ double s_dg[4 * 48];
-    __shared__ double s_k[4 * NP];
     const int lane = threadIdx.x;
     const int blk = (lane >> 2) & 3;
     const int x_ = lane >> 4, y_ = lane & 3;
     const int g = x_ * 4 + y_;
     double *xy = s_xy + blk * NP * D;
-    double *vs = s_v + blk * NP;
+    double *rhs = s_rhs + blk * 4 * NP;
+    double *fs = rhs, *ks = rhs + NP, *vs = rhs + 2 * NP;
     double *tt = s_t + blk * 16;
+    double *lt = s_li + blk * 16;
     double *dgs = s_dg + blk * 48;
-    double *ks = s_k + blk * NP;
     const DevModel &Mdl = *P.model;
     const double c0 = Mdl.c0;
     const LeanRegs<D> L = load_lean<D>(Mdl.s[0]);
     const double thresh = 1e-12 * (M >= 1 ? fmax(c0, 1.0) : c0);
+    for (int j = g; j < NP; j += 16) rhs[3 * NP + j] = 0.0;
+    lt[g] = 0.0;
     load_exp_table();
     wave_sync();
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
@@ -313,6 +355,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 #pragma unroll
             for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt_ ? P.coords[id * D + d] : 0.0;
             vs[j] = j < cnt_ ? (M == 0 ? P.values[id] - P.sk_mean : P.values[id]) : 0.0;
+            fs[j] = (M == 1 && j < cnt_) ? 1.0 : 0.0;
         }
         wave_sync();
         const int y = opaque(y_), x = opaque(x_), cnt = opaque(cnt_);
@@ -324,9 +367,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             const int r = e >= 3 ? 3 : (e >= 1 ? 2 : 1);
             const int c = e - r * (r - 1) / 2;
             const int a = 4 * tt6 + r, b = 4 * tt6 + c;
-            const double v = pin(cov<D, ANISO, KIND>(L, c0, xy + a * D, xy + b * D));
+            const double v = pin(cov_nz<D, ANISO, KIND>(L, xy + a * D, xy + b * D));
             dgs[u] = (a < cnt && b < cnt) ? v : 0.0;
         }
+        // a target may sit on a datum: c0 there
 #pragma unroll
         for (int e2 = 0; e2 < 2; e2++) {
             const int b = 2 * g + e2;
@@ -345,7 +389,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 double pb[D];
 #pragma unroll
                 for (int d = 0; d < D; d++) pb[d] = xy[b * D + d];
-                const double v = pin(cov<D, ANISO, KIND>(L, c0, pa, pb));
+                const double v = pin(cov_nz<D, ANISO, KIND>(L, pa, pb));
                 X[tix(i, j)] = (a < cnt && b < cnt) ? v : 0.0;
                 if constexpr ((tix(i, j) % 2) == 1) __builtin_amdgcn_sched_barrier(0);
             });
@@ -361,9 +405,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         }
         static_for<0, 8>([&](auto JJ) {
             constexpr int j = decltype(JJ)::value;
-            const int b = 4 * j + x;
-            const double fb = (M == 1 && b < cnt) ? 1.0 : 0.0;
-            X[tix(8, j)] = y == 0 ? fb : (y == 1 ? ks[b] : (y == 2 ? vs[b] : 0.0));
+            X[tix(8, j)] = rhs[y * NP + 4 * j + x];
         });
         X[tix(8, 8)] = 0.0;
         wave_sync();
@@ -372,34 +414,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             constexpr int q = decltype(PP)::value;
             tt[4 * y + x] = X[tix(q, q)];
             wave_sync();
-            const double m00 = tt[0];
-            const double m10 = tt[4], m11 = tt[5];
-            const double m20 = tt[8], m21 = tt[9], m22 = tt[10];
-            const double m30 = tt[12], m31 = tt[13], m32 = tt[14], m33 = tt[15];
-            wave_sync();
-            const double d0 = m00;
-            const double i0 = rsqrt_nr(d0);
-            const double l10 = m10 * i0, l20 = m20 * i0, l30 = m30 * i0;
-            const double d1 = m11 - l10 * l10;
-            const double i1 = rsqrt_nr(d1);
-            const double l21 = (m21 - l20 * l10) * i1, l31 = (m31 - l30 * l10) * i1;
-            const double d2 = (m22 - l20 * l20) - l21 * l21;
-            const double i2 = rsqrt_nr(d2);
-            const double l32 = ((m32 - l30 * l20) - l31 * l21) * i2;
-            const double d3 = ((m33 - l30 * l30) - l31 * l31) - l32 * l32;
-            const double i3 = rsqrt_nr(d3);
-            ok = ok && d0 >= thresh && d1 >= thresh && d2 >= thresh && d3 >= thresh;
-            const double v10 = -l10 * i0 * i1;
-            const double v21 = -l21 * i1 * i2;
-            const double v20 = -(l20 * i0 + l21 * v10) * i2;
-            const double v32 = -l32 * i2 * i3;
-            const double v31 = -(l31 * i1 + l32 * v21) * i3;
-            const double v30 = -((l30 * i0 + l31 * v10) + l32 * v20) * i3;
-            double linv = 0.0;
-            if (y == x) linv = y == 0 ? i0 : (y == 1 ? i1 : (y == 2 ? i2 : i3));
-            else if (y == 1 && x == 0) linv = v10;
-            else if (y == 2) linv = x == 0 ? v20 : (x == 1 ? v21 : 0.0);
-            else if (y == 3) linv = x == 0 ? v30 : (x == 1 ? v31 : (x == 2 ? v32 : 0.0));
+            const double linv = tile_linv<true>(tt, lt, g, y, x, thresh, ok);
             static_for<q + 1, 9>([&](auto II) {
                 constexpr int i = decltype(II)::value;
                 X[tix(i, q)] = mfma(linv, X[tix(i, q)], 0.0);
