@@ -227,15 +227,13 @@ __device__ __forceinline__ double cov_q(const Frame<D, ANISO, KIND> &F, const do
 }
 
 // Element [y][x] of L^-1 for the diagonal tile in tt (row-major 4 x 4,
-// written by the block): every lane of the block factors the tile
-// redundantly, lane 0 of the block stores the ten entries of the inverse
-// factor in lt (whose upper triangle is kept at zero), and each lane reads
-// its own element back, so the choice is an LDS address rather than a
-// chain of lane-dependent selects.  ok is cleared when a pivot fails the
-// singular test (GE: pivot >= thresh, else pivot > thresh).
+// written by the block): every lane of the block factors the tile and
+// inverts the factor redundantly, then keeps its own element (selects
+// between values every lane holds, no lane-dependent branch).  ok is
+// cleared when a pivot fails the singular test (GE: pivot >= thresh, else
+// pivot > thresh).
 template <bool GE>
-__device__ __forceinline__ double tile_linv(const double *tt, double *lt, int g, int y, int x,
-                                            double thresh, bool &ok) {
+__device__ __forceinline__ double tile_linv(const double *tt, int y, int x, double thresh, bool &ok) {
     const double m00 = tt[0];
     const double m10 = tt[4], m11 = tt[5];
     const double m20 = tt[8], m21 = tt[9], m22 = tt[10];
@@ -253,28 +251,63 @@ __device__ __forceinline__ double tile_linv(const double *tt, double *lt, int g,
     const double i3 = rsqrt_nr(d3);
     if constexpr (GE) ok = ok && d0 >= thresh && d1 >= thresh && d2 >= thresh && d3 >= thresh;
     else ok = ok && d0 > thresh && d1 > thresh && d2 > thresh && d3 > thresh;
-    const double v10 = -l10 * i0 * i1;
-    const double v21 = -l21 * i1 * i2;
-    const double v20 = -(l20 * i0 + l21 * v10) * i2;
-    const double v32 = -l32 * i2 * i3;
-    const double v31 = -(l31 * i1 + l32 * v21) * i3;
-    const double v30 = -((l30 * i0 + l31 * v10) + l32 * v20) * i3;
-    if (g == 0) {
-        lt[0] = i0;
-        lt[4] = v10;
-        lt[5] = i1;
-        lt[8] = v20;
-        lt[9] = v21;
-        lt[10] = i2;
-        lt[12] = v30;
-        lt[13] = v31;
-        lt[14] = v32;
-        lt[15] = i3;
-    }
+    // pinned: computed in every lane before the selects, not sunk into
+    // lane-dependent branches
+    const double v10 = pin(-l10 * i0 * i1);
+    const double v21 = pin(-l21 * i1 * i2);
+    const double v20 = pin(-(l20 * i0 + l21 * v10) * i2);
+    const double v32 = pin(-l32 * i2 * i3);
+    const double v31 = pin(-(l31 * i1 + l32 * v21) * i3);
+    const double v30 = pin(-((l30 * i0 + l31 * v10) + l32 * v20) * i3);
+    double dv = y == 1 ? pin(i1) : pin(i0);
+    dv = y == 2 ? pin(i2) : dv;
+    dv = y == 3 ? pin(i3) : dv;
+    double c0s = y == 1 ? v10 : v20;
+    c0s = y == 3 ? v30 : c0s;
+    const double c1s = y == 3 ? v31 : v21;
+    double ov = x == 1 ? c1s : v32;
+    ov = x == 0 ? c0s : ov;
+    return y == x ? dv : (y > x ? ov : 0.0);
+}
+
+// Blocked right-looking Cholesky of the 9 x 9 tile system over its eight
+// tile columns (the last tile row carried along).  Per column q: the tiles
+// below the diagonal are multiplied by L_qq^-T (one MFMA each) and the
+// trailing tiles updated (one MFMA each).  The next diagonal tile is
+// updated first and written to tt before the other trailing updates are
+// issued, so its LDS round trip runs under them.
+template <bool GE>
+__device__ __forceinline__ void factor_tiles(double (&X)[kTiles], double *tt, int y, int x,
+                                             double thresh, bool &ok) {
+    tt[4 * y + x] = X[tix(0, 0)];
     wave_sync();
-    const double linv = lt[4 * y + x];
-    wave_sync();
-    return linv;
+    double linv = tile_linv<GE>(tt, y, x, thresh, ok);
+    static_for<0, 8>([&](auto PP) {
+        constexpr int q = decltype(PP)::value;
+        // tiles below: L_iq = X_iq L_qq^-T
+        static_for<q + 1, 9>([&](auto II) {
+            constexpr int i = decltype(II)::value;
+            X[tix(i, q)] = mfma(linv, X[tix(i, q)], 0.0);
+        });
+        if constexpr (q < 7) {
+            X[tix(q + 1, q + 1)] = mfma_nega(X[tix(q + 1, q)], X[tix(q + 1, q)], X[tix(q + 1, q + 1)]);
+            wave_sync();
+            tt[4 * y + x] = X[tix(q + 1, q + 1)];
+        }
+        // trailing tiles: X_ij -= L_iq L_jq^T
+        static_for<q + 1, 9>([&](auto II) {
+            constexpr int i = decltype(II)::value;
+            static_for<q + 1, i + 1>([&](auto JJ) {
+                constexpr int j = decltype(JJ)::value;
+                if constexpr (!(i == q + 1 && j == q + 1 && q < 7))
+                    X[tix(i, j)] = mfma_nega(X[tix(j, q)], X[tix(i, q)], X[tix(i, j)]);
+            });
+        });
+        if constexpr (q < 7) {
+            wave_sync();
+            linv = tile_linv<GE>(tt, y, x, thresh, ok);
+        }
+    });
 }
 
 }  // namespace vm
@@ -287,7 +320,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     __shared__ double s_q[4 * NP * D];        // per block: the same in the current model's frame
     __shared__ double s_rhs[4 * 2 * NP + NP]; // per block: the last tile row's rows k, z; a zero row
     __shared__ double s_t[4 * 16];
-    __shared__ double s_li[4 * 16];           // per block: L^-1 of the current diagonal tile
     __shared__ double s_dg[4 * 48];           // per block: the diagonal tiles' entries below the diagonal
     __shared__ double s_sum[4 * kMaxModelsPerLaunch];
     const int lane = threadIdx.x;
@@ -301,13 +333,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     // the lane's row of the last tile row (rows 2 and 3 are zero)
     const double *rrow = y_ < 2 ? rhs + y_ * NP : s_rhs + 8 * NP;
     double *tt = s_t + blk * 16;
-    double *lt = s_li + blk * 16;
     double *dgs = s_dg + blk * 48;
     const int g = x_ * 4 + y_;                // rank of the lane inside its block
     const int nm = P.n_models;
     for (int p = g; p < nm; p += 16) s_sum[blk * kMaxModelsPerLaunch + p] = 0.0;
     if (lane < NP) s_rhs[8 * NP + lane] = 0.0;
-    lt[g] = 0.0;
     load_exp256();
     wave_sync();
     const double ln2pi = 1.8378770664093453;  // ln(2 pi)
@@ -417,26 +447,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             wave_sync();
             // ---- blocked Cholesky over the eight tile columns of K
             bool ok = true;
-            static_for<0, 8>([&](auto PP) {
-                constexpr int q = decltype(PP)::value;
-                // the diagonal tile, factored by every lane of the block
-                tt[4 * y + x] = X[tix(q, q)];
-                wave_sync();
-                const double linv = tile_linv<false>(tt, lt, g, y, x, thresh, ok);
-                // tiles below: L_iq = X_iq L_qq^-T
-                static_for<q + 1, 9>([&](auto II) {
-                    constexpr int i = decltype(II)::value;
-                    X[tix(i, q)] = mfma(linv, X[tix(i, q)], 0.0);
-                });
-                // trailing tiles: X_ij -= L_iq L_jq^T (the next diagonal first)
-                static_for<q + 1, 9>([&](auto II) {
-                    constexpr int i = decltype(II)::value;
-                    static_for<q + 1, i + 1>([&](auto JJ) {
-                        constexpr int j = decltype(JJ)::value;
-                        X[tix(i, j)] = mfma_nega(X[tix(j, q)], X[tix(i, q)], X[tix(i, j)]);
-                    });
-                });
-            });
+            factor_tiles<false>(X, tt, y, x, thresh, ok);
             // bottom-right tile: [0][0] = -|L^-1 k|^2 (lane x = 0, y = 0),
             // [0][1] = -(L^-1 k).(L^-1 z) (lane x = 1, y = 0)
             const double t00 = __shfl(X[tix(8, 8)], 4 * blk);
@@ -485,7 +496,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     __shared__ double s_xy[4 * NP * D];       // per block: the neighbours in the frame, target at the origin
     __shared__ double s_rhs[4 * 3 * NP + NP]; // per block: the last tile row's rows f, k, z; a zero row
     __shared__ double s_t[4 * 16];
-    __shared__ double s_li[4 * 16];
     __shared__ double s_dg[4 * 48];
     const int lane = threadIdx.x;
     const int blk = (lane >> 2) & 3;
@@ -496,14 +506,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     double *fs = rhs, *ks = rhs + NP, *vs = rhs + 2 * NP;
     const double *rrow = y_ < 3 ? rhs + y_ * NP : s_rhs + 12 * NP;
     double *tt = s_t + blk * 16;
-    double *lt = s_li + blk * 16;
     double *dgs = s_dg + blk * 48;
     const DevModel &Mdl = *P.model;
     const double c0 = Mdl.c0;
     const Frame<D, ANISO, KIND> F = make_frame<D, ANISO, KIND>(Mdl.s[0]);
     const double thresh = 1e-12 * (M >= 1 ? fmax(c0, 1.0) : c0);
     if (lane < NP) s_rhs[12 * NP + lane] = 0.0;
-    lt[g] = 0.0;
     load_exp256();
     wave_sync();
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
@@ -599,23 +607,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         X[tix(8, 8)] = 0.0;
         wave_sync();
         bool ok = true;
-        static_for<0, 8>([&](auto PP) {
-            constexpr int q = decltype(PP)::value;
-            tt[4 * y + x] = X[tix(q, q)];
-            wave_sync();
-            const double linv = tile_linv<true>(tt, lt, g, y, x, thresh, ok);
-            static_for<q + 1, 9>([&](auto II) {
-                constexpr int i = decltype(II)::value;
-                X[tix(i, q)] = mfma(linv, X[tix(i, q)], 0.0);
-            });
-            static_for<q + 1, 9>([&](auto II) {
-                constexpr int i = decltype(II)::value;
-                static_for<q + 1, i + 1>([&](auto JJ) {
-                    constexpr int j = decltype(JJ)::value;
-                    X[tix(i, j)] = mfma_nega(X[tix(j, q)], X[tix(i, q)], X[tix(i, j)]);
-                });
-            });
-        });
+        factor_tiles<true>(X, tt, y, x, thresh, ok);
         // Gram entries (negated) of a, b, c: element [r][s] at lane x = s, y = r
         const double gaa = -__shfl(X[tix(8, 8)], 4 * blk);
         const double gab = -__shfl(X[tix(8, 8)], 16 + 4 * blk);
