@@ -212,6 +212,13 @@ __device__ __forceinline__ double cov_h2(const Frame<D, ANISO, KIND> &F, double 
     return F.a0 * exp_neg256(d);
 }
 
+// Empty slots of a system sit far out on the first axis, slot j at
+// kFar (j + 1) relative to the centre: at least 10^60 apart from each other
+// and from every datum, so their covariances with everything evaluate to
+// exactly zero (the exponential underflows; the spherical kind is zero past
+// its range) and no mask is needed on the entries.
+constexpr double kFar = 1e60;
+
 // Covariance of two distinct points from their frame coordinates.  The
 // handles reject duplicate coordinates (kriging.rs:312, vecchia.rs:248),
 // so two data points are never at zero separation.
@@ -358,13 +365,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 #pragma unroll
         for (int d = 0; d < D; d++) tg[d] = P.coords[self * D + d];
         const double zi = P.zc[self];
-        // the conditioning set relative to the point (slots past the count
-        // are zero: every entry evaluated, used or not, comes from finite
-        // data)
+        // the conditioning set relative to the point; empty slots far out
         for (int j = g; j < NP; j += 16) {
             const int64_t id = j < cnt_ ? nbl[j] : 0;
 #pragma unroll
-            for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : 0.0;
+            for (int d = 0; d < D; d++)
+                xy[j * D + d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : (d == 0 ? kFar * (j + 1) : 0.0);
             zs[j] = j < cnt_ ? P.zc[id] : 0.0;
         }
         wave_sync();
@@ -382,10 +388,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 for (int d = 0; d < D; d++) qs[j * D + d] = q[d];
             }
             wave_sync();
-            // lane coordinates made opaque per model: the coordinates and
-            // masks of the build are then read inside the model loop instead
-            // of being hoisted out of it into long-lived registers
-            const int y = opaque(y_), x = opaque(x_), cnt = opaque(cnt_);
+            // lane coordinates made opaque per model: the coordinates of the
+            // build are then read inside the model loop instead of being
+            // hoisted out of it into long-lived registers
+            const int y = opaque(y_), x = opaque(x_);
             double X[kTiles];
             // ---- build: element [y][x] of tile (i, j) is row 4i+y, column 4j+x.
             // The 48 distinct entries of the diagonal tiles (below their
@@ -400,8 +406,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 const int r = e >= 3 ? 3 : (e >= 1 ? 2 : 1);
                 const int c = e - r * (r - 1) / 2;
                 const int a = 4 * t + r, b = 4 * t + c;
-                const double v = pin(cov_q(F, qs + a * D, qs + b * D));
-                dgs[u] = (a < cnt && b < cnt) ? v : 0.0;
+                dgs[u] = cov_q(F, qs + a * D, qs + b * D);
             }
             // to the point, at the origin (it is not in its own conditioning set)
 #pragma unroll
@@ -410,8 +415,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 double h2 = 0.0;
 #pragma unroll
                 for (int d = 0; d < D; d++) h2 = fma(qs[b * D + d], qs[b * D + d], h2);
-                const double v = pin(cov_h2(F, h2));
-                ks[b] = b < cnt ? v : 0.0;
+                ks[b] = cov_h2(F, h2);
             }
             static_for<1, 8>([&](auto II) {
                 constexpr int i = decltype(II)::value;
@@ -425,8 +429,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                     double pb[D];
 #pragma unroll
                     for (int d = 0; d < D; d++) pb[d] = qs[b * D + d];
-                    const double v = pin(cov_q(F, pa, pb));
-                    X[tix(i, j)] = (a < cnt && b < cnt) ? v : 0.0;
+                    X[tix(i, j)] = pin(cov_q(F, pa, pb));
                     if constexpr ((tix(i, j) % 2) == 1) __builtin_amdgcn_sched_barrier(0);
                 });
             });
@@ -528,14 +531,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         const int32_t *nbl = P.nbr + t * (int64_t)P.stride;
         // the neighbours relative to the target, in the frame; a neighbour
         // on the target maps to the origin, any other to a nonzero point
-        // (an image that underflows to zero is moved off the origin)
+        // (an image that underflows to zero is moved off the origin); empty
+        // slots far out
         for (int j = g; j < NP; j += 16) {
             const int64_t id = j < cnt_ ? nbl[j] : 0;
             double c[D], q[D];
             bool on = true, qz = true;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                c[d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : 0.0;
+                c[d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : (d == 0 ? kFar * (j + 1) : 0.0);
                 on = on && c[d] == 0.0;
             }
             to_frame(F, c, q);
@@ -548,7 +552,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             fs[j] = (M == 1 && j < cnt_) ? 1.0 : 0.0;
         }
         wave_sync();
-        const int y = opaque(y_), x = opaque(x_), cnt = opaque(cnt_);
+        const int y = opaque(y_), x = opaque(x_);
         double X[kTiles];
 #pragma unroll
         for (int e3 = 0; e3 < 3; e3++) {
@@ -557,8 +561,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             const int r = e >= 3 ? 3 : (e >= 1 ? 2 : 1);
             const int c = e - r * (r - 1) / 2;
             const int a = 4 * tt6 + r, b = 4 * tt6 + c;
-            const double v = pin(cov_q(F, xy + a * D, xy + b * D));
-            dgs[u] = (a < cnt && b < cnt) ? v : 0.0;
+            dgs[u] = cov_q(F, xy + a * D, xy + b * D);
         }
         // to the target, at the origin: c0 for a neighbour on it (gamma(0) = 0)
 #pragma unroll
@@ -571,8 +574,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 h2 = fma(xy[b * D + d], xy[b * D + d], h2);
                 on = on && xy[b * D + d] == 0.0;
             }
-            const double v = pin(cov_h2(F, h2));
-            ks[b] = b < cnt ? (on ? c0 : v) : 0.0;
+            const double v = cov_h2(F, h2);
+            ks[b] = on ? c0 : v;
         }
         static_for<1, 8>([&](auto II) {
             constexpr int i = decltype(II)::value;
@@ -586,8 +589,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
                 double pb[D];
 #pragma unroll
                 for (int d = 0; d < D; d++) pb[d] = xy[b * D + d];
-                const double v = pin(cov_q(F, pa, pb));
-                X[tix(i, j)] = (a < cnt && b < cnt) ? v : 0.0;
+                X[tix(i, j)] = pin(cov_q(F, pa, pb));
                 if constexpr ((tix(i, j) % 2) == 1) __builtin_amdgcn_sched_barrier(0);
             });
         });
