@@ -533,6 +533,14 @@ struct gs_vplan {
     std::vector<int64_t> order;
     std::vector<int32_t> nb;
     std::vector<int32_t> nb_count;
+    // Device storage order: position h of the device arrays holds point
+    // perm[h] (points along a space-filling curve), so a block of
+    // consecutive positions covers a compact region and the gathers of its
+    // conditioning sets share cache lines.  d_order is the identity over
+    // positions, d_nb / d_count are the plan rows of the points at those
+    // positions, with the conditioning sets as positions, and zc is stored
+    // by position.
+    std::vector<int64_t> perm;
     int32_t max_count = 0;
     double extent = 0.0;     // length of the bounding-box diagonal (MLE start scale)
     DBuf coords, d_order, d_nb, d_count, zc, models, partial, out, status;
@@ -2677,6 +2685,64 @@ int host_threads() {
     return (int)std::max(1u, std::min(h, 64u));
 }
 
+// Morton (Z-order) index of a cell: the bits of the D axes interleaved
+// (16 bits per axis in 2-D, 10 in 3-D).
+uint32_t morton_key(const uint32_t *X, int D) {
+    if (D == 2) {
+        auto spread = [](uint32_t v) {
+            v = (v | (v << 8)) & 0x00FF00FFu;
+            v = (v | (v << 4)) & 0x0F0F0F0Fu;
+            v = (v | (v << 2)) & 0x33333333u;
+            return (v | (v << 1)) & 0x55555555u;
+        };
+        return spread(X[0]) | (spread(X[1]) << 1);
+    }
+    auto spread3 = [](uint32_t v) {
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        return (v | (v << 2)) & 0x09249249u;
+    };
+    return spread3(X[0]) | (spread3(X[1]) << 1) | (spread3(X[2]) << 2);
+}
+
+// Points ordered along the Morton curve of their bounding box (ties by
+// index): an LSD radix sort of (key << 32 | index) in 11-bit digits.
+std::vector<int64_t> curve_order(int D, const double *c, int64_t n) {
+    const int bits = D == 2 ? 16 : 10;
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) { mn[d] = INFINITY; mx[d] = -INFINITY; }
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < D; d++) {
+            mn[d] = std::fmin(mn[d], c[i * D + d]);
+            mx[d] = std::fmax(mx[d], c[i * D + d]);
+        }
+    const double top = (double)((1u << bits) - 1);
+    std::vector<uint64_t> kv((size_t)n), tmp((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        uint32_t X[3] = {0, 0, 0};
+        for (int d = 0; d < D; d++) {
+            const double e = mx[d] - mn[d];
+            const double f = e > 0.0 ? (c[i * D + d] - mn[d]) / e * top : 0.0;
+            X[d] = (uint32_t)std::min(top, std::max(0.0, f));
+        }
+        kv[(size_t)i] = ((uint64_t)morton_key(X, D) << 32) | (uint64_t)i;
+    }
+    auto pass = [&](const std::vector<uint64_t> &src, std::vector<uint64_t> &dst, int shift) {
+        std::vector<int64_t> cnt(2049, 0);
+        for (int64_t i = 0; i < n; i++) cnt[((src[(size_t)i] >> shift) & 2047u) + 1]++;
+        for (size_t j = 1; j < cnt.size(); j++) cnt[j] += cnt[j - 1];
+        for (int64_t i = 0; i < n; i++) dst[(size_t)cnt[(src[(size_t)i] >> shift) & 2047u]++] = src[(size_t)i];
+    };
+    pass(kv, tmp, 32);                    // key bits 0-10, 11-21, 22-31
+    pass(tmp, kv, 43);
+    pass(kv, tmp, 54);
+    kv.swap(tmp);
+    std::vector<int64_t> perm((size_t)n);
+    for (int64_t h = 0; h < n; h++) perm[(size_t)h] = (int64_t)(kv[(size_t)h] & 0xffffffffu);
+    return perm;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2835,15 +2901,49 @@ int32_t gs_vecchia_plan_new(gs_ctx *ctx, int32_t dim, const double *coords, int6
         P->extent = std::sqrt(e2);
     }
 
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(P->coords.ensure(sizeof(double) * n * dim));
-    HIP_TRY(P->d_order.ensure(sizeof(int64_t) * n));
-    HIP_TRY(P->d_nb.ensure(sizeof(int32_t) * n * m));
-    HIP_TRY(P->d_count.ensure(sizeof(int32_t) * n));
-    HIP_TRY(hipMemcpy(P->coords.p, coords, sizeof(double) * n * dim, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P->d_order.p, P->order.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P->d_nb.p, P->nb.data(), sizeof(int32_t) * n * m, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P->d_count.p, P->nb_count.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    // device arrays in curve order (the plan's terms are independent, so
+    // the likelihood is the same sum in another order)
+    P->perm = curve_order(dim, coords, n);
+    {
+        std::vector<int64_t> rank((size_t)n), pos((size_t)n), ident((size_t)n);
+        for (int64_t k = 0; k < n; k++) rank[(size_t)P->order[(size_t)k]] = k;
+        for (int64_t h = 0; h < n; h++) pos[(size_t)P->perm[(size_t)h]] = h;
+        std::vector<double> hc((size_t)(n * dim));
+        std::unique_ptr<int32_t[]> hnb(new int32_t[(size_t)(n * m)]);
+        std::vector<int32_t> hcnt((size_t)n);
+        std::atomic<int64_t> nxt(0);
+        auto fill = [&]() {
+            for (;;) {
+                const int64_t s0 = nxt.fetch_add(8192);
+                if (s0 >= n) break;
+                const int64_t e0 = std::min(n, s0 + 8192);
+                for (int64_t h = s0; h < e0; h++) {
+                    const int64_t i = P->perm[(size_t)h];
+                    const int64_t k = rank[(size_t)i];
+                    ident[(size_t)h] = h;
+                    for (int d = 0; d < dim; d++) hc[(size_t)(h * dim + d)] = coords[i * dim + d];
+                    const int32_t c = P->nb_count[(size_t)k];
+                    hcnt[(size_t)h] = c;
+                    for (int32_t q = 0; q < c; q++)
+                        hnb[(size_t)(h * m + q)] = (int32_t)pos[(size_t)P->nb[(size_t)(k * m + q)]];
+                    for (int64_t q = c; q < m; q++) hnb[(size_t)(h * m + q)] = 0;
+                }
+            }
+        };
+        std::vector<std::thread> ft;
+        for (int i = 1; i < nt; i++) ft.emplace_back(fill);
+        fill();
+        for (auto &t : ft) t.join();
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(P->coords.ensure(sizeof(double) * n * dim));
+        HIP_TRY(P->d_order.ensure(sizeof(int64_t) * n));
+        HIP_TRY(P->d_nb.ensure(sizeof(int32_t) * n * m));
+        HIP_TRY(P->d_count.ensure(sizeof(int32_t) * n));
+        HIP_TRY(hipMemcpy(P->coords.p, hc.data(), sizeof(double) * n * dim, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_order.p, ident.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_nb.p, hnb.get(), sizeof(int32_t) * n * m, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_count.p, hcnt.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    }
     *out = P.release();
     return GS_OK;
 }
@@ -2890,7 +2990,7 @@ int32_t vecchia_batch(gs_vplan *p, const double *values, const gs_model *models,
     for (int64_t i = 0; i < n; i++) mean += values[i];
     mean = mean / (double)n;
     std::vector<double> zc((size_t)n);
-    for (int64_t i = 0; i < n; i++) zc[(size_t)i] = values[i] - mean;
+    for (int64_t h = 0; h < n; h++) zc[(size_t)h] = values[p->perm[(size_t)h]] - mean;   // by position
     std::vector<DevModel> dm((size_t)n_models);
     for (int32_t i = 0; i < n_models; i++) dm[(size_t)i] = resolve_model(models[i], p->D, p->ctx->nodes.as<gs::BesselNodes>());
     const int NB = pad_nb(std::max<int32_t>(p->max_count, 1));
@@ -3049,7 +3149,7 @@ int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs
         for (int64_t i = 0; i < n; i++) mean += values[i];
         mean = mean / (double)n;
         std::vector<double> zc((size_t)n);
-        for (int64_t i = 0; i < n; i++) zc[(size_t)i] = values[i] - mean;
+        for (int64_t h = 0; h < n; h++) zc[(size_t)h] = values[p->perm[(size_t)h]] - mean;   // by position
         const int32_t ne = (int32_t)elig.size();
         std::vector<DevModel> dm((size_t)ne);
         for (int32_t i = 0; i < ne; i++)
