@@ -35,6 +35,16 @@ namespace vm {
 
 constexpr int kTiles = 45;   // 36 of K's lower triangle + 9 of the last tile row
 
+// Per-block LDS strides (doubles), padded so that the four blocks' copies
+// start on different banks: the lanes of a wavefront read the same element
+// of four blocks at once (coordinates of a tile row or column, the diagonal
+// tile, its entries), and unpadded power-of-two strides put all four on one
+// bank.
+template <int D>
+constexpr int kXYS = 32 * D + 8;   // 32 slots of D coordinates, +64 B
+constexpr int kTS = 18;            // a 4 x 4 tile, +16 B (16-byte aligned)
+constexpr int kDGS = 50;           // 48 diagonal-tile entries, +16 B
+
 __host__ __device__ constexpr int tix(int i, int j) { return i < 8 ? i * (i + 1) / 2 + j : 36 + j; }
 
 // The value, pinned where it is computed: the selects of the build then
@@ -323,24 +333,24 @@ template <int D, bool ANISO, int KIND>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void vecchia_mfma_kernel(VecchiaParams P) {
     using namespace vm;
     constexpr int NP = 32;                    // conditioning slots per point
-    __shared__ double s_xy[4 * NP * D];       // per block: the conditioning set relative to the point
-    __shared__ double s_q[4 * NP * D];        // per block: the same in the current model's frame
+    __shared__ double s_xy[4 * kXYS<D>];      // per block: the conditioning set relative to the point
+    __shared__ double s_q[4 * kXYS<D>];       // per block: the same in the current model's frame
     __shared__ double s_rhs[4 * 2 * NP + NP]; // per block: the last tile row's rows k, z; a zero row
-    __shared__ double s_t[4 * 16];
-    __shared__ double s_dg[4 * 48];           // per block: the diagonal tiles' entries below the diagonal
+    __shared__ double s_t[4 * kTS];
+    __shared__ double s_dg[4 * kDGS];         // per block: the diagonal tiles' entries below the diagonal
     __shared__ double s_sum[4 * kMaxModelsPerLaunch];
     const int lane = threadIdx.x;
     const int blk = (lane >> 2) & 3;          // the point of this lane
     const int x_ = lane >> 4, y_ = lane & 3;  // element [y][x] of every tile
-    double *xy = s_xy + blk * NP * D;
-    double *qs = s_q + blk * NP * D;
+    double *xy = s_xy + blk * kXYS<D>;
+    double *qs = s_q + blk * kXYS<D>;
     double *rhs = s_rhs + blk * 2 * NP;
     double *ks = rhs;                         // row 0: covariances to the point
     double *zs = rhs + NP;                    // row 1: centred values of the conditioning set
     // the lane's row of the last tile row (rows 2 and 3 are zero)
     const double *rrow = y_ < 2 ? rhs + y_ * NP : s_rhs + 8 * NP;
-    double *tt = s_t + blk * 16;
-    double *dgs = s_dg + blk * 48;
+    double *tt = s_t + blk * kTS;
+    double *dgs = s_dg + blk * kDGS;
     const int g = x_ * 4 + y_;                // rank of the lane inside its block
     const int nm = P.n_models;
     for (int p = g; p < nm; p += 16) s_sum[blk * kMaxModelsPerLaunch + p] = 0.0;
@@ -496,20 +506,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     using namespace vm;
     static_assert(M == 0 || M == 1, "simple or ordinary kriging");
     constexpr int NP = 32;
-    __shared__ double s_xy[4 * NP * D];       // per block: the neighbours in the frame, target at the origin
+    __shared__ double s_xy[4 * kXYS<D>];      // per block: the neighbours in the frame, target at the origin
     __shared__ double s_rhs[4 * 3 * NP + NP]; // per block: the last tile row's rows f, k, z; a zero row
-    __shared__ double s_t[4 * 16];
-    __shared__ double s_dg[4 * 48];
+    __shared__ double s_t[4 * kTS];
+    __shared__ double s_dg[4 * kDGS];
     const int lane = threadIdx.x;
     const int blk = (lane >> 2) & 3;
     const int x_ = lane >> 4, y_ = lane & 3;
     const int g = x_ * 4 + y_;
-    double *xy = s_xy + blk * NP * D;
+    double *xy = s_xy + blk * kXYS<D>;
     double *rhs = s_rhs + blk * 3 * NP;
     double *fs = rhs, *ks = rhs + NP, *vs = rhs + 2 * NP;
     const double *rrow = y_ < 3 ? rhs + y_ * NP : s_rhs + 12 * NP;
-    double *tt = s_t + blk * 16;
-    double *dgs = s_dg + blk * 48;
+    double *tt = s_t + blk * kTS;
+    double *dgs = s_dg + blk * kDGS;
     const DevModel &Mdl = *P.model;
     const double c0 = Mdl.c0;
     const Frame<D, ANISO, KIND> F = make_frame<D, ANISO, KIND>(Mdl.s[0]);
