@@ -222,12 +222,13 @@ __device__ __forceinline__ double cov_h2(const Frame<D, ANISO, KIND> &F, double 
     return F.a0 * exp_neg256(d);
 }
 
-// Empty slots of a system sit far out on the first axis, slot j at
-// kFar (j + 1) relative to the centre: at least 10^60 apart from each other
-// and from every datum, so their covariances with everything evaluate to
-// exactly zero (the exponential underflows; the spherical kind is zero past
-// its range) and no mask is needed on the entries.
-constexpr double kFar = 1e60;
+// Empty slots of a system sit far out on the first axis of the frame, slot
+// j at kFar (j + 1) range units from the centre: at least 10^30 ranges from
+// each other and from every datum, so their covariances with everything
+// evaluate to exactly zero (the exponential underflows; the spherical kind
+// is zero past its range) and no entry needs a mask.  Frame units keep the
+// squared distances finite whatever the range.
+constexpr double kFar = 1e30;
 
 // Covariance of two distinct points from their frame coordinates.  The
 // handles reject duplicate coordinates (kriging.rs:312, vecchia.rs:248),
@@ -375,12 +376,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 #pragma unroll
         for (int d = 0; d < D; d++) tg[d] = P.coords[self * D + d];
         const double zi = P.zc[self];
-        // the conditioning set relative to the point; empty slots far out
+        // the conditioning set relative to the point (empty slots at the point)
         for (int j = g; j < NP; j += 16) {
             const int64_t id = j < cnt_ ? nbl[j] : 0;
 #pragma unroll
-            for (int d = 0; d < D; d++)
-                xy[j * D + d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : (d == 0 ? kFar * (j + 1) : 0.0);
+            for (int d = 0; d < D; d++) xy[j * D + d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : 0.0;
             zs[j] = j < cnt_ ? P.zc[id] : 0.0;
         }
         wave_sync();
@@ -389,13 +389,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             const double c0 = Mdl.c0;
             const Frame<D, ANISO, KIND> F = make_frame<D, ANISO, KIND>(Mdl.s[0]);
             const double thresh = 1e-12 * c0;
+            // the set in this model's frame; empty slots far out
             for (int j = g; j < NP; j += 16) {
                 double c[D], q[D];
 #pragma unroll
                 for (int d = 0; d < D; d++) c[d] = xy[j * D + d];
                 to_frame(F, c, q);
 #pragma unroll
-                for (int d = 0; d < D; d++) qs[j * D + d] = q[d];
+                for (int d = 0; d < D; d++)
+                    qs[j * D + d] = j < cnt_ ? q[d] : (d == 0 ? kFar * (j + 1) : 0.0);
             }
             wave_sync();
             // lane coordinates made opaque per model: the coordinates of the
@@ -549,13 +551,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             bool on = true, qz = true;
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                c[d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : (d == 0 ? kFar * (j + 1) : 0.0);
+                c[d] = j < cnt_ ? P.coords[id * D + d] - tg[d] : 0.0;
                 on = on && c[d] == 0.0;
             }
             to_frame(F, c, q);
 #pragma unroll
             for (int d = 0; d < D; d++) qz = qz && q[d] == 0.0;
             if (qz && !on) q[0] = 0x1p-1074;
+            if (j >= cnt_) {
+#pragma unroll
+                for (int d = 0; d < D; d++) q[d] = d == 0 ? kFar * (j + 1) : 0.0;
+            }
 #pragma unroll
             for (int d = 0; d < D; d++) xy[j * D + d] = q[d];
             vs[j] = j < cnt_ ? (M == 0 ? P.values[id] - P.sk_mean : P.values[id]) : 0.0;
