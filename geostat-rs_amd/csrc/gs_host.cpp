@@ -2643,7 +2643,7 @@ int32_t maxmin_order_gpu(gs_ctx *ctx, int D, const double *coords, int64_t n,
         HIP_TRY(hipStreamSynchronize(s));
         if (st.done) break;
         const double rate = (double)(st.s - prev_s) / (double)std::max<int64_t>(1, st.rounds - prev_rounds);
-        // early rounds append few points whatever the input: judge from 256 rounds on
+        // early rounds append few points whatever the input: test the rate from 256 rounds on
         if (st.rounds >= 256 && rate < kMinRate) break;
         prev_s = st.s;
         prev_rounds = st.rounds;
