@@ -384,6 +384,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             zs[j] = j < cnt_ ? P.zc[id] : 0.0;
         }
         wave_sync();
+        double bvar = 1.0, brr = 0.0;
         for (int p = 0; p < nm; p++) {
             const DevModel &Mdl = P.models[p];
             const double c0 = Mdl.c0;
@@ -467,17 +468,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             // [0][1] = -(L^-1 k).(L^-1 z) (lane x = 1, y = 0)
             const double t00 = __shfl(X[tix(8, 8)], 4 * blk);
             const double t01 = __shfl(X[tix(8, 8)], 16 + 4 * blk);
-            if (cnt_ > 0 && g == 0) {
-                const double mu = -t01;
-                const double var = fmax(c0 + t00, 1e-12);
-                const double rr = zi - mu;
-                if (live_pt) {
-                    s_sum[blk * kMaxModelsPerLaunch + p] += -0.5 * (ln2pi + log(var) + rr * rr / var);
-                    if (!ok) atomicMax(&P.status[p], 3);
-                }
-            } else if (g == 0 && live_pt) {
-                const double rr = zi;
-                s_sum[blk * kMaxModelsPerLaunch + p] += -0.5 * (ln2pi + log(c0) + rr * rr / c0);
+            // conditional variance and residual (mu = -t01; a point without
+            // predecessors has the marginal c0); lane p mod 16 of the block
+            // keeps them, and every 16 models each lane turns its pair into
+            // its model's term: one log and one division per 16 models
+            const double var = cnt_ > 0 ? fmax(c0 + t00, 1e-12) : c0;
+            const double rr = cnt_ > 0 ? zi - (-t01) : zi;
+            const bool mine = g == (p & 15);
+            bvar = mine ? var : bvar;
+            brr = mine ? rr : brr;
+            if (cnt_ > 0 && g == 0 && live_pt && !ok) atomicMax(&P.status[p], 3);
+            if ((p & 15) == 15 || p == nm - 1) {
+                const int pm = (p & ~15) + g;
+                if (live_pt && pm <= p)
+                    s_sum[blk * kMaxModelsPerLaunch + pm] += -0.5 * (ln2pi + log(bvar) + brr * brr / bvar);
             }
         }
         wave_sync();
