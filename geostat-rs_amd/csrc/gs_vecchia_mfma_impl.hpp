@@ -47,10 +47,10 @@ constexpr int kDGS = 50;           // 48 diagonal-tile entries, +16 B
 
 __host__ __device__ constexpr int tix(int i, int j) { return i < 8 ? i * (i + 1) / 2 + j : 36 + j; }
 
-// The value, pinned where it is computed: the selects of the build then
-// stay selects (evaluated in every lane) instead of branches around the
-// evaluation, which would split the code into blocks that all carry the
-// tiles.
+// The value, pinned where it is computed: a later select between it and
+// another value then stays a select (evaluated in every lane) instead of a
+// branch around the evaluation, which would split the code into blocks
+// that all carry the tiles.
 __device__ __forceinline__ double pin(double v) {
     asm volatile("" : "+v"(v));
     return v;
