@@ -58,6 +58,8 @@ VECCHIA32R = [(d, nb) for d in (2, 3) for nb in (16, 32, 64)]
 # Vecchia likelihood on the matrix cores: single lean structure, by
 # (dimension, anisotropy, kind 0..4)
 VMFMA = [(d, an, k) for d in (2, 3) for an in (0, 1) for k in range(5)]
+# Vecchia gradient on the matrix cores: (dimension, kind)
+VGMFMA = [(d, k) for d in (2, 3) for k in range(5)]
 # simple / ordinary kriging on the matrix cores: (dimension, anisotropy, kind, M)
 KMFMA = [(d, an, k, m) for d in (2, 3) for an in (0, 1) for k in range(5) for m in (0, 1)]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
@@ -92,6 +94,14 @@ def _gen_sources() -> list[Path]:
         txt = (f'#include "{CSRC}/gs_vecchia_mfma_impl.hpp"\n'
                f"namespace gs {{\ntemplate hipError_t launch_vecchia_mfma_inst<{d}, "
                f"{'true' if an else 'false'}, {k}>"
+               f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
+    for d, k in VGMFMA:
+        p = GEN / f"vgmfma_d{d}_k{k}.hip"
+        txt = (f'#include "{CSRC}/gs_vecchia_grad_mfma_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_vecchia_grad_mfma_inst<{d}, {k}>"
                f"(const VecchiaParams &, int64_t, hipStream_t);\n}}\n")
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
@@ -138,7 +148,8 @@ def _deps_mtime(src: Path) -> float:
     kernel = ["gs_solve_impl.hpp", "gs_internal.hpp", "gs_device.hpp"]
     if src.parent == GEN:
         names = kernel + (["gs_fp32r_impl.hpp"] if "32r" in src.stem else []) + \
-            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith(("vmfma", "kmfma")) else [])
+            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith(("vmfma", "kmfma", "vgmfma")) else []) + \
+            (["gs_vecchia_grad_mfma_impl.hpp"] if src.stem.startswith("vgmfma") else [])
         hdrs = [CSRC / n for n in names]
     else:
         hdrs = [p for p in CSRC.iterdir() if p.suffix in (".hpp", ".h", ".inc")]

@@ -3181,10 +3181,17 @@ int32_t gs_vecchia_loglik_grad_batch(gs_vplan *p, const double *values, const gs
             vp.partial = p->partial.as<double>();
             vp.status = p->status.as<int32_t>() + b0;
             vp.points_per_block = ppb;
-            hipError_t e = gs::launch_vecchia_grad(p->D, NB, vp, n_blocks, s);
+            // one kind across the batch, 17-32 conditioning slots: the
+            // matrix-core kernel (GS_VECCHIA_MFMA=0 keeps the vector one)
+            const char *mf_env = std::getenv("GS_VECCHIA_MFMA");
+            bool mf_aniso = false;
+            const int mf_kind = (NB == 32 && !(mf_env && mf_env[0] == '0'))
+                ? gs::vecchia_mfma_kind(dm.data() + b0, nm, p->max_count, &mf_aniso) : -1;
+            hipError_t e = mf_kind >= 0 && !mf_aniso ? gs::launch_vecchia_grad_mfma(p->D, mf_kind, vp, n_blocks, s)
+                                                     : gs::launch_vecchia_grad(p->D, NB, vp, n_blocks, s);
             if (e == hipErrorInvalidValue) return fail(GS_ERR_INVALID_PARAMETER, "unsupported on GPU: Vecchia size");
             HIP_TRY(e);
-            g_solve_kernel = "vecchia_grad_kernel";
+            g_solve_kernel = mf_kind >= 0 && !mf_aniso ? "vecchia_grad_mfma_kernel" : "vecchia_grad_kernel";
             HIP_TRY(gs::launch_vecchia_reduce(p->partial.as<double>(), n_blocks, nm * 4,
                                               p->out.as<double>() + (int64_t)b0 * 4, s));
         }

@@ -8,6 +8,8 @@ template <int D, bool ANISO, int KIND>
 hipError_t launch_vecchia_mfma_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
 template <int D, bool ANISO, int KIND, int M>
 hipError_t launch_krige_mfma_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
+template <int D, int KIND>
+hipError_t launch_vecchia_grad_mfma_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
 
 int vecchia_mfma_kind(const DevModel *models, int n_models, int max_count, bool *aniso) {
     if (n_models <= 0 || max_count > 32) return -1;
@@ -80,6 +82,26 @@ hipError_t launch_krige_mfma(int D, bool aniso, int kind, int M, const SolvePara
                               : (M == 1 ? krige_by_aniso<2, 1>(aniso, kind, p, grid_groups, s) : hipErrorInvalidValue);
     if (D == 3) return M == 0 ? krige_by_aniso<3, 0>(aniso, kind, p, grid_groups, s)
                               : (M == 1 ? krige_by_aniso<3, 1>(aniso, kind, p, grid_groups, s) : hipErrorInvalidValue);
+    return hipErrorInvalidValue;
+}
+
+template <int D>
+static hipError_t grad_by_kind(int kind, const VecchiaParams &p, int64_t nb, hipStream_t s) {
+    switch (kind) {
+    case 0: return launch_vecchia_grad_mfma_inst<D, 0>(p, nb, s);
+    case 1: return launch_vecchia_grad_mfma_inst<D, 1>(p, nb, s);
+    case 2: return launch_vecchia_grad_mfma_inst<D, 2>(p, nb, s);
+    case 3: return launch_vecchia_grad_mfma_inst<D, 3>(p, nb, s);
+    case 4: return launch_vecchia_grad_mfma_inst<D, 4>(p, nb, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_vecchia_grad_mfma(int D, int kind, const VecchiaParams &p, int64_t n_blocks,
+                                    hipStream_t s) {
+    if (p.points_per_block % 4 != 0) return hipErrorInvalidValue;
+    if (D == 2) return grad_by_kind<2>(kind, p, n_blocks, s);
+    if (D == 3) return grad_by_kind<3>(kind, p, n_blocks, s);
     return hipErrorInvalidValue;
 }
 

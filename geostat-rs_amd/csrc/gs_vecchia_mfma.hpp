@@ -20,4 +20,10 @@ hipError_t launch_vecchia_mfma(int D, bool aniso, int kind, const VecchiaParams 
 hipError_t launch_krige_mfma(int D, bool aniso, int kind, int M, const SolveParams &p,
                              int64_t grid_groups, hipStream_t s);
 
+// The log-likelihood gradient batch (vecchia_grad_kernel's quantities) on
+// the matrix cores: one isotropic structure of the same kind for every
+// model, conditioning sets of at most 32 points, points_per_block % 4 == 0.
+hipError_t launch_vecchia_grad_mfma(int D, int kind, const VecchiaParams &p, int64_t n_blocks,
+                                    hipStream_t s);
+
 }  // namespace gs

@@ -1,12 +1,14 @@
-"""GPU parity of the matrix-core solves (krige_mfma_kernel, vecchia_mfma_kernel).
+"""GPU parity of the matrix-core solves (krige_mfma_kernel, vecchia_mfma_kernel,
+vecchia_grad_mfma_kernel).
 
 Simple / ordinary kriging with 17-32 neighbours and Vecchia conditioning
 sets of at most 32 points, one lean structure: every case checks that the
 call ran the matrix-core kernel (gs_last_solve_kernel), that it matches the
 oracle (statuses and NaN patterns exact, estimates within 1e-10 relative),
 and that the vector-unit kernel (GS_KRIGE_MFMA=0 / GS_VECCHIA_MFMA=0) gives
-the same statuses and agrees within the same bar.  The cases cover every
-lean kind, 2-D and 3-D, isotropic and anisotropic structures, neighbourhoods
+the same statuses and agrees within the same bar (the likelihood gradient:
+log-likelihood 1e-10, gradient 1e-8 normwise).  The cases cover every lean
+kind, 2-D and 3-D, isotropic and anisotropic structures, neighbourhoods
 that leave slots empty (search radius), targets on data points, and systems
 the singular test rejects.
 """
@@ -144,3 +146,45 @@ def test_vecchia_2d_kinds(oracle, name, kind, aniso):
 def test_vecchia_set_sizes_and_3d(oracle, m):
     vecchia_case(oracle, 2, gh.ModelKind.MATERN25, False, m, seed=43 + m)
     vecchia_case(oracle, 3, gh.ModelKind.EXPONENTIAL, True, m, seed=47 + m)
+
+
+GRAD_TOL = 1e-8      # normwise relative, floor 1 (as the vector kernel's gradient tests)
+
+
+def gradient_case(oracle, D, kind, m, seed):
+    c, _ = workloads.points(seed, 2000, [1.0] * D, recipe=2)
+    p = gh.vecchia_plan(c, m)
+    o, nb, cnt = p.arrays()
+    vals = workloads.vecchia_field(c, o, nb, cnt, m, seed=seed + 1)
+    params = [(0.01, 1.0, 0.05), (0.2, 0.7, 0.12), (0.0, 1.3, 0.03), (0.5, 1e-3, 0.08), (0.05, 2.0, 0.2)]
+    if kind == gh.ModelKind.GAUSSIAN:      # a nugget-free Gaussian model is numerically singular
+        params = [(max(nug, 0.02), s, r) for nug, s, r in params]
+    models = [gh.VariogramModel(nug, [gh.Structure(kind, s, r)]) for nug, s, r in params]
+    ll, g, st = p.loglik_grad_batch(vals, models)
+    assert gh._abi.last_solve_kernel() == "vecchia_grad_mfma_kernel"
+    assert (st == 0).all(), st
+    for i, md in enumerate(models):
+        s0 = md.structures[0]
+        rc, oll, og = oracle.vecchia_loglik_grad_plan(c, vals, s0.kind[0], md.nugget, s0.sill,
+                                                      s0.range, o, nb, cnt)
+        assert rc == 0
+        assert abs(ll[i] - oll) <= TOL64 * abs(oll), (i, ll[i], oll)
+        assert np.max(np.abs(g[i] - og)) <= GRAD_TOL * max(np.max(np.abs(og)), 1.0), (i, g[i], og)
+    with vector_kernels():
+        llv, gv, stv = p.loglik_grad_batch(vals, models)
+        assert gh._abi.last_solve_kernel() == "vecchia_grad_kernel"
+    assert np.array_equal(st, stv)
+    assert np.all(np.abs(ll - llv) <= TOL64 * np.abs(llv))
+    for i in range(len(models)):
+        assert np.max(np.abs(g[i] - gv[i])) <= GRAD_TOL * max(np.max(np.abs(gv[i])), 1.0)
+
+
+@pytest.mark.parametrize("name,kind", LEAN)
+def test_vecchia_gradient_kinds(oracle, name, kind):
+    gradient_case(oracle, 2, kind, 30, 61)
+
+
+@pytest.mark.parametrize("m", [17, 32])
+def test_vecchia_gradient_set_sizes_and_3d(oracle, m):
+    gradient_case(oracle, 2, gh.ModelKind.MATERN25, m, 71 + m)
+    gradient_case(oracle, 3, gh.ModelKind.SPHERICAL, m, 73 + m)
