@@ -62,6 +62,9 @@ VMFMA = [(d, an, k) for d in (2, 3) for an in (0, 1) for k in range(5)]
 VGMFMA = [(d, k) for d in (2, 3) for k in range(5)]
 # simple / ordinary kriging on the matrix cores: (dimension, anisotropy, kind, M)
 KMFMA = [(d, an, k, m) for d in (2, 3) for an in (0, 1) for k in range(5) for m in (0, 1)]
+# 33-64-neighbour kriging on the matrix cores: (dimension, covariance
+# signature, M); 16 = exponential + spherical sharing an anisotropic frame
+K64 = [(3, 16, 4)]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
 # two row blocks per lane
 VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]
@@ -115,6 +118,14 @@ def _gen_sources() -> list[Path]:
         if not p.exists() or p.read_text() != txt:
             p.write_text(txt)
         out.append(p)
+    for d, sig, m in K64:
+        p = GEN / f"k64mfma_d{d}_s{sig}_m{m}.hip"
+        txt = (f'#include "{CSRC}/gs_krige_mfma64_impl.hpp"\n'
+               f"namespace gs {{\ntemplate hipError_t launch_krige_mfma64_inst<{d}, {sig}, {m}>"
+               f"(const SolveParams &, int64_t, hipStream_t);\n}}\n")
+        if not p.exists() or p.read_text() != txt:
+            p.write_text(txt)
+        out.append(p)
     for d, nb, g in VGRAD:
         p = GEN / f"vgrad_d{d}_nb{nb}_g{g}.hip"
         txt = (f'#include "{CSRC}/gs_solve_impl.hpp"\n'
@@ -148,7 +159,8 @@ def _deps_mtime(src: Path) -> float:
     kernel = ["gs_solve_impl.hpp", "gs_internal.hpp", "gs_device.hpp"]
     if src.parent == GEN:
         names = kernel + (["gs_fp32r_impl.hpp"] if "32r" in src.stem else []) + \
-            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith(("vmfma", "kmfma", "vgmfma")) else []) + \
+            (["gs_vecchia_mfma_impl.hpp"] if src.stem.startswith(("vmfma", "kmfma", "vgmfma", "k64mfma")) else []) + \
+            (["gs_krige_mfma64_impl.hpp"] if src.stem.startswith("k64mfma") else []) + \
             (["gs_vecchia_grad_mfma_impl.hpp"] if src.stem.startswith("vgmfma") else [])
         hdrs = [CSRC / n for n in names]
     else:

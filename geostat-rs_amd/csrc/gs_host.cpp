@@ -476,6 +476,7 @@ struct gs_kriging {
     bool lean = true;        // every structure in the fast closed-form class
     int mfma_kind = -1;      // one lean structure: its kind for the matrix-core solve (-1: none)
     bool mfma_aniso = false;
+    int mfma64_sig = -1;     // covariance signature of the 64-slot matrix-core solve (-1: none)
     bool has_tree = false;
     int64_t k_heap = 0;
     int per_oct = 0;
@@ -917,6 +918,7 @@ int32_t kriging_build(gs_ctx *ctx, int32_t dim, const double *coords, const doub
     DevModel dm = resolve_model(*model, dim, ctx->nodes.as<gs::BesselNodes>());
     K->lean = gs::model_is_lean(dm);
     K->mfma_kind = gs::vecchia_mfma_kind(&dm, 1, 32, &K->mfma_aniso);
+    K->mfma64_sig = gs::krige_mfma64_sig(dm, dim);
     HIP_TRY(K->dmodel.ensure(sizeof(DevModel)));
     HIP_TRY(hipMemcpy(K->dmodel.p, &dm, sizeof(DevModel), hipMemcpyHostToDevice));
     if (K->has_tree) {
@@ -1322,8 +1324,19 @@ int32_t run_solves(gs_kriging *K, KWork &W, const double *d_targets, int64_t T,
     // vector-unit kernel, for comparison)
     const char *mf_env = std::getenv("GS_KRIGE_MFMA");
     const bool mfma = K->mfma_kind >= 0 && K->M <= 1 && !ext && !(mf_env && mf_env[0] == '0');
+    // 33-64 neighbours: one system per wavefront on the matrix cores, for
+    // the covariance signatures the library holds a kernel for
+    const bool mfma64 = K->mfma64_sig >= 0 && !ext && !(mf_env && mf_env[0] == '0') &&
+                        gs::has_krige_mfma64(K->D, K->mfma64_sig, K->M);
     auto solve64 = [&](int NB, const gs::SolveParams &pp, bool persistent) -> int32_t {
         const Schedule sc = schedule_for("krige", K->D, NB, K->M, K->lean, ext);
+        if (mfma64 && NB == 64) {
+            // two wavefronts per SIMD are resident; each strides over the systems
+            const int64_t groups = std::min<int64_t>(pp.T, (int64_t)cus * 8);
+            HIP_TRY(gs::launch_krige_mfma64(K->D, K->mfma64_sig, K->M, pp, groups, s));
+            g_solve_kernel = "krige_mfma64_kernel";
+            return GS_OK;
+        }
         if (mfma && NB == 32) {
             int64_t groups = pp.T;
             if (persistent) groups = std::min<int64_t>(pp.T, (int64_t)cus * sc.waves_per_cu * 4);

@@ -10,6 +10,8 @@ template <int D, bool ANISO, int KIND, int M>
 hipError_t launch_krige_mfma_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
 template <int D, int KIND>
 hipError_t launch_vecchia_grad_mfma_inst(const VecchiaParams &p, int64_t n_blocks, hipStream_t s);
+template <int D, int SIG, int M>
+hipError_t launch_krige_mfma64_inst(const SolveParams &p, int64_t grid_groups, hipStream_t s);
 
 int vecchia_mfma_kind(const DevModel *models, int n_models, int max_count, bool *aniso) {
     if (n_models <= 0 || max_count > 32) return -1;
@@ -82,6 +84,46 @@ hipError_t launch_krige_mfma(int D, bool aniso, int kind, int M, const SolvePara
                               : (M == 1 ? krige_by_aniso<2, 1>(aniso, kind, p, grid_groups, s) : hipErrorInvalidValue);
     if (D == 3) return M == 0 ? krige_by_aniso<3, 0>(aniso, kind, p, grid_groups, s)
                               : (M == 1 ? krige_by_aniso<3, 1>(aniso, kind, p, grid_groups, s) : hipErrorInvalidValue);
+    return hipErrorInvalidValue;
+}
+
+static int lean_code(int32_t kind) {
+    switch (kind) {
+    case K_SPH: return 0;
+    case K_MAT15: return 1;
+    case K_MAT25: return 2;
+    case K_EXP: return 3;
+    case K_GAU: return 4;
+    default: return -1;
+    }
+}
+
+int krige_mfma64_sig(const DevModel &m, int D) {
+    if (m.ns == 1) {
+        const int k = lean_code(m.s[0].kind);
+        return k < 0 ? -1 : (m.s[0].aniso ? 8 : 0) + k;
+    }
+    if (m.ns == 2 && m.s[0].kind == K_EXP && m.s[1].kind == K_SPH && m.s[0].aniso == m.s[1].aniso) {
+        if (!m.s[0].aniso) return 17;
+        for (int i = 0; i < D * D; i++)
+            if (m.s[0].m[i] != m.s[1].m[i]) return -1;
+        return 16;
+    }
+    return -1;
+}
+
+// (D, signature, M) of the instantiated kernels (the generated units)
+static const int kK64[][3] = {{3, 16, 4}};
+
+bool has_krige_mfma64(int D, int sig, int M) {
+    for (const auto &e : kK64)
+        if (e[0] == D && e[1] == sig && e[2] == M) return true;
+    return false;
+}
+
+hipError_t launch_krige_mfma64(int D, int sig, int M, const SolveParams &p, int64_t grid_groups,
+                               hipStream_t s) {
+    if (D == 3 && sig == 16 && M == 4) return launch_krige_mfma64_inst<3, 16, 4>(p, grid_groups, s);
     return hipErrorInvalidValue;
 }
 

@@ -20,6 +20,21 @@ hipError_t launch_vecchia_mfma(int D, bool aniso, int kind, const VecchiaParams 
 hipError_t launch_krige_mfma(int D, bool aniso, int kind, int M, const SolveParams &p,
                              int64_t grid_groups, hipStream_t s);
 
+// Covariance signature of a model for the 33-64-neighbour matrix-core
+// kriging kernel: 0-4 one isotropic lean structure of that kind (as
+// vecchia_mfma_kind), 8-12 one anisotropic, 16 / 17 an exponential and a
+// spherical structure (in that order) sharing one anisotropic / isotropic
+// frame; -1 when the model has none.
+int krige_mfma64_sig(const DevModel &m, int D);
+
+// Whether the library holds that kernel for (D, signature, M).
+bool has_krige_mfma64(int D, int sig, int M);
+
+// Simple, ordinary or universal kriging with 33-64 neighbours on the
+// matrix cores, one system per wavefront, grid_groups wavefronts.
+hipError_t launch_krige_mfma64(int D, int sig, int M, const SolveParams &p, int64_t grid_groups,
+                               hipStream_t s);
+
 // The log-likelihood gradient batch (vecchia_grad_kernel's quantities) on
 // the matrix cores: one isotropic structure of the same kind for every
 // model, conditioning sets of at most 32 points, points_per_block % 4 == 0.
