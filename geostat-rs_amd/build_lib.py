@@ -63,8 +63,12 @@ VGMFMA = [(d, k) for d in (2, 3) for k in range(5)]
 # simple / ordinary kriging on the matrix cores: (dimension, anisotropy, kind, M)
 KMFMA = [(d, an, k, m) for d in (2, 3) for an in (0, 1) for k in range(5) for m in (0, 1)]
 # 33-64-neighbour kriging on the matrix cores: (dimension, covariance
-# signature, M); 16 = exponential + spherical sharing an anisotropic frame
-K64 = [(3, 16, 4)]
+# signature, M).  Signatures: 0-4 one isotropic lean structure, 8-12 one
+# anisotropic, 16 / 17 exponential + spherical sharing an anisotropic /
+# isotropic frame; M: simple, ordinary, linear and (2-D) quadratic drift.
+K64_SIGS = (0, 1, 2, 3, 4, 8, 9, 10, 11, 12, 16, 17)
+K64 = [(2, g, m) for g in K64_SIGS for m in (0, 1, 3, 6)] + \
+      [(3, g, m) for g in K64_SIGS for m in (0, 1, 4)]
 # Vecchia gradient: lean single-structure models, conditioning sets <= 32,
 # two row blocks per lane
 VGRAD = [(d, nb, nb // 2) for d in (2, 3) for nb in (16, 32)]

@@ -112,19 +112,52 @@ int krige_mfma64_sig(const DevModel &m, int D) {
     return -1;
 }
 
-// (D, signature, M) of the instantiated kernels (the generated units)
-static const int kK64[][3] = {{3, 16, 4}};
+// The instantiated kernels (one generated unit each): every signature for
+// M in {0, 1, 3, 6} in 2-D and {0, 1, 4} in 3-D (simple, ordinary, linear
+// and, in 2-D, quadratic drift; the border holds at most six drift rows).
+static bool k64_m(int D, int M) {
+    return D == 2 ? (M == 0 || M == 1 || M == 3 || M == 6) : (D == 3 && (M == 0 || M == 1 || M == 4));
+}
 
-bool has_krige_mfma64(int D, int sig, int M) {
-    for (const auto &e : kK64)
-        if (e[0] == D && e[1] == sig && e[2] == M) return true;
-    return false;
+static bool k64_sig(int sig) { return (sig >= 0 && sig <= 4) || (sig >= 8 && sig <= 12) || sig == 16 || sig == 17; }
+
+bool has_krige_mfma64(int D, int sig, int M) { return k64_m(D, M) && k64_sig(sig); }
+
+template <int D, int M>
+static hipError_t k64_by_sig(int sig, const SolveParams &p, int64_t gg, hipStream_t s) {
+    switch (sig) {
+    case 0: return launch_krige_mfma64_inst<D, 0, M>(p, gg, s);
+    case 1: return launch_krige_mfma64_inst<D, 1, M>(p, gg, s);
+    case 2: return launch_krige_mfma64_inst<D, 2, M>(p, gg, s);
+    case 3: return launch_krige_mfma64_inst<D, 3, M>(p, gg, s);
+    case 4: return launch_krige_mfma64_inst<D, 4, M>(p, gg, s);
+    case 8: return launch_krige_mfma64_inst<D, 8, M>(p, gg, s);
+    case 9: return launch_krige_mfma64_inst<D, 9, M>(p, gg, s);
+    case 10: return launch_krige_mfma64_inst<D, 10, M>(p, gg, s);
+    case 11: return launch_krige_mfma64_inst<D, 11, M>(p, gg, s);
+    case 12: return launch_krige_mfma64_inst<D, 12, M>(p, gg, s);
+    case 16: return launch_krige_mfma64_inst<D, 16, M>(p, gg, s);
+    case 17: return launch_krige_mfma64_inst<D, 17, M>(p, gg, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_krige_mfma64(int D, int sig, int M, const SolveParams &p, int64_t grid_groups,
                                hipStream_t s) {
-    if (D == 3 && sig == 16 && M == 4) return launch_krige_mfma64_inst<3, 16, 4>(p, grid_groups, s);
-    return hipErrorInvalidValue;
+    if (!has_krige_mfma64(D, sig, M)) return hipErrorInvalidValue;
+    if (D == 2) {
+        switch (M) {
+        case 0: return k64_by_sig<2, 0>(sig, p, grid_groups, s);
+        case 1: return k64_by_sig<2, 1>(sig, p, grid_groups, s);
+        case 3: return k64_by_sig<2, 3>(sig, p, grid_groups, s);
+        default: return k64_by_sig<2, 6>(sig, p, grid_groups, s);
+        }
+    }
+    switch (M) {
+    case 0: return k64_by_sig<3, 0>(sig, p, grid_groups, s);
+    case 1: return k64_by_sig<3, 1>(sig, p, grid_groups, s);
+    default: return k64_by_sig<3, 4>(sig, p, grid_groups, s);
+    }
 }
 
 template <int D>
