@@ -245,9 +245,9 @@ __device__ __forceinline__ double cov_q(const Frame<D, ANISO, KIND> &F, const do
 }
 
 // Element [y][x] of L^-1 for the diagonal tile in tt (row-major 4 x 4,
-// written by the block): every lane of the block factors the tile and
-// inverts the factor redundantly, then keeps its own element (selects
-// between values every lane holds, no lane-dependent branch).  ok is
+// written by the block): every lane of the block factors the tile
+// redundantly and forms column x of the factor's inverse, then keeps entry
+// y (selects between values every lane holds, no lane-dependent branch).  ok is
 // cleared when a pivot fails the singular test (GE: pivot >= thresh, else
 // pivot > thresh).
 template <bool GE>
@@ -269,23 +269,18 @@ __device__ __forceinline__ double tile_linv(const double *tt, int y, int x, doub
     const double i3 = rsqrt_nr(d3);
     if constexpr (GE) ok = ok && d0 >= thresh && d1 >= thresh && d2 >= thresh && d3 >= thresh;
     else ok = ok && d0 > thresh && d1 > thresh && d2 > thresh && d3 > thresh;
-    // pinned: computed in every lane before the selects, not sunk into
-    // lane-dependent branches
-    const double v10 = pin(-l10 * i0 * i1);
-    const double v21 = pin(-l21 * i1 * i2);
-    const double v20 = pin(-(l20 * i0 + l21 * v10) * i2);
-    const double v32 = pin(-l32 * i2 * i3);
-    const double v31 = pin(-(l31 * i1 + l32 * v21) * i3);
-    const double v30 = pin(-((l30 * i0 + l31 * v10) + l32 * v20) * i3);
-    double dv = y == 1 ? pin(i1) : pin(i0);
-    dv = y == 2 ? pin(i2) : dv;
-    dv = y == 3 ? pin(i3) : dv;
-    double c0s = y == 1 ? v10 : v20;
-    c0s = y == 3 ? v30 : c0s;
-    const double c1s = y == 3 ? v31 : v21;
-    double ov = x == 1 ? c1s : v32;
-    ov = x == 0 ? c0s : ov;
-    return y == x ? dv : (y > x ? ov : 0.0);
+    // column x of L^-1 by forward substitution, L u = e_x: zero above the
+    // diagonal, 1 / L_xx on it, and below it the products of the closed
+    // form (for x = 0: u1 = -l10 i0 i1, u2 = -(l20 i0 + l21 u1) i2, ...).
+    // Pinned: computed in every lane before the selects, not sunk into
+    // lane-dependent branches.
+    const double u0 = x == 0 ? pin(i0) : 0.0;
+    const double u1 = x == 1 ? pin(i1) : pin(-l10 * u0 * i1);
+    const double u2 = x == 2 ? pin(i2) : pin(-(l20 * u0 + l21 * u1) * i2);
+    const double u3 = x == 3 ? pin(i3) : pin(-((l30 * u0 + l31 * u1) + l32 * u2) * i3);
+    double r = y == 1 ? u1 : u0;
+    r = y == 2 ? u2 : r;
+    return y == 3 ? u3 : r;
 }
 
 // Blocked right-looking Cholesky of the 9 x 9 tile system over its eight
