@@ -11,9 +11,9 @@
 // their values (z; minus the mean for simple kriging), zero rows after.
 // The four MFMA blocks own the tile rows cyclically: block b holds rows
 // 4r + b of K (row group r = 0..3, slots kslot(r, j) for j <= 4r + 3; the
-// slots right of a block's diagonal are carried along unused) and the
-// border rows in blocks 0 and 1 (row 16 + b).  One instruction thus works
-// on four tiles of one system, one per block.
+// slots right of a block's diagonal are carried along unused) and border
+// row 16 + (b & 1) at the even (b < 2) or odd (b >= 2) tile columns.  One
+// instruction thus works on four tiles of one system, one per block.
 //
 // A blocked right-looking Cholesky over the 16 tile columns, carried
 // through the border rows, leaves L^-1 [F k z] in them.  Per column q: the
@@ -41,8 +41,8 @@ namespace vm64 {
 using namespace vm;
 
 constexpr int kNP = 64;       // neighbour slots
-constexpr int kBorder = 40;   // slots 40..55: border tile (16 + b, j), blocks 0 and 1
-constexpr int kSlots = 56;
+constexpr int kBorder = 40;   // slots 40..47: border tile (16 + (b & 1), 2m + (b >> 1))
+constexpr int kSlots = 48;
 constexpr int kRowS = 66;     // LDS stride of a 64-entry row (+16 B: rows on different banks)
 constexpr int kColRows = 20;  // published column: 16 tiles of K, 4 border blocks
 
@@ -127,6 +127,18 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
     // largest entry, max(c0, 1) with a drift (the basis stays in [-1, 1])
     const double thresh = 1e-12 * (M >= 1 ? fmax(c0, 1.0) : c0);
     s_row[(M + 2) * kRowS + lane] = 0.0;
+    // the two entries (pi, pj), pi > pj, of a 16 x 16 diagonal super-block
+    // this lane evaluates: pairs lane and lane + 64 of the 120 in row-major
+    // order (the second clamped to the last pair; its stores are skipped)
+    int pi[2], pj[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int pp = min(lane + 64 * h, 119);
+        int i = 1;
+        while ((i + 1) * i / 2 <= pp) i++;
+        pi[h] = i;
+        pj[h] = pp - i * (i - 1) / 2;
+    }
     load_exp256();
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
     for (int64_t sys = blockIdx.x; sys < nsys; sys += gridDim.x) {
@@ -181,14 +193,34 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
         const int y = opaque(y_), x = opaque(x_);
         double X[kSlots];
         // K: tile (4r + blk, j), element [y][x] = C(point 16r + 4 blk + y,
-        // point 4j + x); c0 on the diagonal
+        // point 4j + x).  Left of the diagonal super-block (j < 4r) each lane
+        // evaluates its entries; the super-block's 120 distinct entries are
+        // evaluated packed over the lanes into a 16 x 17 LDS copy (over the
+        // column buffer, idle until the factorization), c0 on its diagonal.
         static_for<0, 4>([&](auto RR) {
             constexpr int r = decltype(RR)::value;
+            double *sd = s_col;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int a = 16 * r + pi[h], b = 16 * r + pj[h];
+                double h2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    const double e = s_q[a * D + d] - s_q[b * D + d];
+                    h2 = fma(e, e, h2);
+                }
+                const double v = cv.cov(h2);
+                if (h == 0 || lane + 64 < 120) {
+                    sd[pi[h] * 17 + pj[h]] = v;
+                    sd[pj[h] * 17 + pi[h]] = v;
+                }
+            }
+            if (lane < 16) sd[lane * 18] = c0;
             const int a = 16 * r + 4 * blk + y;
             double pa[D];
 #pragma unroll
             for (int d = 0; d < D; d++) pa[d] = s_q[a * D + d];
-            static_for<0, 4 * r + 4>([&](auto JJ) {
+            static_for<0, 4 * r>([&](auto JJ) {
                 constexpr int j = decltype(JJ)::value;
                 const int b = 4 * j + x;
                 double h2 = 0.0;
@@ -197,18 +229,24 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
                     const double e = pa[d] - s_q[b * D + d];
                     h2 = fma(e, e, h2);
                 }
-                const double v = cv.cov(h2);
-                X[kslot(r, j)] = pin(a == b ? c0 : v);
+                X[kslot(r, j)] = pin(cv.cov(h2));
                 if constexpr ((j % 2) == 1) __builtin_amdgcn_sched_barrier(0);
             });
+            wave_sync();
+            static_for<0, 4>([&](auto CC) {
+                constexpr int c = decltype(CC)::value;
+                X[kslot(r, 4 * r + c)] = sd[(4 * blk + y) * 17 + 4 * c + x];
+            });
+            wave_sync();   // the next row group rewrites sd
         });
+        // border tile row 16 + rb, tile columns 2m + hb
+        const int rb = blk & 1, hb = blk >> 1;
         {
-            // border rows 4 blk + y of blocks 0 and 1; the zero row elsewhere
-            const int br = 4 * blk + y;
-            const double *rrow = (blk < 2 && br < M + 2) ? s_row + br * kRowS : s_row + (M + 2) * kRowS;
-            static_for<0, 16>([&](auto JJ) {
-                constexpr int j = decltype(JJ)::value;
-                X[kBorder + j] = rrow[4 * j + x];
+            const int br = 4 * rb + y;
+            const double *rrow = br < M + 2 ? s_row + br * kRowS : s_row + (M + 2) * kRowS;
+            static_for<0, 8>([&](auto MM) {
+                constexpr int m = decltype(MM)::value;
+                X[kBorder + m] = rrow[4 * (2 * m + hb) + x];
             });
         }
         bool ok = true;
@@ -223,14 +261,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
                 constexpr int r = decltype(RR)::value;
                 X[kslot(r, q)] = mfma(linv, X[kslot(r, q)], 0.0);
             });
-            X[kBorder + q] = mfma(linv, X[kBorder + q], 0.0);
+            {
+                // the border tile of column q, in the blocks holding that column
+                const double sc = mfma(linv, X[kBorder + (q >> 1)], 0.0);
+                X[kBorder + (q >> 1)] = hb == (q & 1) ? pin(sc) : X[kBorder + (q >> 1)];
+            }
             if constexpr (q < 15) {
                 constexpr int q1 = q + 1, r1 = q1 >> 2;
                 static_for<rq, 4>([&](auto RR) {
                     constexpr int r = decltype(RR)::value;
                     s_col[(4 * r + blk) * 16 + 4 * y + x] = X[kslot(r, q)];
                 });
-                s_col[(16 + blk) * 16 + 4 * y + x] = X[kBorder + q];
+                // rows 16 and 17 from the blocks holding column q (the
+                // others write rows 18 and 19, never read)
+                s_col[(16 + rb + (hb == (q & 1) ? 0 : 2)) * 16 + 4 * y + x] = X[kBorder + (q >> 1)];
                 wave_sync();
                 // the next diagonal tile first, handed on through LDS
                 const double l1 = s_col[q1 * 16 + 4 * y + x];
@@ -245,17 +289,28 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
                         if constexpr (!(j == q1 && r == r1))
                             X[kslot(r, j)] = mfma_nega(l, X[kslot(r, q)], X[kslot(r, j)]);
                     });
-                    X[kBorder + j] = mfma_nega(l, X[kBorder + q], X[kBorder + j]);
+                });
+                // border: tile (16 + rb, 2m + hb) -= L_{16+rb,q} L_{2m+hb,q}^T for
+                // the columns right of q (at 2m = q only the odd-column blocks)
+                const double lb = s_col[(16 + rb) * 16 + 4 * y + x];
+                static_for<(q >> 1), 8>([&](auto MM) {
+                    constexpr int m = decltype(MM)::value;
+                    if constexpr (2 * m + 1 > q) {
+                        const double la = s_col[(2 * m + hb) * 16 + 4 * y + x];
+                        const double nv = mfma_nega(la, lb, X[kBorder + m]);
+                        if constexpr (2 * m > q) X[kBorder + m] = nv;
+                        else X[kBorder + m] = hb == 1 ? pin(nv) : X[kBorder + m];
+                    }
                 });
                 wave_sync();
                 linv = tile_linv<true>(s_t + (q1 & 3) * kTS, y, x, thresh, ok);
             }
         });
         // L^-1 of the border rows, one neighbour per lane
-        if (blk < 2) {
-            static_for<0, 16>([&](auto JJ) {
-                constexpr int j = decltype(JJ)::value;
-                s_y[(4 * blk + y) * kRowS + 4 * j + x] = X[kBorder + j];
+        if (4 * rb + y < M + 2) {
+            static_for<0, 8>([&](auto MM) {
+                constexpr int m = decltype(MM)::value;
+                s_y[(4 * rb + y) * kRowS + 4 * (2 * m + hb) + x] = X[kBorder + m];
             });
         }
         wave_sync();
