@@ -16,6 +16,17 @@ namespace gs {
 
 constexpr int kMaxStructures = 8;
 
+// The chip hands workgroups to its eight XCDs round-robin by workgroup id,
+// and each XCD has its own L2.  xcd_block renumbers workgroup b of nb so
+// that every XCD takes a contiguous run of the new ids (a bijection on
+// [0, nb)): work items in neighbouring positions of a Hilbert-ordered list
+// (neighbouring targets, which share most of their neighbours) are then
+// gathered through the same L2 instead of eight.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t x = b & 7, per = nb >> 3, extra = nb & 7;
+    return x * per + (x < extra ? x : extra) + (b >> 3);
+}
+
 enum Kind : int32_t {
     K_SPH = 0, K_EXP = 1, K_GAU = 2, K_MAT15 = 3, K_MAT25 = 4, K_MAT = 5,
     K_CIR = 6, K_STA = 7, K_HOL = 8, K_WAV = 9, K_POW = 10

@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kWave) void krige_fp32r_kernel(SolveParams P) {
     const DevModel &Mdl = *P.model;
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
 
-    for (int64_t sys = (int64_t)blockIdx.x * GPW + grp; sys < nsys;
+    for (int64_t sys = xcd_block(blockIdx.x, gridDim.x) * GPW + grp; sys < nsys;
          sys += (int64_t)gridDim.x * GPW) {
         const int64_t t = P.list ? (int64_t)P.list[sys] : sys;
         const int cnt = P.count[t];

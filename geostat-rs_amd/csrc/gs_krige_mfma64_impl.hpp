@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
     }
     load_exp256();
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
-    for (int64_t sys = blockIdx.x; sys < nsys; sys += gridDim.x) {
+    for (int64_t sys = xcd_block(blockIdx.x, gridDim.x); sys < nsys; sys += gridDim.x) {
         const int64_t t = P.list ? (int64_t)P.list[sys] : sys;
         const int craw = P.count[t];
         const bool bad = craw < P.min_nb || craw < M || craw > kNP;

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     RowTable *RT = reinterpret_cast<RowTable *>(smem_raw + (size_t)slots * kTile * 8);
     const CellGridView &G = P.G;
 
-    const int64_t pos = (int64_t)blockIdx.x * kTile + lane;
+    const int64_t pos = xcd_block(blockIdx.x, gridDim.x) * kTile + lane;
     const bool live = pos < P.T;
     const int64_t t = live ? (int64_t)P.perm[pos] : -1;
     double st[D], ft[D];

@@ -529,7 +529,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     load_exp256();
     wave_sync();
     const int64_t nsys = P.list ? (int64_t)(*P.list_len) : P.T;
-    for (int64_t sb = (int64_t)blockIdx.x * 4; sb < nsys; sb += (int64_t)gridDim.x * 4) {
+    for (int64_t sb = xcd_block(blockIdx.x, gridDim.x) * 4; sb < nsys; sb += (int64_t)gridDim.x * 4) {
         const int64_t sys = sb + blk;
         const bool live = sys < nsys;
         const int64_t t = P.list ? (int64_t)P.list[live ? sys : sb] : (live ? sys : sb);
