@@ -386,6 +386,10 @@ def main() -> int:
     ap.add_argument("--host-io", action="store_true",
                     help="also time predict_many on host arrays (PCIe-inclusive rate, reported "
                          "as config.host_io_targets_per_s; never the headline value)")
+    ap.add_argument("--serving", choices=["on", "off"], default=None,
+                    help="ok2d_large: also run the serving workload (request loop, p50/p99 "
+                         "latency, config.serving_*); on by default for ok2d_large, so one "
+                         "default run reports both halves of the metric")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times "
                          "(scales rocprof FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -546,6 +550,17 @@ def main() -> int:
         out["config"]["host_io_targets_per_s"] = T * args.steps / (time.perf_counter() - t0)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, targets)
+    serving = args.serving == "on" or (args.serving is None and args.workload == "ok2d_large")
+    if serving:
+        # the metric's latency half: the serving workload's figures beside the
+        # throughput (its CPU baseline only with --workload serving)
+        del k, d_t, d_v, d_var, d_lag, d_st
+        torch.cuda.synchronize()
+        sv = bench_serving(argparse.Namespace(**{**vars(args), "no_cpu_baseline": True}))
+        c = sv["config"]
+        out["config"]["serving"] = {key: c[key] for key in (
+            "requests", "slots", "p50_ms", "p99_ms", "one_slot_p50_ms", "one_slot_p99_ms",
+            "service_mean_ms_one_slot", "kriged_targets", "targets_per_s_wall")}
     print(json.dumps(out), flush=True)
     return 0
 
